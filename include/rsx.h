@@ -1,0 +1,137 @@
+/*
+ * rsx.h — C ABI of the MI355X erasure-coding engine (librsx.so).
+ *
+ * The entry points are the drop-in boundary for klauspost/reedsolomon's
+ * Encoder / Extensions API (reedsolomon.go:25-165): a thin cgo shim (or any
+ * FFI) binds them one to one.  Semantics, argument meaning, validation order
+ * and error outcomes follow the Go methods cited on each declaration.
+ *
+ * Shards are passed Go-style: an array of pointers plus an array of lengths,
+ * where length 0 marks an absent shard (a nil or zero-length slice).
+ * All shards of one call live either in device memory (HBM; the call is then
+ * stream-ordered on the codec's stream and returns without waiting) or in
+ * host memory (staged through HBM; the call returns when the results are in
+ * host memory).  Mixing the two in one call is rejected.
+ *
+ * The library never allocates caller memory.  For Reconstruct*, the caller
+ * passes, for every absent shard, a buffer of at least the shard size (the Go
+ * shim applies the reference's "reuse capacity else allocate" rule before the
+ * call); the library writes the shards it reconstructs and reports them in
+ * `filled`.
+ */
+#ifndef RSX_H
+#define RSX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Return codes: 0 or one negative code per reference sentinel error. */
+#define RSX_OK 0
+#define RSX_ERR_INV_SHARD_NUM -1        /* ErrInvShardNum       reedsolomon.go:216 */
+#define RSX_ERR_MAX_SHARD_NUM -2        /* ErrMaxShardNum       reedsolomon.go:221 */
+#define RSX_ERR_NOT_SUPPORTED -3        /* ErrNotSupported      reedsolomon.go:224 */
+#define RSX_ERR_TOO_FEW_SHARDS -4       /* ErrTooFewShards      reedsolomon.go:678 */
+#define RSX_ERR_INVALID_INPUT -5        /* ErrInvalidInput      reedsolomon.go:760 */
+#define RSX_ERR_SHARD_NO_DATA -6        /* ErrShardNoData       reedsolomon.go:1391 */
+#define RSX_ERR_SHARD_SIZE -7           /* ErrShardSize         reedsolomon.go:1395 */
+#define RSX_ERR_INVALID_SHARD_SIZE -8   /* ErrInvalidShardSize  reedsolomon.go:1399 */
+#define RSX_ERR_SHORT_DATA -9           /* ErrShortData         reedsolomon.go:1785 */
+#define RSX_ERR_RECONSTRUCT_REQUIRED -10 /* ErrReconstructRequired reedsolomon.go:1862 */
+#define RSX_ERR_SINGULAR -11            /* errSingular          matrix.go:184 */
+#define RSX_ERR_MATRIX -12              /* custom-matrix shape errors of New, reedsolomon.go:466-474 */
+#define RSX_ERR_DEVICE -20              /* HIP runtime error (no reference counterpart) */
+#define RSX_ERR_NOMEM -21               /* device allocation failed (no reference counterpart) */
+#define RSX_ERR_MIXED_MEMORY -22        /* host and device shards in one call (no counterpart) */
+
+/* Engine selection, options.go:283-305 (WithLeopardGF16 / WithLeopardGF). */
+#define RSX_LEOPARD_AS_NEEDED 0
+#define RSX_LEOPARD_GF16 1
+#define RSX_LEOPARD_GF8 2
+
+/* Matrix selection, options.go:225-276. */
+#define RSX_MATRIX_DEFAULT 0  /* Vandermonde * inv(top), reedsolomon.go:232 */
+#define RSX_MATRIX_CAUCHY 1   /* WithCauchyMatrix */
+#define RSX_MATRIX_PAR1 2     /* WithPAR1Matrix */
+#define RSX_MATRIX_JERASURE 3 /* WithJerasureMatrix */
+
+typedef struct rsx_codec rsx_codec;
+
+/* The options of options.go that change engine, matrix or output bytes, plus
+ * the new device option.  Options that only tune CPU goroutines/caches
+ * (WithMaxGoroutines, WithMinSplitSize, WithAutoGoroutines, WithSSSE3, ...)
+ * have no meaning here and are accepted by the shim as no-ops. */
+typedef struct rsx_options {
+    int leopard;               /* RSX_LEOPARD_*                                   */
+    int matrix;                /* RSX_MATRIX_*                                    */
+    int fast_one_parity;       /* WithFastOneParityMatrix      options.go:260     */
+    int inversion_cache;       /* WithInversionCache (default 1) options.go:152   */
+    const uint8_t *custom_matrix; /* WithCustomMatrix, row-major custom_rows x custom_cols */
+    int custom_rows;
+    int custom_cols;
+    int device;                /* HIP device ordinal: the new WithMI355X option   */
+} rsx_options;
+
+/* Fills *o with the reference defaults (options.go:46-62) on device 0. */
+void rsx_options_default(rsx_options *o);
+
+/* New(dataShards, parityShards, opts...) reedsolomon.go:425-601.
+ * Engine routing as reedsolomon.go:434-443; Leopard GF(2^8) additionally
+ * rejects k + ceilPow2(m) > 256 with RSX_ERR_MAX_SHARD_NUM where the
+ * reference indexes past its tables. */
+int rsx_new(int data_shards, int parity_shards, const rsx_options *opt, rsx_codec **out);
+void rsx_free(rsx_codec *c);
+
+/* Extensions, reedsolomon.go:137-165. */
+int rsx_shard_size_multiple(const rsx_codec *c); /* 1 classic, 64 Leopard */
+int rsx_data_shards(const rsx_codec *c);
+int rsx_parity_shards(const rsx_codec *c);
+int rsx_total_shards(const rsx_codec *c);
+/* 0 classic, RSX_LEOPARD_GF16 or RSX_LEOPARD_GF8: the engine New chose. */
+int rsx_engine(const rsx_codec *c);
+/* Copies the (k+m) x k coding matrix (classic engine only). */
+int rsx_coding_matrix(const rsx_codec *c, uint8_t *out);
+
+/* Encoder.Encode, reedsolomon.go:686 (classic), leopard.go:139, leopard8.go:120. */
+int rsx_encode(rsx_codec *c, uint8_t *const *shards, const size_t *lens, int nshards);
+
+/* Encoder.EncodeIdx, reedsolomon.go:708 (Leopard: ErrNotSupported). */
+int rsx_encode_idx(rsx_codec *c, const uint8_t *data, size_t len, int idx, uint8_t *const *parity,
+                   const size_t *parity_lens, int nparity);
+
+/* Encoder.Verify, reedsolomon.go:857, leopard.go:393, leopard8.go:365.
+ * Always synchronous: *ok is valid on return. */
+int rsx_verify(rsx_codec *c, uint8_t *const *shards, const size_t *lens, int nshards, int *ok);
+
+/* Encoder.Reconstruct / ReconstructData / ReconstructSome,
+ * reedsolomon.go:1446 / 1634 / 1655, leopard.go:378-391, leopard8.go:350-363.
+ * filled[i] (nshards entries) is set to 1 for each shard written. */
+int rsx_reconstruct(rsx_codec *c, uint8_t *const *shards, const size_t *lens, int nshards,
+                    uint8_t *filled);
+int rsx_reconstruct_data(rsx_codec *c, uint8_t *const *shards, const size_t *lens, int nshards,
+                         uint8_t *filled);
+int rsx_reconstruct_some(rsx_codec *c, uint8_t *const *shards, const size_t *lens, int nshards,
+                         const uint8_t *required, int nrequired, uint8_t *filled);
+
+/* Stream control for device-resident calls (no reference counterpart: the Go
+ * methods are synchronous; the shim calls rsx_synchronize before returning
+ * when it hands over device buffers it must read). */
+int rsx_set_stream(rsx_codec *c, void *hip_stream);
+void *rsx_get_stream(rsx_codec *c);
+int rsx_synchronize(rsx_codec *c);
+
+/* Number of erasure patterns held in the device-resident decode cache
+ * (inversion_tree.go analogue). */
+int rsx_decode_cache_size(const rsx_codec *c);
+
+/* Message of the last error on the calling thread. */
+const char *rsx_last_error(void);
+int rsx_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,353 @@
+"""MI355X erasure coding with the klauspost/reedsolomon Encoder API.
+
+This module mirrors the reference's Go API (reedsolomon.go:25-165,
+options.go) over the C ABI of include/rsx.h, so code and tests read like the
+reference's:
+
+    enc = New(10, 4, WithMI355X(0))
+    enc.Encode(shards)            # list of uint8 numpy arrays or torch tensors
+    shards[3] = None
+    enc.Reconstruct(shards)
+    ok = enc.Verify(shards)
+
+Shards are numpy ``uint8`` arrays (host memory, staged through HBM) or
+``torch.uint8`` tensors on the GPU (used in place).  ``None`` or a
+zero-length shard is absent, like a nil slice.  All compute runs in the HIP
+library; importing fails loudly if it has not been built.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librsx.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"{LIB_PATH} is missing: build it with `python -m reedsolomon_amd.build`")
+
+# PyTorch-ROCm ships its own libamdhip64 (soname libamdhip64.so.7).  Loading
+# torch first makes librsx.so bind to that same runtime, so torch tensors and
+# this library share one HIP runtime in the process.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
+lib = ctypes.CDLL(LIB_PATH)
+
+
+class _Options(ctypes.Structure):
+    _fields_ = [
+        ("leopard", ctypes.c_int),
+        ("matrix", ctypes.c_int),
+        ("fast_one_parity", ctypes.c_int),
+        ("inversion_cache", ctypes.c_int),
+        ("custom_matrix", ctypes.c_void_p),
+        ("custom_rows", ctypes.c_int),
+        ("custom_cols", ctypes.c_int),
+        ("device", ctypes.c_int),
+    ]
+
+
+_P = ctypes.c_void_p
+_PP = ctypes.POINTER(ctypes.c_void_p)
+_SZ = ctypes.POINTER(ctypes.c_size_t)
+_U8 = ctypes.POINTER(ctypes.c_uint8)
+
+lib.rsx_options_default.argtypes = [ctypes.POINTER(_Options)]
+lib.rsx_new.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(_Options), ctypes.POINTER(_P)]
+lib.rsx_free.argtypes = [_P]
+for _n in ("rsx_shard_size_multiple", "rsx_data_shards", "rsx_parity_shards", "rsx_total_shards",
+           "rsx_engine", "rsx_decode_cache_size", "rsx_synchronize"):
+    getattr(lib, _n).argtypes = [_P]
+lib.rsx_coding_matrix.argtypes = [_P, _U8]
+lib.rsx_encode.argtypes = [_P, _PP, _SZ, ctypes.c_int]
+lib.rsx_encode_idx.argtypes = [_P, _P, ctypes.c_size_t, ctypes.c_int, _PP, _SZ, ctypes.c_int]
+lib.rsx_verify.argtypes = [_P, _PP, _SZ, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+lib.rsx_reconstruct.argtypes = [_P, _PP, _SZ, ctypes.c_int, _U8]
+lib.rsx_reconstruct_data.argtypes = [_P, _PP, _SZ, ctypes.c_int, _U8]
+lib.rsx_reconstruct_some.argtypes = [_P, _PP, _SZ, ctypes.c_int, _U8, ctypes.c_int, _U8]
+lib.rsx_set_stream.argtypes = [_P, _P]
+lib.rsx_get_stream.argtypes = [_P]
+lib.rsx_get_stream.restype = _P
+lib.rsx_last_error.restype = ctypes.c_char_p
+lib.rsx_device_count.restype = ctypes.c_int
+
+
+# --- errors: one class per reference sentinel (reedsolomon.go:216-1862) ---
+class RSError(Exception):
+    code = 0
+
+
+def _err(name: str, code: int, doc: str):
+    cls = type(name, (RSError,), {"code": code, "__doc__": doc})
+    globals()[name] = cls
+    return cls
+
+
+_ERRORS = {
+    -1: _err("ErrInvShardNum", -1, "cannot create Encoder with less than one data shard or less than zero parity shards"),
+    -2: _err("ErrMaxShardNum", -2, "cannot create Encoder with more than 256 data+parity shards"),
+    -3: _err("ErrNotSupported", -3, "operation not supported"),
+    -4: _err("ErrTooFewShards", -4, "too few shards given"),
+    -5: _err("ErrInvalidInput", -5, "invalid input"),
+    -6: _err("ErrShardNoData", -6, "no shard data"),
+    -7: _err("ErrShardSize", -7, "shard sizes do not match"),
+    -8: _err("ErrInvalidShardSize", -8, "invalid shard size"),
+    -9: _err("ErrShortData", -9, "not enough data to fill the number of requested shards"),
+    -10: _err("ErrReconstructRequired", -10, "reconstruction required"),
+    -11: _err("ErrSingular", -11, "matrix is singular"),
+    -12: _err("ErrMatrix", -12, "invalid custom coding matrix"),
+    -20: _err("ErrDevice", -20, "HIP device error"),
+    -21: _err("ErrNoMem", -21, "device allocation failed"),
+    -22: _err("ErrMixedMemory", -22, "host and device shards mixed in one call"),
+}
+
+
+def _check(code: int) -> None:
+    if code != 0:
+        cls = _ERRORS.get(code, RSError)
+        raise cls(f"{cls.__name__}: {lib.rsx_last_error().decode(errors='replace')}")
+
+
+# --- options (options.go) ---
+def WithLeopardGF16(enabled: bool = True):
+    def f(o):
+        o.leopard = 1 if enabled else 0
+    return f
+
+
+def WithLeopardGF(enabled: bool = True):
+    def f(o):
+        o.leopard = 2 if enabled else 0
+    return f
+
+
+def WithCauchyMatrix():
+    def f(o):
+        o.matrix = 1
+    return f
+
+
+def WithPAR1Matrix():
+    def f(o):
+        o.matrix = 2
+    return f
+
+
+def WithJerasureMatrix():
+    def f(o):
+        o.matrix = 3
+    return f
+
+
+def WithFastOneParityMatrix():
+    def f(o):
+        o.fast_one_parity = 1
+    return f
+
+
+def WithInversionCache(enabled: bool = True):
+    def f(o):
+        o.inversion_cache = int(enabled)
+    return f
+
+
+def WithCustomMatrix(matrix):
+    m = np.ascontiguousarray(np.asarray(matrix, dtype=np.uint8))
+
+    def f(o):
+        o.custom_matrix = m.ctypes.data
+        o.custom_rows, o.custom_cols = m.shape
+        o._keep = m
+    return f
+
+
+def WithMI355X(device: int = 0):
+    """The new option: run the codec on HIP device `device`."""
+    def f(o):
+        o.device = device
+    return f
+
+
+def _noop(*_a, **_k):
+    return lambda o: None
+
+
+# CPU-tuning options of the reference have no meaning on the GPU.
+WithMaxGoroutines = WithMinSplitSize = WithAutoGoroutines = _noop
+WithSSSE3 = WithAVX2 = WithSSE2 = WithAVX512 = WithGFNI = WithAVXGFNI = _noop
+WithConcurrentStreams = WithConcurrentStreamReads = WithConcurrentStreamWrites = _noop
+WithStreamBlockSize = WithWorkAllocator = _noop
+
+
+def device_count() -> int:
+    return lib.rsx_device_count()
+
+
+# --- shard marshalling ---
+def _is_torch(x) -> bool:
+    return type(x).__module__.startswith("torch")
+
+
+def _shard_info(x):
+    """(address, len, is_device) of one shard; None/empty -> (0, 0, None)."""
+    if x is None:
+        return 0, 0, None
+    if _is_torch(x):
+        if x.numel() == 0:
+            return 0, 0, None
+        assert x.dtype.itemsize == 1 and x.is_contiguous(), "shards must be contiguous uint8"
+        return x.data_ptr(), x.numel(), bool(x.is_cuda)
+    a = x
+    if len(a) == 0:
+        return 0, 0, None
+    assert a.dtype == np.uint8 and a.flags["C_CONTIGUOUS"], "shards must be contiguous uint8"
+    return a.ctypes.data, len(a), False
+
+
+class _Marshal:
+    def __init__(self, shards):
+        n = len(shards)
+        self.n = n
+        self.ptrs = (ctypes.c_void_p * max(n, 1))()
+        self.lens = (ctypes.c_size_t * max(n, 1))()
+        self.device = None
+        for i, s in enumerate(shards):
+            p, ln, dev = _shard_info(s)
+            self.ptrs[i] = p or None
+            self.lens[i] = ln
+            if dev is not None and self.device is None:
+                self.device = dev
+
+
+class Encoder:
+    """reedsolomon.Encoder + Extensions over the HIP library."""
+
+    def __init__(self, handle, k, m):
+        self._h = handle
+        self._k, self._m = k, m
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib.rsx_free(h)
+            self._h = None
+
+    # Extensions (reedsolomon.go:137-165)
+    def ShardSizeMultiple(self) -> int:
+        return lib.rsx_shard_size_multiple(self._h)
+
+    def DataShards(self) -> int:
+        return self._k
+
+    def ParityShards(self) -> int:
+        return self._m
+
+    def TotalShards(self) -> int:
+        return self._k + self._m
+
+    def Engine(self) -> int:
+        return lib.rsx_engine(self._h)
+
+    def CodingMatrix(self) -> np.ndarray:
+        out = np.zeros((self._k + self._m, self._k), np.uint8)
+        _check(lib.rsx_coding_matrix(self._h, out.ctypes.data_as(_U8)))
+        return out
+
+    def AllocAligned(self, each: int):
+        return [np.zeros(each, np.uint8) for _ in range(self._k + self._m)]
+
+    def Synchronize(self) -> None:
+        _check(lib.rsx_synchronize(self._h))
+
+    def DecodeCacheSize(self) -> int:
+        return lib.rsx_decode_cache_size(self._h)
+
+    @property
+    def handle(self):
+        return self._h
+
+    # Encoder (reedsolomon.go:25-133)
+    def Encode(self, shards) -> None:
+        mm = _Marshal(shards)
+        _check(lib.rsx_encode(self._h, mm.ptrs, mm.lens, mm.n))
+        if mm.device:
+            self.Synchronize()
+
+    def EncodeIdx(self, data_shard, idx: int, parity) -> None:
+        mm = _Marshal(parity)
+        p, ln, dev = _shard_info(data_shard)
+        _check(lib.rsx_encode_idx(self._h, p or None, ln, idx, mm.ptrs, mm.lens, mm.n))
+        if mm.device or dev:
+            self.Synchronize()
+
+    def Verify(self, shards) -> bool:
+        mm = _Marshal(shards)
+        ok = ctypes.c_int(0)
+        _check(lib.rsx_verify(self._h, mm.ptrs, mm.lens, mm.n, ctypes.byref(ok)))
+        return bool(ok.value)
+
+    def _prepare_outputs(self, shards):
+        """Gives every absent shard a buffer of the shard size (the reference's
+        reuse-capacity-else-allocate rule applied before the call)."""
+        size, dev, like = 0, None, None
+        for s in shards:
+            p, ln, d = _shard_info(s)
+            if ln:
+                size, dev, like = ln, d, s
+                break
+        bufs = list(shards)
+        if size == 0:
+            return bufs, size, dev
+        for i, s in enumerate(shards):
+            if _shard_info(s)[1] == 0:
+                if dev:
+                    import torch
+                    bufs[i] = torch.empty(size, dtype=torch.uint8, device=like.device)
+                else:
+                    bufs[i] = np.empty(size, np.uint8)
+        return bufs, size, dev
+
+    def _reconstruct(self, fn, shards, *extra):
+        n = len(shards)
+        bufs, size, dev = self._prepare_outputs(shards)
+        mm = _Marshal(bufs)
+        for i, s in enumerate(shards):  # absent shards keep length 0
+            if _shard_info(s)[1] == 0:
+                mm.lens[i] = 0
+        filled = (ctypes.c_uint8 * max(n, 1))()
+        _check(fn(self._h, mm.ptrs, mm.lens, mm.n, *extra, filled))
+        if dev:
+            self.Synchronize()
+        for i in range(n):
+            if filled[i]:
+                shards[i] = bufs[i]
+
+    def Reconstruct(self, shards) -> None:
+        self._reconstruct(lib.rsx_reconstruct, shards)
+
+    def ReconstructData(self, shards) -> None:
+        self._reconstruct(lib.rsx_reconstruct_data, shards)
+
+    def ReconstructSome(self, shards, required) -> None:
+        req = (ctypes.c_uint8 * max(len(required), 1))(*[1 if r else 0 for r in required])
+        self._reconstruct(lib.rsx_reconstruct_some, shards, req, len(required))
+
+
+def New(data_shards: int, parity_shards: int, *opts) -> Encoder:
+    """reedsolomon.New (reedsolomon.go:425)."""
+    o = _Options()
+    lib.rsx_options_default(ctypes.byref(o))
+    for f in opts:
+        f(o)
+    h = ctypes.c_void_p()
+    _check(lib.rsx_new(data_shards, parity_shards, ctypes.byref(o), ctypes.byref(h)))
+    return Encoder(h.value, data_shards, parity_shards)
+
+
+__all__ = [n for n in dir() if n.startswith(("With", "Err", "New", "Encoder", "RSError"))] + [
+    "device_count", "lib", "LIB_PATH"]

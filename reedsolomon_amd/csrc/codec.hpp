@@ -1,0 +1,79 @@
+// codec.hpp — internal state of an rsx_codec (one Encoder of the reference).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/rsx.h"
+#include "gf256.hpp"
+
+namespace rsx {
+
+// Error helpers: set the thread's last-error message and return the code.
+int fail(int code, const std::string &msg);
+int hip_fail(hipError_t e, const char *what);
+
+#define RSX_HIP(expr)                                  \
+    do {                                               \
+        hipError_t e_ = (expr);                        \
+        if (e_ != hipSuccess) return hip_fail(e_, #expr); \
+    } while (0)
+
+// Device allocation owned by the codec.
+struct DeviceBuffer {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer &) = delete;
+    DeviceBuffer &operator=(const DeviceBuffer &) = delete;
+    ~DeviceBuffer();
+    int reserve(size_t n);  // grows (never shrinks); contents are not kept
+    template <class T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+// One cached reconstruct plan of the classic engine: which shards are read,
+// which are written, and the device-resident coefficient tables of the
+// decode rows (reedsolomon.go:634-673 getDecodeMatrix + 1732-1774).
+struct DecodePlan {
+    std::vector<int> inputs;   // k surviving shard indices (first k present)
+    std::vector<int> outputs;  // shard indices written, in row order
+    DeviceBuffer tables;       // [k][outputs.size()][5]
+};
+
+// Memory class of one call's shards.
+enum class MemKind { Device, Host };
+
+struct LeopardState;  // leopard.cpp
+struct LeopardDeleter {
+    void operator()(LeopardState *s) const;
+};
+
+}  // namespace rsx
+
+struct rsx_codec {
+    int k = 0, m = 0, total = 0;
+    int engine = 0;  // 0 classic, RSX_LEOPARD_GF16, RSX_LEOPARD_GF8
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    bool inversion_cache = true;
+
+    // classic engine
+    rsx::Matrix matrix;              // (k+m) x k
+    rsx::DeviceBuffer parity_tables; // [k][m][5]
+
+    std::mutex mu;  // guards the caches and the staging buffers below
+    std::unordered_map<std::string, std::unique_ptr<rsx::DecodePlan>> decode_cache;
+    rsx::DeviceBuffer staging;  // host-memory calls: shards staged in HBM
+    rsx::DeviceBuffer scratch;  // Verify: recomputed parity
+    rsx::DeviceBuffer flag;     // Verify: mismatch flag
+
+    std::unique_ptr<rsx::LeopardState, rsx::LeopardDeleter> leo;
+};

@@ -1,0 +1,39 @@
+// kernels.hpp — device kernels of the engine and their host launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace rsx {
+
+// Inputs of one GF(2^8) matrix pass (the device form of reedsolomon.go:882
+// codeSomeShards(matrixRows, inputs, outputs, byteCount, clear)).
+constexpr int kMaxIn = 256;   // classic engine: k <= 256
+constexpr int kMaxOut = 8;    // outputs per pass; more outputs => more passes
+
+struct Gf8Pass {
+    const uint8_t *in[kMaxIn];
+    uint8_t *out[kMaxOut];
+    const uint32_t *tables;  // [k][mtot][5] coefficient tables (gf256.hpp)
+    size_t len;              // bytes per shard
+    int k;                   // inputs
+    int mtot;                // outputs described by tables
+    int o0;                  // first output of this pass in tables
+    int nout;                // outputs of this pass (<= kMaxOut)
+    int accumulate;          // 0: out = M*in (clear); 1: out ^= M*in
+};
+
+// Launches the pass on `stream`.  All pointers must be device pointers.
+hipError_t launch_gf8_pass(const Gf8Pass &p, hipStream_t stream);
+
+// Sets n dwords to v (used instead of hipMemsetAsync so the work stays a
+// plain kernel node if a caller captures it in a HIP graph).
+hipError_t launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t stream);
+
+// *flag = 1 if any of the n bytes of a and b differ (Verify).
+hipError_t launch_compare(const uint8_t *a, const uint8_t *b, size_t n, int *flag,
+                          hipStream_t stream);
+
+}  // namespace rsx
