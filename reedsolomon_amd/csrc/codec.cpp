@@ -199,8 +199,9 @@ int rsx_new(int k, int m, const rsx_options *opt, rsx_codec **out) {
         // leopard.go:51-69 / leopard8.go:42-66 newFF16 / newFF8.
         if (k <= 0 || m <= 0) return fail(RSX_ERR_INV_SHARD_NUM, "invalid shard counts");
         if (total > 65536) return fail(RSX_ERR_MAX_SHARD_NUM, "more than 65536 shards");
-        if (engine == RSX_LEOPARD_GF8 && !leopard8_shape_ok(k, m))
-            return fail(RSX_ERR_MAX_SHARD_NUM, "Leopard GF(2^8) needs k + ceilPow2(m) <= 256");
+        if (!leopard_shape_ok(engine == RSX_LEOPARD_GF8 ? 8 : 16, k, m))
+            return fail(RSX_ERR_MAX_SHARD_NUM,
+                        "Leopard needs k + ceilPow2(m) <= field order (the reference indexes past its tables)");
     } else {
         if (total > 256) return fail(RSX_ERR_MAX_SHARD_NUM, "more than 256 shards");
         if (k <= 0 || m < 0) return fail(RSX_ERR_INV_SHARD_NUM, "invalid shard counts");

@@ -33,27 +33,61 @@ __device__ __forceinline__ uint32_t mul4(const Split3 &s, uint32_t t0, uint32_t 
            __builtin_amdgcn_perm(t4, t4, s.g2);
 }
 
-// Tables are read-only for the whole launch: reading them through the
-// constant address space lets the compiler use scalar (SMEM) loads, which
-// keeps the 5 dwords per coefficient in SGPRs, uniform across the wave.
-using ConstU32 = const __attribute__((address_space(4))) uint32_t;
+// XOR of three words in one VALU op (gfx950 v_bitop3_b32, truth table 0x96).
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ uint4 load16(const uint8_t *base, size_t v) {
     return reinterpret_cast<const uint4 *>(base)[v];
 }
 
+// Coefficient tables staged in LDS for the whole block: per (input j,
+// output i) one 16-byte record {c*(0..3), c*(4..7), c*(0..3)<<3, c*(4..7)<<3}
+// read with one ds_read_b128 straight into VGPRs (v_perm_b32 can read only
+// one SGPR, so SGPR-resident tables would cost two v_mov per coefficient),
+// plus the 4-entry table of c*(x & 0xC0) in a second array.
+struct LdsTables {
+    const uint4 *lo;
+    const uint32_t *hi;
+};
+
+// acc[i] ^= c[i][a]*xa ^ c[i][b]*xb for 4 dwords: 6 byte selects folded into
+// the accumulator with 3 three-input XORs.
 template <int M>
-__device__ __forceinline__ void mac16(uint32_t (&acc)[M][4], const uint4 &x, ConstU32 *t) {
-    const Split3 s0 = split(x.x), s1 = split(x.y), s2 = split(x.z), s3 = split(x.w);
+__device__ __forceinline__ void mac_pair(uint32_t (&acc)[M][4], const uint4 &xa, const uint4 &xb,
+                                         const LdsTables &t, int ja, int jb) {
+    const Split3 a[4] = {split(xa.x), split(xa.y), split(xa.z), split(xa.w)};
+    const Split3 b[4] = {split(xb.x), split(xb.y), split(xb.z), split(xb.w)};
 #pragma unroll
     for (int i = 0; i < M; ++i) {
-        const uint32_t t0 = t[i * 5 + 0], t1 = t[i * 5 + 1], t2 = t[i * 5 + 2], t3 = t[i * 5 + 3],
-                       t4 = t[i * 5 + 4];
-        acc[i][0] ^= mul4(s0, t0, t1, t2, t3, t4);
-        acc[i][1] ^= mul4(s1, t0, t1, t2, t3, t4);
-        acc[i][2] ^= mul4(s2, t0, t1, t2, t3, t4);
-        acc[i][3] ^= mul4(s3, t0, t1, t2, t3, t4);
+        const uint4 ta = t.lo[ja * M + i], tb = t.lo[jb * M + i];
+        const uint32_t ha = t.hi[ja * M + i], hb = t.hi[jb * M + i];
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+            uint32_t r = acc[i][d];
+            r = xor3(r, __builtin_amdgcn_perm(ta.y, ta.x, a[d].g0), __builtin_amdgcn_perm(ta.w, ta.z, a[d].g1));
+            r = xor3(r, __builtin_amdgcn_perm(ha, ha, a[d].g2), __builtin_amdgcn_perm(tb.y, tb.x, b[d].g0));
+            acc[i][d] = xor3(r, __builtin_amdgcn_perm(tb.w, tb.z, b[d].g1), __builtin_amdgcn_perm(hb, hb, b[d].g2));
+        }
+    }
+}
+
+template <int M>
+__device__ __forceinline__ void mac_one(uint32_t (&acc)[M][4], const uint4 &xa, const LdsTables &t, int ja) {
+    const Split3 a[4] = {split(xa.x), split(xa.y), split(xa.z), split(xa.w)};
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+        const uint4 ta = t.lo[ja * M + i];
+        const uint32_t ha = t.hi[ja * M + i];
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+            const uint32_t r = xor3(acc[i][d], __builtin_amdgcn_perm(ta.y, ta.x, a[d].g0),
+                                    __builtin_amdgcn_perm(ta.w, ta.z, a[d].g1));
+            acc[i][d] = r ^ __builtin_amdgcn_perm(ha, ha, a[d].g2);
+        }
     }
 }
 
@@ -64,11 +98,20 @@ constexpr int kGroup = 8;
 
 template <int M>
 __global__ __launch_bounds__(kBlock) void gf8_pass_vec16(const Gf8Pass p) {
+    extern __shared__ uint4 lds[];
+    const int k = p.k;
+    uint4 *lo = lds;
+    uint32_t *hi = reinterpret_cast<uint32_t *>(lds + k * M);
+    for (int t = threadIdx.x; t < k * M; t += kBlock) {
+        const uint32_t *src = p.tables + (static_cast<size_t>(t / M) * p.mtot + p.o0 + t % M) * 5;
+        lo[t] = make_uint4(src[0], src[1], src[2], src[3]);
+        hi[t] = src[4];
+    }
+    __syncthreads();
+    const LdsTables tab{lo, hi};
+
     const size_t nvec = p.len >> 4;
     const size_t stride = static_cast<size_t>(gridDim.x) * kBlock;
-    ConstU32 *tables = (ConstU32 *)(reinterpret_cast<uintptr_t>(p.tables)) + static_cast<size_t>(p.o0) * 5;
-    const size_t tstride = static_cast<size_t>(p.mtot) * 5;
-    const int k = p.k;
     for (size_t v = static_cast<size_t>(blockIdx.x) * kBlock + threadIdx.x; v < nvec; v += stride) {
         uint32_t acc[M][4];
 #pragma unroll
@@ -93,8 +136,10 @@ __global__ __launch_bounds__(kBlock) void gf8_pass_vec16(const Gf8Pass p) {
             for (int u = 0; u < kGroup; ++u)
                 if (jn + u < k) nxt[u] = load16(p.in[jn + u], v);
 #pragma unroll
-            for (int u = 0; u < kGroup; ++u)
-                if (j0 + u < k) mac16<M>(acc, cur[u], tables + (j0 + u) * tstride);
+            for (int u = 0; u < kGroup; u += 2) {
+                if (j0 + u + 1 < k) mac_pair<M>(acc, cur[u], cur[u + 1], tab, j0 + u, j0 + u + 1);
+                else if (j0 + u < k) mac_one<M>(acc, cur[u], tab, j0 + u);
+            }
 #pragma unroll
             for (int u = 0; u < kGroup; ++u) cur[u] = nxt[u];
         }
@@ -161,9 +206,10 @@ hipError_t launch_gf8_pass(const Gf8Pass &p, hipStream_t stream) {
         tail_start = nvec << 4;
         if (nvec) {
             const dim3 grid(grid_for(nvec)), block(kBlock);
+            const size_t lds_bytes = static_cast<size_t>(p.k) * p.nout * 20;
             switch (p.nout) {
 #define RSX_CASE(M) \
-    case M: hipLaunchKernelGGL(gf8_pass_vec16<M>, grid, block, 0, stream, p); break;
+    case M: hipLaunchKernelGGL(gf8_pass_vec16<M>, grid, block, lds_bytes, stream, p); break;
                 RSX_CASE(1) RSX_CASE(2) RSX_CASE(3) RSX_CASE(4)
                 RSX_CASE(5) RSX_CASE(6) RSX_CASE(7) RSX_CASE(8)
 #undef RSX_CASE

@@ -1,0 +1,59 @@
+// leo_kernels.hpp — device kernels of the Leopard engines and their launchers.
+//
+// The work array is a block of rows in HBM with a fixed row pitch; each call
+// processes `cols` bytes of every row (a column chunk of the shards).  A
+// GF(2^16) element is split over a 64-byte chunk (32 low bytes, then 32 high
+// bytes, leopard.go:907-925); a GF(2^8) element is one byte.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace rsx {
+
+struct LeoDev {
+    int bits = 0;                     // 16 or 8
+    unsigned modulus = 0;
+    const uint32_t *tables = nullptr; // (modulus+1) multiply records
+    const uint16_t *skew = nullptr;   // fftSkew, `modulus` entries
+};
+
+struct LeoWork {
+    uint8_t *base = nullptr;  // row 0
+    size_t pitch = 0;         // bytes between rows
+    size_t cols = 0;          // bytes processed per row (multiple of 64)
+};
+
+// work[t] = ptrs[t] ? (logs ? ptrs[t]*exp(logs[t]) : ptrs[t]) : 0 for t < nrows,
+// reading ptrs[t] + col_off.  ptrs and logs are device arrays.
+hipError_t leo_load_rows(const LeoDev &f, const uint8_t *const *ptrs, const uint16_t *logs,
+                         size_t col_off, int nrows, const LeoWork &w, hipStream_t s);
+
+// One radix-4 step (two butterfly layers) at distance `dist` over groups
+// r < mtrunc (step 4*dist).  `skew` points at the skew used for iend = r+dist:
+// encoder IFFT passes fftSkew + m-1 + group*m with bias 0 (skewLUT[iend]),
+// decoder IFFT and FFT pass fftSkew with bias -1 (skewLUT[iend-1]).
+hipError_t leo_step4(const LeoDev &f, bool inverse, const LeoWork &w, int dist, int mtrunc,
+                     const uint16_t *skew, int bias, hipStream_t s);
+
+// Final IFFT radix-2 layer: ifftDIT2(work[i], work[i+dist], log_m) for i < dist.
+hipError_t leo_ifft2_last(const LeoDev &f, const LeoWork &w, int dist, uint16_t log_m,
+                          hipStream_t s);
+
+// Final FFT radix-2 layer: fftDIT2(work[r], work[r+1], skew[r]) for even r < mtrunc.
+hipError_t leo_fft2_last(const LeoDev &f, const LeoWork &w, int mtrunc, hipStream_t s);
+
+// dst[t] ^= src[t] for t < nrows (work arrays with the same pitch/cols).
+hipError_t leo_xor_rows(const LeoWork &dst, const LeoWork &src, int nrows, hipStream_t s);
+
+// Formal derivative of leopard.go:619-623, out of place:
+// out[t] = in[t] ^ XOR over bits b with (t & 2^b) == 0 of in[t | 2^b].
+hipError_t leo_derivative(const LeoWork &in, const LeoWork &out, int n, hipStream_t s);
+
+// dst[q] + col_off = logs ? work[rows[q]]*exp(logs[q]) : work[rows[q]] for q < count.
+hipError_t leo_store_rows(const LeoDev &f, const LeoWork &w, const int *rows, const uint16_t *logs,
+                          uint8_t *const *dst, size_t col_off, int count, hipStream_t s);
+
+}  // namespace rsx

@@ -1,26 +1,483 @@
-// leopard.cpp — Leopard engines (not yet on the device path).
+// leopard.cpp — the Leopard GF(2^16) / GF(2^8) engines on the GPU.
+//
+// Encode (leopard.go:169-266, leopard8.go:139-243): per column chunk, the
+// IFFT of each group of m data rows (zero-padded) is XORed into an m-row
+// accumulator, then an FFT truncated to the parity count produces the
+// parity rows.  Reconstruct (leopard.go:421-647, leopard8.go:393-634): rows
+// are scaled by the error-locator logs, IFFT over n rows, formal derivative,
+// FFT truncated to m+k rows, and the erased rows are scaled back.
+//
+// The work arrays live in HBM.  Columns are processed in chunks sized so the
+// work set stays within the GPU's last-level (Infinity) cache.  Error
+// locators are computed on the host once per erasure pattern and kept in
+// HBM, keyed by the full erasure mask and mode.
 #include "leopard.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+
+#include "kernels.hpp"
+#include "leo_kernels.hpp"
+#include "leopard_field.hpp"
 
 namespace rsx {
 
-struct LeopardState {};
+// Field tables resident on one device (shared by every codec on it).
+struct LeoDeviceTables {
+    DeviceBuffer tables, skew;
+    LeoDev dev;
+};
+
+static std::mutex g_leo_mu;
+static std::map<std::pair<int, int>, std::unique_ptr<LeoDeviceTables>> g_leo_tables;
+
+static int device_tables(int device, int bits, hipStream_t stream, const LeoDev **out) {
+    std::lock_guard<std::mutex> lock(g_leo_mu);
+    auto key = std::make_pair(device, bits);
+    auto it = g_leo_tables.find(key);
+    if (it == g_leo_tables.end()) {
+        const LeoField &f = leo_field(bits);
+        auto t = std::make_unique<LeoDeviceTables>();
+        std::vector<uint32_t> mt = f.mul_tables();
+        int err = t->tables.reserve(mt.size() * 4);
+        if (err) return err;
+        err = t->skew.reserve(f.skew.size() * 2 + 64);
+        if (err) return err;
+        RSX_HIP(hipMemcpyAsync(t->tables.ptr, mt.data(), mt.size() * 4, hipMemcpyHostToDevice, stream));
+        RSX_HIP(hipMemcpyAsync(t->skew.ptr, f.skew.data(), f.skew.size() * 2, hipMemcpyHostToDevice, stream));
+        RSX_HIP(hipStreamSynchronize(stream));
+        t->dev.bits = bits;
+        t->dev.modulus = f.modulus;
+        t->dev.tables = t->tables.as<uint32_t>();
+        t->dev.skew = t->skew.as<uint16_t>();
+        it = g_leo_tables.emplace(key, std::move(t)).first;
+    }
+    *out = &it->second->dev;
+    return RSX_OK;
+}
+
+// Error locators of one erasure pattern (host copy + HBM copy).
+struct LeoPattern {
+    std::vector<uint16_t> errlocs;  // n entries
+    int input_count = 0;
+    DeviceBuffer d_errlocs;
+};
+
+struct LeopardState {
+    int bits = 16;
+    const LeoField *field = nullptr;
+    const LeoDev *dev = nullptr;
+    DeviceBuffer work, work2, ptrs, rows, logs, stage, scratch, flag;
+    std::vector<uint8_t *> h_ptrs;
+    std::unordered_map<std::string, std::unique_ptr<LeoPattern>> patterns;
+};
 
 void LeopardDeleter::operator()(LeopardState *s) const { delete s; }
 
-bool leopard8_shape_ok(int k, int m) {
-    int p = 1;
-    while (p < m) p <<= 1;
-    return k + p <= 256;
+bool leopard8_shape_ok(int k, int m) { return leopard_shape_ok(8, k, m); }
+
+// Largest fftSkew index the encoder's grouped IFFT reads (leopard.go:764-873
+// on skewLUT = fftSkew[m-1+g*m:]).
+static long max_encode_skew_index(int k, int p) {
+    const int m = ceil_pow2(p);
+    long mx = -1;
+    for (int g = 0; g * m < k; ++g) {
+        const int mtrunc = std::min(m, k - g * m);
+        const long off = m - 1 + static_cast<long>(g) * m;
+        int dist = 1, dist4 = 4;
+        while (dist4 <= m) {
+            const long r_last = static_cast<long>((mtrunc - 1) / dist4) * dist4;
+            mx = std::max(mx, off + r_last + 3L * dist);
+            dist = dist4;
+            dist4 <<= 2;
+        }
+        if (dist < m) mx = std::max(mx, off + dist);
+    }
+    return mx;
 }
-int leopard_init(rsx_codec *) { return RSX_OK; }
-int leopard_encode(rsx_codec *, uint8_t *const *, const size_t *, int) {
-    return fail(RSX_ERR_NOT_SUPPORTED, "Leopard engine not built yet");
+
+// Shapes whose tables the reference indexes in range: errLocs has `order`
+// entries and row m+k-1 is written, fftSkew has `modulus` entries.
+bool leopard_shape_ok(int bits, int k, int m) {
+    const long order = 1L << bits;
+    if (static_cast<long>(k) + ceil_pow2(m) > order) return false;
+    return max_encode_skew_index(k, m) < order - 1;
 }
-int leopard_verify(rsx_codec *, uint8_t *const *, const size_t *, int, int *) {
-    return fail(RSX_ERR_NOT_SUPPORTED, "Leopard engine not built yet");
+
+int leopard_init(rsx_codec *c) {
+    std::unique_ptr<LeopardState, LeopardDeleter> s(new LeopardState);
+    s->bits = c->engine == RSX_LEOPARD_GF8 ? 8 : 16;
+    s->field = &leo_field(s->bits);
+    int err = device_tables(c->device, s->bits, c->stream, &s->dev);
+    if (err) return err;
+    c->leo = std::move(s);
+    return RSX_OK;
 }
-int leopard_reconstruct(rsx_codec *, uint8_t *const *, const size_t *, int, bool, uint8_t *) {
-    return fail(RSX_ERR_NOT_SUPPORTED, "Leopard engine not built yet");
+
+namespace {
+
+size_t shard_size_of(const size_t *lens, int n) {
+    for (int i = 0; i < n; ++i)
+        if (lens[i]) return lens[i];
+    return 0;
+}
+
+int check(const size_t *lens, int n, bool nilok) {
+    const size_t size = shard_size_of(lens, n);
+    if (size == 0) return fail(RSX_ERR_SHARD_NO_DATA, "no shard data");
+    for (int i = 0; i < n; ++i)
+        if (lens[i] != size && (lens[i] != 0 || !nilok))
+            return fail(RSX_ERR_SHARD_SIZE, "shard sizes do not match");
+    return RSX_OK;
+}
+
+// Column chunk (multiple of 64 bytes) so that `rows` rows stay within budget.
+size_t chunk_for(size_t size, int rows, size_t budget) {
+    size_t c = budget / static_cast<size_t>(rows);
+    c -= c % 64;
+    c = std::max<size_t>(c, 64);
+    return std::min(c, size);
+}
+
+constexpr size_t kEncodeBudget = 64u << 20;
+constexpr size_t kDecodeBudget = 64u << 20;
+
+bool all_aligned(const std::vector<uint8_t *> &p) {
+    uintptr_t bits = 0;
+    for (uint8_t *q : p) bits |= reinterpret_cast<uintptr_t>(q);
+    return (bits & 15) == 0;
+}
+
+// Uploads a pointer table into `buf` (synchronously ordered on the stream;
+// the host copy stays alive in the state until the call has synchronised).
+int upload_ptrs(rsx_codec *c, DeviceBuffer &buf, const std::vector<uint8_t *> &p) {
+    int err = buf.reserve(std::max<size_t>(p.size(), 1) * sizeof(void *));
+    if (err) return err;
+    RSX_HIP(hipMemcpyAsync(buf.ptr, p.data(), p.size() * sizeof(void *), hipMemcpyHostToDevice, c->stream));
+    return RSX_OK;
+}
+
+// IFFT of one encoder group into `w` (m rows), leopard.go:764-873.
+int encode_group(rsx_codec *c, const LeopardState &s, const uint8_t *const *dptrs, size_t col_off,
+                 int mtrunc, int m, const LeoWork &w, int skew_off) {
+    const LeoDev &f = *s.dev;
+    RSX_HIP(leo_load_rows(f, dptrs, nullptr, col_off, m, w, c->stream));
+    const uint16_t *skew = f.skew + skew_off;
+    int dist = 1, dist4 = 4;
+    while (dist4 <= m) {
+        RSX_HIP(leo_step4(f, true, w, dist, mtrunc, skew, 0, c->stream));
+        dist = dist4;
+        dist4 <<= 2;
+    }
+    if (dist < m) RSX_HIP(leo_ifft2_last(f, w, dist, s.field->skew[skew_off + dist], c->stream));
+    return RSX_OK;
+}
+
+// In-place FFT truncated to mtrunc outputs, leopard.go:696-735.
+int fft(rsx_codec *c, const LeopardState &s, const LeoWork &w, int mtrunc, int m) {
+    const LeoDev &f = *s.dev;
+    int dist4 = m, dist = m >> 2;
+    while (dist != 0) {
+        RSX_HIP(leo_step4(f, false, w, dist, mtrunc, f.skew, -1, c->stream));
+        dist4 = dist;
+        dist >>= 2;
+    }
+    if (dist4 == 2) RSX_HIP(leo_fft2_last(f, w, mtrunc, c->stream));
+    return RSX_OK;
+}
+
+// Encode with device pointers (16-byte aligned) for data (k) and parity (p).
+int encode_device(rsx_codec *c, const std::vector<uint8_t *> &data, const std::vector<uint8_t *> &parity,
+                  size_t size) {
+    LeopardState &s = *c->leo;
+    const int k = c->k, p = c->m;
+    const int m = ceil_pow2(p);
+    const int mtrunc = std::min(k, m);
+    const int last_count = k % m;
+    const int groups = (k + m - 1) / m;
+    // Group-major pointer table: group g occupies entries [g*m, g*m+m), null past its rows.
+    std::vector<uint8_t *> tab(static_cast<size_t>(groups) * m + p, nullptr);
+    for (int j = 0; j < k; ++j) tab[j] = data[j];
+    for (int i = 0; i < p; ++i) tab[static_cast<size_t>(groups) * m + i] = parity[i];
+    s.h_ptrs = tab;
+    int err = upload_ptrs(c, s.ptrs, s.h_ptrs);
+    if (err) return err;
+    std::vector<int> rows(p);
+    for (int i = 0; i < p; ++i) rows[i] = i;
+    if ((err = s.rows.reserve(p * sizeof(int)))) return err;
+    RSX_HIP(hipMemcpyAsync(s.rows.ptr, rows.data(), p * sizeof(int), hipMemcpyHostToDevice, c->stream));
+
+    const size_t chunk = chunk_for(size, 2 * m, kEncodeBudget);
+    if ((err = s.work.reserve(chunk * 2 * m))) return err;
+    const uint8_t *const *dptr = reinterpret_cast<const uint8_t *const *>(s.ptrs.ptr);
+    uint8_t *const *pptr = reinterpret_cast<uint8_t *const *>(s.ptrs.as<uint8_t *>() + static_cast<size_t>(groups) * m);
+    for (size_t off = 0; off < size; off += chunk) {
+        const size_t cols = std::min(chunk, size - off);
+        LeoWork acc{s.work.as<uint8_t>(), chunk, cols};
+        LeoWork tmp{s.work.as<uint8_t>() + chunk * m, chunk, cols};
+        if ((err = encode_group(c, s, dptr, off, mtrunc, m, acc, m - 1))) return err;
+        if (m < k) {
+            int g = 1;
+            for (int i = m; i + m <= k; i += m, ++g) {
+                if ((err = encode_group(c, s, dptr + static_cast<size_t>(g) * m, off, m, m, tmp, m - 1 + g * m)))
+                    return err;
+                RSX_HIP(leo_xor_rows(acc, tmp, m, c->stream));
+            }
+            if (last_count) {
+                if ((err = encode_group(c, s, dptr + static_cast<size_t>(g) * m, off, last_count, m, tmp,
+                                        m - 1 + g * m)))
+                    return err;
+                RSX_HIP(leo_xor_rows(acc, tmp, m, c->stream));
+            }
+        }
+        if ((err = fft(c, s, acc, p, m))) return err;
+        RSX_HIP(leo_store_rows(*s.dev, acc, s.rows.as<int>(), nullptr, pptr, off, p, c->stream));
+    }
+    return RSX_OK;
+}
+
+int pattern_for(rsx_codec *c, const uint8_t *erased, bool recover_all, LeoPattern **out) {
+    LeopardState &s = *c->leo;
+    const int k = c->k, p = c->m, total = c->total;
+    std::string key(1 + (total + 7) / 8, '\0');
+    key[0] = recover_all ? 1 : 0;
+    for (int i = 0; i < total; ++i)
+        if (erased[i]) key[1 + i / 8] |= static_cast<char>(1 << (i % 8));
+    auto it = s.patterns.find(key);
+    if (it != s.patterns.end()) {
+        *out = it->second.get();
+        return RSX_OK;
+    }
+    if (s.patterns.size() >= 4096) {
+        RSX_HIP(hipStreamSynchronize(c->stream));
+        s.patterns.clear();
+    }
+    auto pat = std::make_unique<LeoPattern>();
+    pat->errlocs = leo_error_locators(*s.field, k, p, erased, &pat->input_count);
+    int err = pat->d_errlocs.reserve(pat->errlocs.size() * 2);
+    if (err) return err;
+    RSX_HIP(hipMemcpyAsync(pat->d_errlocs.ptr, pat->errlocs.data(), pat->errlocs.size() * 2,
+                           hipMemcpyHostToDevice, c->stream));
+    RSX_HIP(hipStreamSynchronize(c->stream));
+    *out = pat.get();
+    s.patterns.emplace(std::move(key), std::move(pat));
+    return RSX_OK;
+}
+
+// Reconstruct with aligned device pointers; shards[i] is the buffer of shard i
+// (input when present, output when erased and requested).
+int reconstruct_device(rsx_codec *c, const std::vector<uint8_t *> &shards, const uint8_t *erased,
+                       bool recover_all, size_t size, uint8_t *filled) {
+    LeopardState &s = *c->leo;
+    const int k = c->k, p = c->m, total = c->total;
+    const int m = ceil_pow2(p), n = ceil_pow2(m + k);
+    LeoPattern *pat = nullptr;
+    int err = pattern_for(c, erased, recover_all, &pat);
+    if (err) return err;
+    // Row table for the scaled load: parity rows, zero rows, data rows, zero rows.
+    std::vector<uint8_t *> tab(static_cast<size_t>(n), nullptr);
+    for (int i = 0; i < p; ++i) tab[i] = erased[k + i] ? nullptr : shards[k + i];
+    for (int i = 0; i < k; ++i) tab[m + i] = erased[i] ? nullptr : shards[i];
+    const int end = recover_all ? total : k;
+    std::vector<int> rows;
+    std::vector<uint16_t> logs;
+    for (int i = 0; i < end; ++i) {
+        if (!erased[i]) continue;
+        const int pos = i >= k ? i - k : i + m;
+        rows.push_back(pos);
+        logs.push_back(static_cast<uint16_t>(s.field->modulus - pat->errlocs[pos]));
+        tab.push_back(shards[i]);
+        filled[i] = 1;
+    }
+    const int nout = static_cast<int>(rows.size());
+    s.h_ptrs = tab;
+    if ((err = upload_ptrs(c, s.ptrs, s.h_ptrs))) return err;
+    if ((err = s.rows.reserve(std::max(nout, 1) * sizeof(int)))) return err;
+    if ((err = s.logs.reserve(std::max(nout, 1) * sizeof(uint16_t)))) return err;
+    if (nout) {
+        RSX_HIP(hipMemcpyAsync(s.rows.ptr, rows.data(), nout * sizeof(int), hipMemcpyHostToDevice, c->stream));
+        RSX_HIP(hipMemcpyAsync(s.logs.ptr, logs.data(), nout * sizeof(uint16_t), hipMemcpyHostToDevice, c->stream));
+    }
+    const int ifft_trunc = s.bits == 16 ? pat->input_count : m + k;
+    const size_t chunk = chunk_for(size, 2 * n, kDecodeBudget);
+    if ((err = s.work.reserve(chunk * n))) return err;
+    if ((err = s.work2.reserve(chunk * n))) return err;
+    const LeoDev &f = *s.dev;
+    const uint8_t *const *lptr = reinterpret_cast<const uint8_t *const *>(s.ptrs.ptr);
+    uint8_t *const *optr = reinterpret_cast<uint8_t *const *>(s.ptrs.as<uint8_t *>() + n);
+    for (size_t off = 0; off < size; off += chunk) {
+        const size_t cols = std::min(chunk, size - off);
+        LeoWork w{s.work.as<uint8_t>(), chunk, cols};
+        LeoWork w2{s.work2.as<uint8_t>(), chunk, cols};
+        RSX_HIP(leo_load_rows(f, lptr, pat->d_errlocs.as<uint16_t>(), off, n, w, c->stream));
+        // leopard.go:650-693 ifftDITDecoder
+        int dist = 1, dist4 = 4;
+        while (dist4 <= n) {
+            RSX_HIP(leo_step4(f, true, w, dist, ifft_trunc, f.skew, -1, c->stream));
+            dist = dist4;
+            dist4 <<= 2;
+        }
+        if (dist < n) RSX_HIP(leo_ifft2_last(f, w, dist, s.field->skew[dist - 1], c->stream));
+        RSX_HIP(leo_derivative(w, w2, n, c->stream));
+        if ((err = fft(c, s, w2, m + k, n))) return err;
+        if (nout)
+            RSX_HIP(leo_store_rows(f, w2, s.rows.as<int>(), s.logs.as<uint16_t>(), optr, off, nout, c->stream));
+    }
+    return RSX_OK;
+}
+
+// Resolves the device addresses of a call: device shards in place when
+// aligned, otherwise staged in HBM.  `load[i]` = shard i must be copied in,
+// `save[i]` = shard i must be copied back after the work.
+struct Resolved {
+    std::vector<uint8_t *> dev;
+    bool staged = false;
+    MemKind kind = MemKind::Device;
+};
+
+MemKind classify_ptr(const void *p) {
+    hipPointerAttribute_t attr;
+    if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return MemKind::Host;
+    }
+    return (attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged ||
+            attr.type == hipMemoryTypeUnified)
+               ? MemKind::Device
+               : MemKind::Host;
+}
+
+int resolve(rsx_codec *c, uint8_t *const *shards, int total, size_t size, const std::vector<bool> &use,
+            const std::vector<bool> &load, Resolved &r) {
+    LeopardState &s = *c->leo;
+    bool have = false;
+    for (int i = 0; i < total; ++i) {
+        if (!use[i] || !shards[i]) continue;
+        MemKind kk = classify_ptr(shards[i]);
+        if (!have) {
+            r.kind = kk;
+            have = true;
+        } else if (kk != r.kind) {
+            return fail(RSX_ERR_MIXED_MEMORY, "host and device shards mixed in one call");
+        }
+    }
+    r.dev.assign(total, nullptr);
+    for (int i = 0; i < total; ++i)
+        if (use[i]) r.dev[i] = shards[i];
+    std::vector<uint8_t *> used;
+    for (int i = 0; i < total; ++i)
+        if (use[i]) used.push_back(shards[i]);
+    r.staged = r.kind == MemKind::Host || !all_aligned(used);
+    if (!r.staged) return RSX_OK;
+    int err = s.stage.reserve(size * total);
+    if (err) return err;
+    const hipMemcpyKind dir = r.kind == MemKind::Host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice;
+    for (int i = 0; i < total; ++i) {
+        if (!use[i]) continue;
+        r.dev[i] = s.stage.as<uint8_t>() + size * i;
+        if (load[i]) RSX_HIP(hipMemcpyAsync(r.dev[i], shards[i], size, dir, c->stream));
+    }
+    return RSX_OK;
+}
+
+int unstage(rsx_codec *c, uint8_t *const *shards, const Resolved &r, const std::vector<bool> &save,
+            size_t size) {
+    if (!r.staged) return RSX_OK;
+    const hipMemcpyKind dir = r.kind == MemKind::Host ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+    for (size_t i = 0; i < save.size(); ++i)
+        if (save[i]) RSX_HIP(hipMemcpyAsync(shards[i], r.dev[i], size, dir, c->stream));
+    return RSX_OK;
+}
+
+}  // namespace
+
+// leopard.go:139-150 Encode / leopard8.go:120-130.
+int leopard_encode(rsx_codec *c, uint8_t *const *shards, const size_t *lens, int nshards) {
+    if (nshards != c->total) return fail(RSX_ERR_TOO_FEW_SHARDS, "too few shards given");
+    int err = check(lens, nshards, false);
+    if (err) return err;
+    const size_t size = shard_size_of(lens, nshards);
+    if (size % 64) return fail(RSX_ERR_INVALID_SHARD_SIZE, "shard size must be a multiple of 64");
+    (void)hipSetDevice(c->device);
+    std::lock_guard<std::mutex> lock(c->mu);
+    const int total = c->total, k = c->k;
+    std::vector<bool> use(total, true), load(total, false), save(total, false);
+    for (int i = 0; i < total; ++i) (i < k ? load : save)[i] = true;
+    Resolved r;
+    if ((err = resolve(c, shards, total, size, use, load, r))) return err;
+    std::vector<uint8_t *> data(r.dev.begin(), r.dev.begin() + k), parity(r.dev.begin() + k, r.dev.end());
+    if ((err = encode_device(c, data, parity, size))) return err;
+    if ((err = unstage(c, shards, r, save, size))) return err;
+    RSX_HIP(hipStreamSynchronize(c->stream));
+    return RSX_OK;
+}
+
+// leopard.go:393-419 Verify: re-encode into temporaries and compare.
+int leopard_verify(rsx_codec *c, uint8_t *const *shards, const size_t *lens, int nshards, int *ok) {
+    *ok = 0;
+    if (nshards != c->total) return fail(RSX_ERR_TOO_FEW_SHARDS, "too few shards given");
+    int err = check(lens, nshards, false);
+    if (err) return err;
+    const size_t size = shard_size_of(lens, nshards);
+    if (size % 64) return fail(RSX_ERR_INVALID_SHARD_SIZE, "shard size must be a multiple of 64");
+    (void)hipSetDevice(c->device);
+    std::lock_guard<std::mutex> lock(c->mu);
+    LeopardState &s = *c->leo;
+    const int total = c->total, k = c->k, p = c->m;
+    std::vector<bool> use(total, true), load(total, true), save(total, false);
+    Resolved r;
+    if ((err = resolve(c, shards, total, size, use, load, r))) return err;
+    if ((err = s.scratch.reserve(size * p))) return err;
+    if ((err = s.flag.reserve(sizeof(int)))) return err;
+    std::vector<uint8_t *> data(r.dev.begin(), r.dev.begin() + k), tmp(p);
+    for (int i = 0; i < p; ++i) tmp[i] = s.scratch.as<uint8_t>() + size * i;
+    if ((err = encode_device(c, data, tmp, size))) return err;
+    RSX_HIP(launch_fill_u32(s.flag.as<uint32_t>(), 1, 0u, c->stream));
+    for (int i = 0; i < p; ++i) RSX_HIP(launch_compare(tmp[i], r.dev[k + i], size, s.flag.as<int>(), c->stream));
+    int mismatch = 0;
+    RSX_HIP(hipMemcpyAsync(&mismatch, s.flag.ptr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    RSX_HIP(hipStreamSynchronize(c->stream));
+    *ok = mismatch == 0;
+    return RSX_OK;
+}
+
+// leopard.go:421-580 / leopard8.go:393-634 reconstruct.
+int leopard_reconstruct(rsx_codec *c, uint8_t *const *shards, const size_t *lens, int nshards,
+                        bool recover_all, uint8_t *filled) {
+    const int k = c->k, total = c->total;
+    if (nshards != total) return fail(RSX_ERR_TOO_FEW_SHARDS, "too few shards given");
+    int err = check(lens, nshards, true);
+    if (err) return err;
+    int present = 0, data_present = 0;
+    for (int i = 0; i < total; ++i)
+        if (lens[i]) {
+            ++present;
+            if (i < k) ++data_present;
+        }
+    if (present == total || (!recover_all && data_present == k)) return RSX_OK;
+    if (present < k) return fail(RSX_ERR_TOO_FEW_SHARDS, "too few shards given");
+    const size_t size = shard_size_of(lens, nshards);
+    if (size % 64) return fail(RSX_ERR_INVALID_SHARD_SIZE, "shard size must be a multiple of 64");
+    (void)hipSetDevice(c->device);
+    std::lock_guard<std::mutex> lock(c->mu);
+    std::vector<uint8_t> erased(total);
+    const int end = recover_all ? total : k;
+    std::vector<bool> use(total, false), load(total, false), save(total, false);
+    for (int i = 0; i < total; ++i) {
+        erased[i] = lens[i] == 0;
+        if (!erased[i]) use[i] = load[i] = true;
+        else if (i < end) {
+            if (!shards[i]) return fail(RSX_ERR_INVALID_INPUT, "no output buffer for an absent shard");
+            use[i] = save[i] = true;
+        }
+    }
+    Resolved r;
+    if ((err = resolve(c, shards, total, size, use, load, r))) return err;
+    if ((err = reconstruct_device(c, r.dev, erased.data(), recover_all, size, filled))) return err;
+    if ((err = unstage(c, shards, r, save, size))) return err;
+    RSX_HIP(hipStreamSynchronize(c->stream));
+    return RSX_OK;
 }
 
 }  // namespace rsx

@@ -10,6 +10,8 @@ namespace rsx {
 
 // k + ceilPow2(m) <= 256: the shapes the reference's GF(2^8) tables cover.
 bool leopard8_shape_ok(int k, int m);
+// Shapes inside the tables of the reference (see leopard.cpp).
+bool leopard_shape_ok(int bits, int k, int m);
 int leopard_init(rsx_codec *c);
 int leopard_encode(rsx_codec *c, uint8_t *const *shards, const size_t *lens, int nshards);
 int leopard_verify(rsx_codec *c, uint8_t *const *shards, const size_t *lens, int nshards, int *ok);

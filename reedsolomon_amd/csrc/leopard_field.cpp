@@ -1,0 +1,167 @@
+// leopard_field.cpp — see leopard_field.hpp.
+#include "leopard_field.hpp"
+
+#include <cstring>
+
+namespace rsx {
+
+int ceil_pow2(int n) {
+    int p = 1;
+    while (p < n) p <<= 1;
+    return p;
+}
+
+LeoField::LeoField(int b) : bits(b), order(1u << b), modulus((1u << b) - 1) {
+    static const uint16_t cantor16[16] = {0x0001, 0xACCA, 0x3C0E, 0x163E, 0xC582, 0xED2E,
+                                          0x914C, 0x4012, 0x6C98, 0x10D8, 0x6A72, 0xB900,
+                                          0xFDB8, 0xFB34, 0xFF38, 0x991E};
+    static const uint16_t cantor8[8] = {1, 214, 152, 146, 86, 200, 88, 230};
+    const uint16_t *cantor = bits == 16 ? cantor16 : cantor8;
+    const unsigned poly = bits == 16 ? 0x1002D : 0x11D;
+    exp.assign(order, 0);
+    log.assign(order, 0);
+    // LFSR, then conversion of the log table to the Cantor basis
+    // (leopard.go:1104-1146).
+    unsigned state = 1;
+    for (unsigned i = 0; i < modulus; ++i) {
+        exp[state] = static_cast<uint16_t>(i);
+        state <<= 1;
+        if (state >= order) state ^= poly;
+    }
+    exp[0] = static_cast<uint16_t>(modulus);
+    log[0] = 0;
+    for (int i = 0; i < bits; ++i) {
+        const int width = 1 << i;
+        for (int j = 0; j < width; ++j) log[j + width] = log[j] ^ cantor[i];
+    }
+    for (unsigned i = 0; i < order; ++i) log[i] = exp[log[i]];
+    for (unsigned i = 0; i < order; ++i) exp[log[i]] = static_cast<uint16_t>(i);
+    exp[modulus] = exp[0];
+
+    // FFT skew (leopard.go:1150-1195).
+    skew.assign(modulus, 0);
+    walsh.assign(order, 0);
+    std::vector<uint16_t> temp(bits - 1);
+    for (int i = 1; i < bits; ++i) temp[i - 1] = static_cast<uint16_t>(1u << i);
+    for (int m = 0; m < bits - 1; ++m) {
+        const int step = 1 << (m + 1);
+        skew[(1 << m) - 1] = 0;
+        for (int i = m; i < bits - 1; ++i) {
+            const int s = 1 << (i + 1);
+            for (int j = (1 << m) - 1; j < s; j += step) skew[j + s] = skew[j] ^ temp[i];
+        }
+        temp[m] = static_cast<uint16_t>(modulus - log[mul_log(temp[m], log[temp[m] ^ 1])]);
+        for (int i = m + 1; i < bits - 1; ++i) {
+            const uint16_t sum = add_mod(log[temp[i] ^ 1], temp[m]);
+            temp[i] = mul_log(temp[i], sum);
+        }
+    }
+    for (unsigned i = 0; i < modulus; ++i) skew[i] = log[skew[i]];
+    for (unsigned i = 0; i < order; ++i) walsh[i] = log[i];
+    walsh[0] = 0;
+    fwht(walsh.data(), static_cast<int>(order));
+}
+
+uint16_t LeoField::add_mod(uint32_t a, uint32_t b) const {
+    // leopard.go:1001-1006: ffe(sum + sum>>bitwidth), truncated to the field width.
+    const uint64_t sum = static_cast<uint64_t>(a) + b;
+    return static_cast<uint16_t>((sum + (sum >> bits)) & (order - 1));
+}
+
+uint16_t LeoField::mul_log(uint16_t a, uint16_t log_b) const {
+    if (a == 0) return 0;
+    return exp[add_mod(log[a], log_b)];
+}
+
+void LeoField::fwht(uint16_t *data, int mtrunc) const {
+    const uint64_t mask = order - 1;
+    int dist = 1, dist4 = 4;
+    while (dist4 <= static_cast<int>(order)) {
+        for (int r = 0; r < mtrunc; r += dist4)
+            for (int off = r; off < r + dist; ++off) {
+                uint64_t t0 = data[off], t1 = data[off + dist], t2 = data[off + 2 * dist],
+                         t3 = data[off + 3 * dist];
+                auto fwht2 = [&](uint64_t &a, uint64_t &b) {
+                    uint64_t s = a + b, d = a - b;
+                    s = (s + (s >> bits)) & mask;
+                    d = (d + (d >> bits)) & mask;
+                    a = s;
+                    b = d;
+                };
+                fwht2(t0, t1);
+                fwht2(t2, t3);
+                fwht2(t0, t2);
+                fwht2(t1, t3);
+                data[off] = static_cast<uint16_t>(t0);
+                data[off + dist] = static_cast<uint16_t>(t1);
+                data[off + 2 * dist] = static_cast<uint16_t>(t2);
+                data[off + 3 * dist] = static_cast<uint16_t>(t3);
+            }
+        dist = dist4;
+        dist4 <<= 2;
+    }
+}
+
+std::vector<uint32_t> LeoField::mul_tables() const {
+    const int tw = table_dwords();
+    std::vector<uint32_t> out(static_cast<size_t>(order) * tw);
+    for (unsigned lm = 0; lm < order; ++lm) {
+        uint32_t *rec = &out[static_cast<size_t>(lm) * tw];
+        auto prod = [&](unsigned x) { return mul_log(static_cast<uint16_t>(x), static_cast<uint16_t>(lm)); };
+        if (bits == 8) {
+            uint8_t t[20];
+            for (int v = 0; v < 8; ++v) {
+                t[v] = static_cast<uint8_t>(prod(v));
+                t[8 + v] = static_cast<uint8_t>(prod(v << 3));
+            }
+            for (int v = 0; v < 4; ++v) t[16 + v] = static_cast<uint8_t>(prod(v << 6));
+            std::memcpy(rec, t, sizeof t);
+            continue;
+        }
+        // bits 16: for each source byte (lo, hi) and group (0-2, 3-5, 6-7):
+        // tables of the product's lo byte and hi byte.
+        uint8_t t[80];
+        int pos = 0;
+        for (int src = 0; src < 2; ++src) {
+            const int base = src * 8;
+            for (int grp = 0; grp < 3; ++grp) {
+                const int shift = base + grp * 3;
+                const int entries = grp == 2 ? 4 : 8;
+                for (int ob = 0; ob < 2; ++ob)
+                    for (int v = 0; v < entries; ++v)
+                        t[pos++] = static_cast<uint8_t>(prod(static_cast<unsigned>(v) << shift) >> (8 * ob));
+            }
+        }
+        std::memcpy(rec, t, sizeof t);
+    }
+    return out;
+}
+
+const LeoField &leo_field(int bits) {
+    static const LeoField f16(16), f8(8);
+    return bits == 16 ? f16 : f8;
+}
+
+std::vector<uint16_t> leo_error_locators(const LeoField &f, int k, int p, const uint8_t *erased,
+                                         int *input_count) {
+    const int m = ceil_pow2(p);
+    const int n = ceil_pow2(m + k);
+    std::vector<uint16_t> e(f.order, 0);
+    int ic = p;
+    for (int i = 0; i < p; ++i)
+        if (erased[k + i]) e[i] = 1;
+    for (int i = p; i < m; ++i) e[i] = 1;
+    for (int i = 0; i < k; ++i) {
+        if (erased[i]) e[i + m] = 1;
+        else ic = m + i + 1;
+    }
+    f.fwht(e.data(), m + k);
+    for (unsigned i = 0; i < f.order; ++i)
+        e[i] = static_cast<uint16_t>((static_cast<uint64_t>(e[i]) * f.walsh[i]) % f.modulus);
+    f.fwht(e.data(), static_cast<int>(f.order));
+    if (input_count) *input_count = ic;
+    e.resize(n);
+    return e;
+}
+
+}  // namespace rsx

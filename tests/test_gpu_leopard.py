@@ -1,0 +1,164 @@
+"""GPU parity of the Leopard GF(2^16) and GF(2^8) engines against the oracle.
+
+Parity bytes must equal the oracle's (a line-faithful restatement of
+leopard.go / leopard8.go); reconstructed bytes must equal the oracle's and the
+original data.  Leopard is column-separable per 64-byte chunk, so the
+k=4096, m=1024, 64 KiB workload is checked against the oracle on a column
+slice of every shard and by a full encode -> erase 1024 -> reconstruct
+round trip.
+"""
+import numpy as np
+import pytest
+
+from oracle import rsref
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+import reedsolomon_amd as rs  # noqa: E402
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if not torch.cuda.is_available() or rs.device_count() == 0:
+        pytest.skip("no GPU")
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda")
+
+
+def host(t):
+    return t.cpu().numpy()
+
+
+def opt(bits):
+    return rs.WithLeopardGF16(True) if bits == 16 else rs.WithLeopardGF(True)
+
+
+def oracle_encode(bits, k, m, data):
+    o = rsref.Leopard(k, m, bits)
+    shards = [d.copy() for d in data] + [np.zeros(len(data[0]), np.uint8) for _ in range(m)]
+    assert o.encode(shards) == 0
+    return o, shards
+
+
+SHAPES = [(16, 1, 1), (16, 2, 1), (16, 3, 2), (16, 4, 4), (16, 5, 3), (16, 10, 3), (16, 8, 8),
+          (16, 17, 3), (16, 33, 7), (16, 3, 9), (16, 100, 20), (16, 260, 40), (16, 300, 300),
+          (8, 1, 1), (8, 3, 2), (8, 4, 4), (8, 10, 3), (8, 17, 3), (8, 33, 7), (8, 64, 64),
+          (8, 100, 20), (8, 192, 56), (8, 128, 128)]
+
+
+@pytest.mark.parametrize("bits,k,m", SHAPES)
+def test_encode_reconstruct_vs_oracle(bits, k, m):
+    rng = np.random.default_rng(bits * 10000 + k * 10 + m)
+    size = 64 * int(rng.integers(1, 5))
+    data = [rng.integers(0, 256, size, dtype=np.uint8) for _ in range(k)]
+    o, want = oracle_encode(bits, k, m, data)
+    enc = rs.New(k, m, opt(bits))
+    assert enc.ShardSizeMultiple() == 64
+    for mode in ("device", "host"):
+        shards = [d.copy() for d in data] + [np.zeros(size, np.uint8) for _ in range(m)]
+        if mode == "device":
+            shards = [dev(s) for s in shards]
+        enc.Encode(shards)
+        for i in range(m):
+            got = host(shards[k + i]) if mode == "device" else shards[k + i]
+            assert np.array_equal(got, want[k + i]), (mode, i)
+        assert enc.Verify(shards)
+    for trial in range(3):
+        e = int(rng.integers(1, m + 1))
+        lost = set(rng.choice(k + m, e, replace=False).tolist())
+        for method, ometh in (("Reconstruct", "reconstruct"), ("ReconstructData", "reconstruct_data")):
+            ref = [None if i in lost else want[i].copy() for i in range(k + m)]
+            assert getattr(o, ometh)(ref) == 0
+            work = [None if i in lost else dev(want[i]) for i in range(k + m)]
+            getattr(enc, method)(work)
+            for i in range(k + m):
+                assert (work[i] is None) == (ref[i] is None), (method, i)
+                if ref[i] is not None:
+                    assert np.array_equal(host(work[i]), ref[i]), (method, i, sorted(lost))
+                    assert np.array_equal(ref[i], want[i])
+
+
+@pytest.mark.parametrize("bits", [16, 8])
+def test_verify_false_and_errors(bits):
+    rng = np.random.default_rng(bits)
+    k, m, size = 12, 4, 256
+    data = [rng.integers(0, 256, size, dtype=np.uint8) for _ in range(k)]
+    _, full = oracle_encode(bits, k, m, data)
+    enc = rs.New(k, m, opt(bits))
+    shards = [dev(s) for s in full]
+    assert enc.Verify(shards)
+    shards[k + 2][77] ^= 0x10
+    assert not enc.Verify(shards)
+    with pytest.raises(rs.ErrInvalidShardSize):
+        enc.Encode([dev(np.zeros(100, np.uint8)) for _ in range(k + m)])
+    with pytest.raises(rs.ErrTooFewShards):
+        enc.Encode(shards[:-1])
+    work = [None] * (m + 1) + [dev(s) for s in full[m + 1:]]
+    with pytest.raises(rs.ErrTooFewShards):
+        enc.Reconstruct(work)
+    with pytest.raises(rs.ErrNotSupported):
+        enc.EncodeIdx(dev(data[0]), 0, [dev(np.zeros(size, np.uint8)) for _ in range(m)])
+
+
+def test_reconstruct_some_maps_to_reference_modes():
+    # leopard.go:378-383: all shards only when len(required) == total.
+    rng = np.random.default_rng(5)
+    k, m, size = 6, 3, 128
+    data = [rng.integers(0, 256, size, dtype=np.uint8) for _ in range(k)]
+    _, full = oracle_encode(16, k, m, data)
+    enc = rs.New(k, m, rs.WithLeopardGF16(True))
+    work = [None, dev(full[1])] + [dev(s) for s in full[2:k]] + [None, dev(full[k + 1]), dev(full[k + 2])]
+    enc.ReconstructSome(work, [True] * k)
+    assert work[k] is None and np.array_equal(host(work[0]), full[0])
+    work = [None, dev(full[1])] + [dev(s) for s in full[2:k]] + [None, dev(full[k + 1]), dev(full[k + 2])]
+    enc.ReconstructSome(work, [True] * (k + m))
+    assert np.array_equal(host(work[k]), full[k])
+
+
+def test_leopard16_4096_1024_64kib():
+    # The Leopard GF(2^16) workload: k=4096, m=1024, 64 KiB shards; erase a
+    # seeded random 1024 shards and reconstruct via the FFT path.
+    k, m, size = 4096, 1024, 64 << 10
+    g = torch.Generator(device="cuda")
+    g.manual_seed(4096)
+    buf = torch.empty((k + m, size), dtype=torch.uint8, device="cuda")
+    buf[:k].random_(0, 256, generator=g)
+    buf[k:].zero_()
+    shards = [buf[i] for i in range(k + m)]
+    enc = rs.New(k, m, rs.WithLeopardGF16(True))
+    enc.Encode(shards)
+    # oracle on a 128-byte column slice of every shard (column-separable)
+    sl = slice(64 * 100, 64 * 102)
+    cols = buf[:, sl].cpu().numpy()
+    _, want = oracle_encode(16, k, m, [cols[i].copy() for i in range(k)])
+    for i in range(m):
+        assert np.array_equal(cols[k + i], want[k + i]), i
+    orig = buf.clone()
+    rng = np.random.default_rng(1024)
+    lost = sorted(rng.choice(k + m, m, replace=False).tolist())
+    work = [None if i in set(lost) else shards[i] for i in range(k + m)]
+    enc.Reconstruct(work)
+    for i in lost:
+        assert torch.equal(work[i], orig[i]), i
+
+
+def test_leopard8_192_56_64kib():
+    # The Leopard GF(2^8) slice of the served stream at (192, 56) 64 KiB
+    # ((200, 56) indexes past the reference's tables).
+    k, m, size = 192, 56, 64 << 10
+    rng = np.random.default_rng(192)
+    data = [rng.integers(0, 256, size, dtype=np.uint8) for _ in range(k)]
+    o, want = oracle_encode(8, k, m, data)
+    enc = rs.New(k, m, rs.WithLeopardGF(True))
+    shards = [dev(d) for d in data] + [torch.zeros(size, dtype=torch.uint8, device="cuda") for _ in range(m)]
+    enc.Encode(shards)
+    for i in range(m):
+        assert np.array_equal(host(shards[k + i]), want[k + i]), i
+    lost = set(rng.choice(k + m, m, replace=False).tolist())
+    work = [None if i in lost else shards[i] for i in range(k + m)]
+    enc.Reconstruct(work)
+    for i in lost:
+        assert np.array_equal(host(work[i]), want[i]), i
