@@ -127,6 +127,44 @@ int rsx_synchronize(rsx_codec *c);
  * (inversion_tree.go analogue). */
 int rsx_decode_cache_size(const rsx_codec *c);
 
+/* ---- Batched path -------------------------------------------------------
+ * Many Encode / Reconstruct / ReconstructData calls of classic codecs (any
+ * (k, m), shard size and erasure pattern) planned once into a device work
+ * list and run as one persistent launch per output-count class, captured in
+ * a HIP graph that rsx_batch_launch replays.  There is no single reference
+ * counterpart: each item stands for one call of reedsolomon.go:686 / 1446 /
+ * 1634 and is validated as that call is; status[i] receives its result.
+ * Items of one plan must be independent (no item may read a shard another
+ * item writes), shards must be device memory, and the codecs must outlive
+ * the plan.  Outputs are overwritten, so a plan may be launched repeatedly. */
+#define RSX_OP_ENCODE 1
+#define RSX_OP_RECONSTRUCT 2
+#define RSX_OP_RECONSTRUCT_DATA 3
+
+typedef struct rsx_batch_item {
+    rsx_codec *codec;       /* classic codec from rsx_new */
+    int op;                 /* RSX_OP_* */
+    int nshards;            /* entries of shards / lens (the codec's TotalShards) */
+    uint8_t *const *shards; /* device memory; absent shards carry an output buffer */
+    const size_t *lens;     /* Go-style lengths, 0 = absent */
+} rsx_batch_item;
+
+typedef struct rsx_batch rsx_batch;
+
+int rsx_batch_create(int device, rsx_batch **out);
+void rsx_batch_free(rsx_batch *b);
+/* filled receives, item after item, TotalShards flags per item. */
+int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int nitems, int *status,
+                   uint8_t *filled);
+int rsx_batch_launch(rsx_batch *b);       /* stream-ordered on rsx_batch_stream(b) */
+int rsx_batch_synchronize(rsx_batch *b);
+void *rsx_batch_stream(rsx_batch *b);
+int rsx_batch_num_tasks(const rsx_batch *b);
+/* Tile bytes per work unit and resident blocks per CU; 0, 0 = autotune at
+ * the next plan (the choice is kept for later plans of this batch). */
+int rsx_batch_set_tuning(rsx_batch *b, int tile_bytes, int blocks_per_cu);
+int rsx_batch_get_tuning(const rsx_batch *b, int *tile_bytes, int *blocks_per_cu);
+
 /* Message of the last error on the calling thread. */
 const char *rsx_last_error(void);
 int rsx_device_count(void);

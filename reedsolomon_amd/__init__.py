@@ -349,5 +349,114 @@ def New(data_shards: int, parity_shards: int, *opts) -> Encoder:
     return Encoder(h.value, data_shards, parity_shards)
 
 
+# --- batched path (include/rsx.h rsx_batch_*) ---
+OP_ENCODE, OP_RECONSTRUCT, OP_RECONSTRUCT_DATA = 1, 2, 3
+
+
+class _BatchItem(ctypes.Structure):
+    _fields_ = [("codec", ctypes.c_void_p), ("op", ctypes.c_int), ("nshards", ctypes.c_int),
+                ("shards", ctypes.c_void_p), ("lens", ctypes.c_void_p)]
+
+
+lib.rsx_batch_create.argtypes = [ctypes.c_int, ctypes.POINTER(_P)]
+lib.rsx_batch_free.argtypes = [_P]
+lib.rsx_batch_plan.argtypes = [_P, ctypes.POINTER(_BatchItem), ctypes.c_int,
+                               ctypes.POINTER(ctypes.c_int), _U8]
+lib.rsx_batch_launch.argtypes = [_P]
+lib.rsx_batch_synchronize.argtypes = [_P]
+lib.rsx_batch_stream.argtypes = [_P]
+lib.rsx_batch_stream.restype = _P
+lib.rsx_batch_num_tasks.argtypes = [_P]
+lib.rsx_batch_set_tuning.argtypes = [_P, ctypes.c_int, ctypes.c_int]
+lib.rsx_batch_get_tuning.argtypes = [_P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+
+
+class Batch:
+    """Many Encode / Reconstruct / ReconstructData calls as one planned,
+    graph-captured launch.  Items are (encoder, op, shards) with shards on
+    the GPU; absent shards of reconstruct items get output tensors at plan
+    time, holding their results after launch()."""
+
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        _check(lib.rsx_batch_create(device, ctypes.byref(h)))
+        self._h = h.value
+        self._keep = []
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib.rsx_batch_free(h)
+            self._h = None
+
+    def set_tuning(self, tile_bytes: int = 0, blocks_per_cu: int = 0) -> None:
+        _check(lib.rsx_batch_set_tuning(self._h, tile_bytes, blocks_per_cu))
+
+    def tuning(self):
+        t, b = ctypes.c_int(), ctypes.c_int()
+        lib.rsx_batch_get_tuning(self._h, ctypes.byref(t), ctypes.byref(b))
+        return t.value, b.value
+
+    def num_tasks(self) -> int:
+        return lib.rsx_batch_num_tasks(self._h)
+
+    @property
+    def stream(self):
+        return lib.rsx_batch_stream(self._h)
+
+    def plan(self, items):
+        """Returns the per-item status codes (0 = ok)."""
+        import torch
+        n = len(items)
+        arr = (_BatchItem * max(n, 1))()
+        keep, fills = [], []
+        total_flags = 0
+        for idx, (enc, op, shards) in enumerate(items):
+            bufs = list(shards)
+            size, like = 0, None
+            for s in shards:
+                if s is not None and s.numel():
+                    size, like = s.numel(), s
+                    break
+            if op != OP_ENCODE and size:
+                for i, s in enumerate(shards):
+                    if s is None or s.numel() == 0:
+                        bufs[i] = torch.empty(size, dtype=torch.uint8, device=like.device)
+            mm = _Marshal(bufs)
+            for i, s in enumerate(shards):
+                if s is None or s.numel() == 0:
+                    mm.lens[i] = 0
+            keep.append(mm)
+            arr[idx].codec = enc.handle
+            arr[idx].op = op
+            arr[idx].nshards = len(shards)
+            arr[idx].shards = ctypes.cast(mm.ptrs, ctypes.c_void_p)
+            arr[idx].lens = ctypes.cast(mm.lens, ctypes.c_void_p)
+            fills.append((total_flags, shards, bufs))
+            total_flags += len(shards)
+        status = (ctypes.c_int * max(n, 1))()
+        filled = (ctypes.c_uint8 * max(total_flags, 1))()
+        _check(lib.rsx_batch_plan(self._h, arr, n, status, filled))
+        for off, shards, bufs in fills:
+            for i in range(len(shards)):
+                if filled[off + i]:
+                    shards[i] = bufs[i]
+        self._keep = [keep, arr, items]
+        return [status[i] for i in range(n)]
+
+    def launch(self) -> None:
+        _check(lib.rsx_batch_launch(self._h))
+
+    def synchronize(self) -> None:
+        _check(lib.rsx_batch_synchronize(self._h))
+
+    def run(self, items):
+        st = self.plan(items)
+        self.launch()
+        self.synchronize()
+        return st
+
+
 __all__ = [n for n in dir() if n.startswith(("With", "Err", "New", "Encoder", "RSError"))] + [
-    "device_count", "lib", "LIB_PATH"]
+    "device_count", "lib", "LIB_PATH", "Batch", "OP_ENCODE", "OP_RECONSTRUCT",
+    "OP_RECONSTRUCT_DATA"]

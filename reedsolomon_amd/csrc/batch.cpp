@@ -1,0 +1,387 @@
+// batch.cpp — the batched path of include/rsx.h: many Encode / Reconstruct
+// calls of classic codecs (any (k, m), shard size and erasure pattern) run as
+// one persistent launch per output-count class, captured in a HIP graph.
+//
+// Each item is validated exactly as its per-stripe method validates it
+// (reedsolomon.go:686-702 Encode, 1670-1781 reconstruct), so an item's status
+// equals the error the plain call would return.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <map>
+
+#include "batch.hpp"
+#include "codec.hpp"
+
+namespace rsx {
+int check_shards(const size_t *lens, int n, bool nilok);
+size_t shard_size(const size_t *lens, int n);
+int decode_plan(rsx_codec *c, const size_t *lens, bool data_only, const uint8_t *required,
+                const DecodePlan **plan);
+}  // namespace rsx
+
+using namespace rsx;
+
+struct rsx_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cus = 0;
+    // tuning: 0 = autotune on first launch
+    uint32_t tile = 0;
+    int blocks_per_cu = 0;
+    bool tuned = false;
+
+    // plan
+    struct Task {
+        const uint32_t *tables;  // device tables (batch-owned copy)
+        uint64_t len;
+        uint32_t in_off, out_off;
+        uint16_t k, nout, mtot, o0;
+        uint8_t aligned;
+    };
+    std::vector<Task> tasks[9];          // by output-count class 1..8
+    std::vector<const uint8_t *> pool;   // pointer pool
+    DeviceBuffer d_tasks, d_ptrs, d_tables;
+    std::vector<BatchLaunch> launches;   // one per non-empty class
+    std::vector<int> classes;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool planned = false;
+};
+
+namespace {
+
+void destroy_graph(rsx_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+// Lays the tasks out for one tile size and uploads them.
+int build_launches(rsx_batch *b, uint32_t tile) {
+    std::vector<BatchTask> all;
+    b->launches.clear();
+    b->classes.clear();
+    for (int mc = 1; mc <= 8; ++mc) {
+        if (b->tasks[mc].empty()) continue;
+        BatchLaunch L{};
+        L.ntasks = static_cast<int>(b->tasks[mc].size());
+        L.tile = tile;
+        uint64_t t0 = 0;
+        int maxk = 1;
+        const size_t first = all.size();
+        for (const auto &t : b->tasks[mc]) {
+            BatchTask d{};
+            d.tables = t.tables;
+            d.len = t.len;
+            d.tile0 = static_cast<uint32_t>(t0);
+            d.in_off = t.in_off;
+            d.out_off = t.out_off;
+            d.k = t.k;
+            d.nout = t.nout;
+            d.mtot = t.mtot;
+            d.o0 = t.o0;
+            d.accumulate = 0;
+            d.aligned = t.aligned;
+            all.push_back(d);
+            t0 += (t.len + tile - 1) / tile;
+            maxk = std::max<int>(maxk, t.k);
+        }
+        if (t0 > 0xFFFFFFFFull) return fail(RSX_ERR_INVALID_INPUT, "batch too large for one plan");
+        L.ntiles = static_cast<uint32_t>(t0);
+        L.maxk = maxk;
+        L.tasks = reinterpret_cast<const BatchTask *>(first);  // offset, patched after upload
+        b->launches.push_back(L);
+        b->classes.push_back(mc);
+    }
+    int err = b->d_tasks.reserve(std::max<size_t>(all.size(), 1) * sizeof(BatchTask));
+    if (err) return err;
+    RSX_HIP(hipMemcpyAsync(b->d_tasks.ptr, all.data(), all.size() * sizeof(BatchTask),
+                           hipMemcpyHostToDevice, b->stream));
+    RSX_HIP(hipStreamSynchronize(b->stream));
+    for (auto &L : b->launches) {
+        L.tasks = b->d_tasks.as<BatchTask>() + reinterpret_cast<size_t>(L.tasks);
+        L.ptrs = b->d_ptrs.as<const uint8_t *>();
+    }
+    return RSX_OK;
+}
+
+int enqueue(rsx_batch *b, hipStream_t s) {
+    const unsigned grid = static_cast<unsigned>(std::max(1, b->cus * b->blocks_per_cu));
+    for (size_t i = 0; i < b->launches.size(); ++i) {
+        const unsigned g = std::min<unsigned>(grid, std::max<uint32_t>(b->launches[i].ntiles, 1));
+        RSX_HIP(launch_batch_pass(b->launches[i], b->classes[i], g, s));
+    }
+    return RSX_OK;
+}
+
+int capture(rsx_batch *b) {
+    destroy_graph(b);
+    RSX_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int err = enqueue(b, b->stream);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (err) {
+        if (g) (void)hipGraphDestroy(g);
+        return err;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    b->graph = g;
+    RSX_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    return RSX_OK;
+}
+
+// Times the planned work for a few (tile, blocks/CU) choices and keeps the
+// fastest.  The work is idempotent (outputs are overwritten), so running it
+// repeatedly is harmless.
+int autotune(rsx_batch *b) {
+    static const uint32_t tiles[] = {16u << 10, 64u << 10, 256u << 10};
+    static const int bpcs[] = {2, 4, 8};
+    hipEvent_t e0, e1;
+    RSX_HIP(hipEventCreate(&e0));
+    RSX_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    uint32_t best_tile = tiles[1];
+    int best_bpc = 4;
+    int err = RSX_OK;
+    for (uint32_t tile : tiles) {
+        if ((err = build_launches(b, tile))) break;
+        for (int bpc : bpcs) {
+            b->blocks_per_cu = bpc;
+            if ((err = enqueue(b, b->stream))) break;  // warm
+            (void)hipEventRecord(e0, b->stream);
+            if ((err = enqueue(b, b->stream))) break;
+            (void)hipEventRecord(e1, b->stream);
+            (void)hipEventSynchronize(e1);
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (ms < best) {
+                best = ms;
+                best_tile = tile;
+                best_bpc = bpc;
+            }
+        }
+        if (err) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (err) return err;
+    b->tile = best_tile;
+    b->blocks_per_cu = best_bpc;
+    b->tuned = true;
+    return build_launches(b, b->tile);
+}
+
+}  // namespace
+
+extern "C" {
+
+int rsx_batch_create(int device, rsx_batch **out) {
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        (void)hipGetLastError();
+        return fail(RSX_ERR_DEVICE, "no HIP device available");
+    }
+    if (device < 0 || device >= ndev) return fail(RSX_ERR_DEVICE, "invalid device ordinal");
+    RSX_HIP(hipSetDevice(device));
+    auto *b = new rsx_batch;
+    b->device = device;
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) {
+        delete b;
+        return fail(RSX_ERR_DEVICE, "stream creation failed");
+    }
+    *out = b;
+    return RSX_OK;
+}
+
+void rsx_batch_free(rsx_batch *b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    (void)hipStreamSynchronize(b->stream);
+    destroy_graph(b);
+    (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+int rsx_batch_set_tuning(rsx_batch *b, int tile_bytes, int blocks_per_cu) {
+    if (tile_bytes < 0 || blocks_per_cu < 0 || tile_bytes % 16)
+        return fail(RSX_ERR_INVALID_INPUT, "tile must be a multiple of 16 bytes");
+    b->tile = static_cast<uint32_t>(tile_bytes);
+    b->blocks_per_cu = blocks_per_cu;
+    b->tuned = tile_bytes > 0 && blocks_per_cu > 0;
+    return RSX_OK;
+}
+
+int rsx_batch_get_tuning(const rsx_batch *b, int *tile_bytes, int *blocks_per_cu) {
+    *tile_bytes = static_cast<int>(b->tile);
+    *blocks_per_cu = b->blocks_per_cu;
+    return RSX_OK;
+}
+
+void *rsx_batch_stream(rsx_batch *b) { return b->stream; }
+
+int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status, uint8_t *filled) {
+    (void)hipSetDevice(b->device);
+    RSX_HIP(hipStreamSynchronize(b->stream));
+    destroy_graph(b);
+    for (auto &v : b->tasks) v.clear();
+    b->pool.clear();
+    b->planned = false;
+    std::map<const void *, size_t> table_off;  // source tables -> offset in batch copy (dwords)
+    std::vector<std::pair<const uint32_t *, size_t>> table_src;  // (source, dwords)
+    size_t table_dwords = 0;
+    size_t fill_off = 0;
+
+    struct Pending {
+        int mc;
+        size_t idx;
+        const uint32_t *src;
+    };
+    std::vector<Pending> pend;
+
+    for (int it = 0; it < n; ++it) {
+        const rsx_batch_item &item = items[it];
+        rsx_codec *c = item.codec;
+        status[it] = RSX_OK;
+        uint8_t *fl = filled ? filled + fill_off : nullptr;
+        if (c) fill_off += c->total;
+        if (!c || c->engine) {
+            status[it] = fail(RSX_ERR_NOT_SUPPORTED, "batched path covers classic codecs");
+            continue;
+        }
+        if (fl) std::memset(fl, 0, c->total);
+        const int k = c->k, total = c->total;
+        const size_t *lens = item.lens;
+        std::vector<const uint8_t *> in;
+        std::vector<uint8_t *> out;
+        const uint32_t *tables = nullptr;
+        int mtot = 0;
+        if (item.op == RSX_OP_ENCODE) {
+            // reedsolomon.go:686-702
+            int err = RSX_OK;
+            if (item.nshards != total) err = fail(RSX_ERR_TOO_FEW_SHARDS, "too few shards given");
+            else err = check_shards(lens, total, false);
+            if (err) {
+                status[it] = err;
+                continue;
+            }
+            if (c->m == 0) continue;
+            for (int j = 0; j < k; ++j) in.push_back(item.shards[j]);
+            for (int i = k; i < total; ++i) out.push_back(item.shards[i]);
+            tables = c->parity_tables.as<uint32_t>();
+            mtot = c->m;
+        } else if (item.op == RSX_OP_RECONSTRUCT || item.op == RSX_OP_RECONSTRUCT_DATA) {
+            // reedsolomon.go:1670-1709
+            const bool data_only = item.op == RSX_OP_RECONSTRUCT_DATA;
+            int err = RSX_OK;
+            if (item.nshards != total) err = fail(RSX_ERR_TOO_FEW_SHARDS, "too few shards given");
+            else err = check_shards(lens, total, true);
+            if (err) {
+                status[it] = err;
+                continue;
+            }
+            int present = 0, data_present = 0;
+            for (int i = 0; i < total; ++i)
+                if (lens[i]) {
+                    ++present;
+                    if (i < k) ++data_present;
+                }
+            if (present == total || (data_only && data_present == k)) continue;
+            if (present < k) {
+                status[it] = fail(RSX_ERR_TOO_FEW_SHARDS, "too few shards given");
+                continue;
+            }
+            const DecodePlan *plan = nullptr;
+            {
+                std::lock_guard<std::mutex> lock(c->mu);
+                err = decode_plan(c, lens, data_only, nullptr, &plan);
+            }
+            if (err) {
+                status[it] = err;
+                continue;
+            }
+            for (int i : plan->inputs) in.push_back(item.shards[i]);
+            for (int i : plan->outputs) {
+                out.push_back(item.shards[i]);
+                if (fl) fl[i] = 1;
+            }
+            tables = plan->tables.as<uint32_t>();
+            mtot = static_cast<int>(plan->outputs.size());
+        } else {
+            status[it] = fail(RSX_ERR_INVALID_INPUT, "unknown batch op");
+            continue;
+        }
+        const size_t len = shard_size(lens, total);
+        uintptr_t bits = 0;
+        for (auto *p : in) bits |= reinterpret_cast<uintptr_t>(p);
+        for (auto *p : out) bits |= reinterpret_cast<uintptr_t>(p);
+        if (table_off.find(tables) == table_off.end()) {
+            table_off[tables] = table_dwords;
+            const size_t dw = static_cast<size_t>(k) * mtot * 5;
+            table_src.emplace_back(tables, dw);
+            table_dwords += dw;
+        }
+        const uint32_t in_off = static_cast<uint32_t>(b->pool.size());
+        b->pool.insert(b->pool.end(), in.begin(), in.end());
+        for (int o0 = 0; o0 < mtot; o0 += 8) {
+            const int nout = std::min(8, mtot - o0);
+            const uint32_t out_off = static_cast<uint32_t>(b->pool.size());
+            for (int i = 0; i < nout; ++i) b->pool.push_back(out[o0 + i]);
+            rsx_batch::Task t{};
+            t.tables = nullptr;
+            t.len = len;
+            t.in_off = in_off;
+            t.out_off = out_off;
+            t.k = static_cast<uint16_t>(k);
+            t.nout = static_cast<uint16_t>(nout);
+            t.mtot = static_cast<uint16_t>(mtot);
+            t.o0 = static_cast<uint16_t>(o0);
+            t.aligned = (bits & 15) == 0;
+            b->tasks[nout].push_back(t);
+            pend.push_back({nout, b->tasks[nout].size() - 1, tables});
+        }
+    }
+    // Batch-owned copy of every table the plan reads.
+    int err = b->d_tables.reserve(std::max<size_t>(table_dwords, 1) * 4);
+    if (err) return err;
+    for (auto &ts : table_src)
+        RSX_HIP(hipMemcpyAsync(b->d_tables.as<uint32_t>() + table_off[ts.first], ts.first, ts.second * 4,
+                               hipMemcpyDeviceToDevice, b->stream));
+    for (auto &p : pend) b->tasks[p.mc][p.idx].tables = b->d_tables.as<uint32_t>() + table_off[p.src];
+    if ((err = b->d_ptrs.reserve(std::max<size_t>(b->pool.size(), 1) * sizeof(void *)))) return err;
+    RSX_HIP(hipMemcpyAsync(b->d_ptrs.ptr, b->pool.data(), b->pool.size() * sizeof(void *),
+                           hipMemcpyHostToDevice, b->stream));
+    RSX_HIP(hipStreamSynchronize(b->stream));
+    if (b->tuned) {
+        if ((err = build_launches(b, b->tile))) return err;
+    } else {
+        if ((err = autotune(b))) return err;
+    }
+    if ((err = capture(b))) return err;
+    b->planned = true;
+    return RSX_OK;
+}
+
+int rsx_batch_launch(rsx_batch *b) {
+    if (!b->planned) return fail(RSX_ERR_INVALID_INPUT, "batch has no plan");
+    (void)hipSetDevice(b->device);
+    RSX_HIP(hipGraphLaunch(b->exec, b->stream));
+    return RSX_OK;
+}
+
+int rsx_batch_synchronize(rsx_batch *b) {
+    (void)hipSetDevice(b->device);
+    RSX_HIP(hipStreamSynchronize(b->stream));
+    return RSX_OK;
+}
+
+int rsx_batch_num_tasks(const rsx_batch *b) {
+    size_t n = 0;
+    for (auto &v : b->tasks) n += v.size();
+    return static_cast<int>(n);
+}
+
+}  // extern "C"

@@ -1,0 +1,38 @@
+// batch.hpp — device work list of the batched GF(2^8) path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace rsx {
+
+constexpr int kBatchMaxIn = 256;  // inputs per task (classic engine: k <= 256)
+
+// One matrix pass of one stripe: nout (<= 8) outputs from k inputs.
+struct BatchTask {
+    const uint32_t *tables;  // [k][mtot][5] coefficient tables
+    uint64_t len;            // bytes per shard
+    uint32_t tile0;          // first global tile index of this task
+    uint32_t in_off;         // k input pointers at ptrs[in_off]
+    uint32_t out_off;        // nout output pointers at ptrs[out_off]
+    uint16_t k, nout, mtot, o0;
+    uint8_t accumulate;      // 0: out = M*in, 1: out ^= M*in
+    uint8_t aligned;         // every pointer 16-byte aligned
+    uint8_t pad[6];
+};
+
+struct BatchLaunch {
+    const BatchTask *tasks;
+    const uint8_t *const *ptrs;
+    int ntasks;
+    int maxk;       // largest k among the tasks (sizes the LDS tables)
+    uint32_t ntiles;
+    uint32_t tile;  // bytes per tile (multiple of 16)
+};
+
+// Launches the persistent pass for tasks of exactly m_class outputs.
+hipError_t launch_batch_pass(const BatchLaunch &L, int m_class, unsigned grid, hipStream_t s);
+
+}  // namespace rsx
