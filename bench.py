@@ -1,21 +1,30 @@
 """Throughput of the MI355X erasure-coding hot path on one GPU.
 
-Workload (the plain-GPU RS(16,4) case): 256 stripes of 16 data shards x
+Headline workload (plain-GPU RS(16,4) case): 256 stripes of 16 data shards x
 4 MiB of seeded uniform random bytes plus 4 parity shards, resident in HBM.
-One step = Encode every stripe, then Reconstruct every stripe after erasing
-a seeded random set of 0-4 of its 20 shards (data or parity).  The erased
-shards are rebuilt in place, so every step sees the same state.
+One step = Encode every stripe, then Reconstruct every stripe after erasing a
+seeded random set of 0-4 of its 20 shards (data or parity).  Erased shards
+are rebuilt in place, so every step sees the same state.  The step runs two
+ways: the plain path (one C-ABI call per stripe) and the batched path (one
+planned, graph-replayed launch for all encodes, one for all reconstructs).
+`value` is the batched path; the plain path is reported beside it.
+
+Also measured (unless --quick): the mixed workload (8192 stripes, (k, m) in
+{(4,2),(6,3),(10,4),(17,3),(32,8)}, shards log-uniform 4 KiB-1 MiB rounded to
+64 B, 30% reconstructs with 1..m erasures) batched vs plain, and Leopard
+GF(2^16) k=4096, m=1024, 64 KiB shards (encode + reconstruct of 1024 erasures).
 
 Bytes per operation follow the reference's benchmarks
 (b.SetBytes(shardSize*(dataShards+parityShards))): (k+m)*shard for each
-Encode and for each Reconstruct that has work to do (a stripe with no
-erasure returns immediately and is not counted).
+Encode and for each Reconstruct that has work to do (a stripe with no erasure
+returns immediately and is not counted).
 
-    python bench.py [--steps K] [--warmup W] [--stripes N] [--shard-mib S]
+    python bench.py [--steps K] [--warmup W] [--quick]
 
-Prints one JSON line.  `roofline` is for the dominant kernel (the encode
-pass, 16 inputs -> 4 outputs), timed per launch with HIP events on the
-codec's stream; `cpu_baseline` times the CPU oracle on one stripe.
+Prints one JSON line.  `roofline` is for the dominant kernel (the batched
+RS(16,4) encode pass over all 256 stripes), timed per launch with HIP events
+on the stream it runs on; `cpu_baseline` times the CPU oracle's SIMD
+restatement of the reference on the host cores for a bounded sample.
 """
 from __future__ import annotations
 
@@ -30,25 +39,350 @@ import numpy as np
 
 HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E, 8 TB/s
 METRIC = "encode+reconstruct GB/s and % HBM peak, batched vs plain GPU; p50/p99 latency"
+SEED = 0xABADC0CAC01A
 
 
 def _parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--stripes", type=int, default=256)
     ap.add_argument("--shard-mib", type=float, default=4.0)
-    ap.add_argument("--k", type=int, default=16)
-    ap.add_argument("--m", type=int, default=4)
-    ap.add_argument("--seed", type=int, default=0xABADC0CAC01A)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--quick", action="store_true", help="headline workload only")
     return ap.parse_args()
 
 
 def _log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def _check(lib, rc):
+    if rc != 0:
+        raise RuntimeError(f"rsx error {rc}: {lib.rsx_last_error().decode()}")
+
+
+class Stripes:
+    """Shard pointers / lengths of many stripes laid out in one HBM buffer."""
+
+    def __init__(self, shapes, sizes, torch, gen):
+        offs, total = [], 0
+        for (k, m), s in zip(shapes, sizes):
+            offs.append(total)
+            total += (k + m) * s
+        self.buf = torch.empty(total, dtype=torch.uint8, device="cuda")
+        base = self.buf.data_ptr()
+        self.shapes, self.sizes, self.offs = shapes, sizes, offs
+        self.ptrs, self.full = [], []
+        for (k, m), s, o in zip(shapes, sizes, offs):
+            n = k + m
+            self.buf[o:o + k * s].random_(0, 256, generator=gen)
+            self.buf[o + k * s:o + n * s].zero_()
+            self.ptrs.append((ctypes.c_void_p * n)(*[base + o + i * s for i in range(n)]))
+            self.full.append((ctypes.c_size_t * n)(*([s] * n)))
+
+    def shard(self, i, j):
+        (k, m), s, o = self.shapes[i], self.sizes[i], self.offs[i]
+        return self.buf[o + j * s:o + (j + 1) * s]
+
+
+def batch_items(rs, st, codecs, ops, lens):
+    items = (rs._BatchItem * len(st.shapes))()
+    for i, shape in enumerate(st.shapes):
+        items[i].codec = codecs[shape].handle
+        items[i].op = ops[i]
+        items[i].nshards = shape[0] + shape[1]
+        items[i].shards = ctypes.cast(st.ptrs[i], ctypes.c_void_p)
+        items[i].lens = ctypes.cast(lens[i], ctypes.c_void_p)
+    return items
+
+
+def plan(rs, b, items, n):
+    status = (ctypes.c_int * n)()
+    filled = (ctypes.c_uint8 * (n * 64))()
+    _check(rs.lib, rs.lib.rsx_batch_plan(b._h, items, n, status, filled))
+    bad = [status[i] for i in range(n) if status[i] != 0]
+    if bad:
+        raise RuntimeError(f"batch item errors: {bad[:5]}")
+
+
+def event_time(torch, stream_ptr, fn, reps):
+    """Average duration of fn() on `stream_ptr` over reps, via HIP events."""
+    stream = torch.cuda.ExternalStream(stream_ptr)
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(reps)]
+    for a, b in evs:
+        a.record(stream)
+        fn()
+        b.record(stream)
+    torch.cuda.synchronize()
+    stream.synchronize()
+    return float(np.mean([a.elapsed_time(b) * 1e-3 for a, b in evs]))
+
+
+def headline(args, torch, rs):
+    """The RS(16,4) 256 x 16 x 4 MiB workload, plain and batched."""
+    lib = rs.lib
+    k, m = 16, 4
+    n = k + m
+    S = int(args.shard_mib * (1 << 20))
+    S -= S % 64
+    nst = args.stripes
+    _log(f"bench: RS({k},{m}) stripes={nst} shard={S} B, {nst * n * S / 2**30:.1f} GiB")
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(SEED & 0x7FFFFFFFFFFFFFFF)
+    st = Stripes([(k, m)] * nst, [S] * nst, torch, gen)
+    enc = rs.New(k, m, rs.WithMI355X(0))
+    h = enc.handle
+    rng = np.random.default_rng(SEED)
+    erase, rec_bytes = [], 0
+    for s in range(nst):
+        e = int(rng.integers(0, m + 1))
+        lost = set(rng.choice(n, e, replace=False).tolist()) if e else set()
+        erase.append(((ctypes.c_size_t * n)(*[0 if i in lost else S for i in range(n)]), e))
+        rec_bytes += n * S if e else 0
+    filled = (ctypes.c_uint8 * n)()
+    bytes_per_step = nst * n * S + rec_bytes
+
+    def plain_step():
+        for s in range(nst):
+            _check(lib, lib.rsx_encode(h, st.ptrs[s], st.full[s], n))
+        for s in range(nst):
+            lens, e = erase[s]
+            if e:
+                _check(lib, lib.rsx_reconstruct(h, st.ptrs[s], lens, n, filled))
+
+    # batched: one plan for the encodes, one for the reconstructs, replayed in
+    # order on the encode plan's stream.
+    benc, brec = rs.Batch(0), rs.Batch(0)
+    plan(rs, benc, batch_items(rs, st, {(k, m): enc}, [rs.OP_ENCODE] * nst, st.full), nst)
+    rec_ops = [rs.OP_RECONSTRUCT] * nst
+    plan(rs, brec, batch_items(rs, st, {(k, m): enc}, rec_ops, [e[0] for e in erase]), nst)
+    sbat = benc.stream
+
+    def batched_step():
+        benc.launch()
+        brec.launch(sbat)
+
+    for _ in range(max(args.warmup, 1)):
+        plain_step()
+        batched_step()
+    _check(lib, lib.rsx_synchronize(h))
+    benc.synchronize()
+
+    # parity of one column slice of stripe 0 against the CPU oracle
+    parity_ok = None
+    try:
+        from oracle import rsref
+        cols = 1 << 16
+        ref = [st.shard(0, i)[:cols].cpu().numpy().copy() for i in range(n)]
+        want = [r.copy() for r in ref[:k]] + [np.zeros(cols, np.uint8) for _ in range(m)]
+        rsref.Classic(k, m).encode(want)
+        parity_ok = all(np.array_equal(want[k + i], ref[k + i]) for i in range(m))
+    except Exception as ex:  # the oracle is a checker only
+        _log(f"bench: oracle check skipped: {ex}")
+
+    def timed(step, sync):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        sync()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    t_plain = timed(plain_step, lambda: _check(lib, lib.rsx_synchronize(h)))
+    t_batch = timed(batched_step, benc.synchronize)
+    plain = bytes_per_step * args.steps / t_plain / 1e9
+    batched = bytes_per_step * args.steps / t_batch / 1e9
+    _log(f"bench: RS(16,4) plain {plain:.1f} GB/s, batched {batched:.1f} GB/s")
+
+    # roofline: the batched encode launch (one kernel over all 256 stripes)
+    avg = event_time(torch, sbat, benc.launch, 20)
+    launch_bytes = nst * n * S
+    achieved = launch_bytes / avg / 1e9
+    # plain per-stripe encode launch, for comparison
+    avg_plain = event_time(torch, lib.rsx_get_stream(h),
+                           lambda: _check(lib, lib.rsx_encode(h, st.ptrs[0], st.full[0], n)), 50)
+    tile, bpc = benc.tuning()
+    out = dict(
+        bytes_per_step=bytes_per_step,
+        plain_gbps=round(plain, 1), batched_gbps=round(batched, 1),
+        ms_plain=round(t_plain / args.steps * 1e3, 3), ms_batched=round(t_batch / args.steps * 1e3, 3),
+        parity_ok=parity_ok,
+        roofline={"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                  "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                  "kernel": f"batch_pass<4> (RS(16,4) encode of {nst} stripes, {launch_bytes} B per launch)",
+                  "avg_launch_us": round(avg * 1e6, 2)},
+        plain_encode_launch={"kernel": "gf8_pass_vec16<4> (one stripe)", "avg_us": round(avg_plain * 1e6, 2),
+                             "gbps": round(n * S / avg_plain / 1e9, 1)},
+        tuning={"tile_bytes": tile, "blocks_per_cu": bpc},
+    )
+    del st, benc, brec
+    torch.cuda.empty_cache()
+    return out
+
+
+def mixed(args, torch, rs):
+    """8192 mixed stripes: one batched plan vs one call per stripe."""
+    lib = rs.lib
+    shapes_all = [(4, 2), (6, 3), (10, 4), (17, 3), (32, 8)]
+    rng = np.random.default_rng(SEED + 3)
+    nst = 8192
+    shapes = [shapes_all[int(i)] for i in rng.integers(0, 5, nst)]
+    sizes = [max(64, int(round(np.exp(rng.uniform(np.log(4096), np.log(1 << 20))) / 64)) * 64)
+             for _ in range(nst)]
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(SEED + 3)
+    st = Stripes(shapes, sizes, torch, gen)
+    codecs = {s: rs.New(*s) for s in shapes_all}
+    # make every stripe a valid codeword first (reconstructs need one)
+    b0 = rs.Batch(0)
+    plan(rs, b0, batch_items(rs, st, codecs, [rs.OP_ENCODE] * nst, st.full), nst)
+    b0.launch()
+    b0.synchronize()
+    del b0
+    ops, lens, total = [], [], 0
+    for i, ((k, m), s) in enumerate(zip(shapes, sizes)):
+        n = k + m
+        total += n * s
+        if rng.random() < 0.3:
+            e = int(rng.integers(1, m + 1))
+            lost = set(rng.choice(n, e, replace=False).tolist())
+            ops.append(rs.OP_RECONSTRUCT)
+            lens.append((ctypes.c_size_t * n)(*[0 if j in lost else s for j in range(n)]))
+        else:
+            ops.append(rs.OP_ENCODE)
+            lens.append(st.full[i])
+    bat = rs.Batch(0)
+    t_plan0 = time.perf_counter()
+    plan(rs, bat, batch_items(rs, st, codecs, ops, lens), nst)
+    t_plan = time.perf_counter() - t_plan0
+    handles = [codecs[s].handle for s in shapes]
+    filled = (ctypes.c_uint8 * 64)()
+
+    def plain_pass():
+        for i in range(nst):
+            n = shapes[i][0] + shapes[i][1]
+            if ops[i] == rs.OP_ENCODE:
+                _check(lib, lib.rsx_encode(handles[i], st.ptrs[i], lens[i], n))
+            else:
+                _check(lib, lib.rsx_reconstruct(handles[i], st.ptrs[i], lens[i], n, filled))
+
+    def sync_all():
+        for c in codecs.values():
+            c.Synchronize()
+
+    plain_pass()
+    sync_all()
+    bat.launch()
+    bat.synchronize()
+    reps = max(2, args.steps // 3)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        plain_pass()
+    sync_all()
+    t_plain = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        bat.launch()
+    bat.synchronize()
+    t_bat = (time.perf_counter() - t0) / reps
+    _log(f"bench: mixed plain {total / t_plain / 1e9:.1f} GB/s, batched {total / t_bat / 1e9:.1f} GB/s")
+    tile, bpc = bat.tuning()
+    out = dict(stripes=nst, bytes_per_pass=total, plain_gbps=round(total / t_plain / 1e9, 1),
+               batched_gbps=round(total / t_bat / 1e9, 1), batched_over_plain=round(t_plain / t_bat, 2),
+               ms_plain=round(t_plain * 1e3, 3), ms_batched=round(t_bat * 1e3, 3),
+               plan_ms=round(t_plan * 1e3, 1), tuning={"tile_bytes": tile, "blocks_per_cu": bpc},
+               reconstruct_fraction=round(ops.count(rs.OP_RECONSTRUCT) / nst, 3))
+    del st, bat
+    torch.cuda.empty_cache()
+    return out
+
+
+def leopard(args, torch, rs):
+    """Leopard GF(2^16) k=4096, m=1024, 64 KiB: encode + reconstruct 1024 erasures."""
+    lib = rs.lib
+    k, m, S = 4096, 1024, 64 << 10
+    n = k + m
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(4096)
+    buf = torch.empty((n, S), dtype=torch.uint8, device="cuda")
+    buf[:k].random_(0, 256, generator=gen)
+    buf[k:].zero_()
+    enc = rs.New(k, m, rs.WithLeopardGF16(True))
+    h = enc.handle
+    base = buf.data_ptr()
+    ptrs = (ctypes.c_void_p * n)(*[base + i * S for i in range(n)])
+    full = (ctypes.c_size_t * n)(*([S] * n))
+    rng = np.random.default_rng(1024)
+    lost = set(rng.choice(n, m, replace=False).tolist())
+    lens = (ctypes.c_size_t * n)(*[0 if i in lost else S for i in range(n)])
+    filled = (ctypes.c_uint8 * n)()
+    _check(lib, lib.rsx_encode(h, ptrs, full, n))
+    _check(lib, lib.rsx_reconstruct(h, ptrs, lens, n, filled))
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        _check(lib, lib.rsx_encode(h, ptrs, full, n))
+    t_enc = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        _check(lib, lib.rsx_reconstruct(h, ptrs, lens, n, filled))
+    t_rec = (time.perf_counter() - t0) / reps
+    b = n * S
+    _log(f"bench: leopard16 encode {b / t_enc / 1e9:.1f} GB/s, reconstruct {b / t_rec / 1e9:.1f} GB/s")
+    out = dict(k=k, m=m, shard_bytes=S, erasures=m, encode_ms=round(t_enc * 1e3, 2),
+               reconstruct_ms=round(t_rec * 1e3, 2), encode_gbps=round(b / t_enc / 1e9, 1),
+               reconstruct_gbps=round(b / t_rec / 1e9, 1),
+               encode_plus_reconstruct_gbps=round(2 * b / (t_enc + t_rec) / 1e9, 1))
+    del buf
+    torch.cuda.empty_cache()
+    return out
+
+
+def cpu_baseline(seconds: float, shard_bytes: int):
+    """The oracle's SIMD restatement of the reference CPU path on the host
+    cores: RS(16,4) encode (16 -> 4) and an equal-cost reconstruct product
+    (16 survivors -> 4 rebuilt) on one stripe, alternated for ~`seconds`."""
+    try:
+        from oracle import rsref
+    except Exception as ex:
+        _log(f"bench: cpu baseline skipped: {ex}")
+        return None
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
+    k, m = 16, 4
+    S = shard_bytes
+    rng = np.random.default_rng(1)
+    data = [rng.integers(0, 256, S, dtype=np.uint8) for _ in range(k)]
+    outs = [np.zeros(S, np.uint8) for _ in range(m)]
+    rows_enc = rsref.codec_matrix(k, m)[k:]
+    rows_rec = rng.integers(1, 256, (m, k), dtype=np.uint8)
+    isa = rsref.fast_isa()
+    rsref.code_some_shards_fast(rows_enc, data, outs, threads, isa)
+    t0 = time.perf_counter()
+    reps = 0
+    while time.perf_counter() - t0 < seconds:
+        rsref.code_some_shards_fast(rows_enc if reps % 2 == 0 else rows_rec, data, outs, threads, isa)
+        reps += 1
+    el = time.perf_counter() - t0
+    cpu_model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    cpu_model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": round(reps * (k + m) * S / el / 1e9, 2), "unit": "GB/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{reps} alternating RS(16,4) encode / 4-erasure reconstruct products of one "
+                      f"stripe of {S} B shards in {el:.1f} s; {['scalar', 'AVX2 PSHUFB', 'AVX-512 GFNI'][isa]} "
+                      f"restatement of the reference kernels, {threads} threads, {cpu_model}"}
 
 
 def main() -> int:
@@ -60,166 +394,48 @@ def main() -> int:
     import torch
     import reedsolomon_amd as rs
 
-    k, m = args.k, args.m
-    n = k + m
-    S = int(args.shard_mib * (1 << 20))
-    S -= S % 64
-    nst = args.stripes
-    _log(f"bench: RS({k},{m}) stripes={nst} shard={S} B, allocating {nst * n * S / 2**30:.1f} GiB")
-
     torch.cuda.set_device(0)
-    enc = rs.New(k, m, rs.WithMI355X(0))
-    h = enc.handle
-    lib = rs.lib
-    stream_ptr = lib.rsx_get_stream(h)
-
-    gen = torch.Generator(device="cuda")
-    gen.manual_seed(args.seed & 0x7FFFFFFFFFFFFFFF)
-    buf = torch.empty((nst, n, S), dtype=torch.uint8, device="cuda")
-    buf[:, :k].random_(0, 256, generator=gen)
-    buf[:, k:].zero_()
-    torch.cuda.synchronize()
-    base = buf.data_ptr()
-
-    # Per-stripe pointer/length arrays for the C ABI.
-    PtrArr = ctypes.c_void_p * n
-    LenArr = ctypes.c_size_t * n
-    ptrs = []
-    full_lens = LenArr(*([S] * n))
-    for s in range(nst):
-        ptrs.append(PtrArr(*[base + (s * n + i) * S for i in range(n)]))
-    rng = np.random.default_rng(args.seed)
-    erase_lens = []
-    rec_bytes = 0
-    for s in range(nst):
-        e = int(rng.integers(0, m + 1))
-        lost = rng.choice(n, e, replace=False) if e else []
-        lens = LenArr(*[0 if i in set(np.asarray(lost).tolist()) else S for i in range(n)])
-        erase_lens.append((lens, e))
-        if e:
-            rec_bytes += n * S
-    filled = (ctypes.c_uint8 * n)()
-
-    def check(rc):
-        if rc != 0:
-            raise RuntimeError(f"rsx error {rc}: {lib.rsx_last_error().decode()}")
-
-    def step():
-        for s in range(nst):
-            check(lib.rsx_encode(h, ptrs[s], full_lens, n))
-        for s in range(nst):
-            lens, e = erase_lens[s]
-            if e:
-                check(lib.rsx_reconstruct(h, ptrs[s], lens, n, filled))
-
-    # Parity check of one column slice of stripe 0 against the CPU oracle
-    # (encode is column-separable, so a slice is a complete sub-problem).
-    for _ in range(max(args.warmup, 1)):
-        step()
-    check(lib.rsx_synchronize(h))
-    parity_ok = None
-    try:
-        from oracle import rsref
-        sl = slice(0, 1 << 16)
-        stripe0 = buf[0, :, sl].cpu().numpy()
-        o = rsref.Classic(k, m)
-        ref = [stripe0[i].copy() for i in range(k)] + [np.zeros(sl.stop, np.uint8) for _ in range(m)]
-        o.encode(ref)
-        parity_ok = all(np.array_equal(ref[k + i], stripe0[k + i]) for i in range(m))
-    except Exception as ex:  # the oracle is a checker only
-        _log(f"bench: oracle check skipped: {ex}")
-
-    # Timed region: K steps, bracketed by device synchronisation.
-    torch.cuda.synchronize()
-    check(lib.rsx_synchronize(h))
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    check(lib.rsx_synchronize(h))
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    bytes_per_step = nst * n * S + rec_bytes
-    gbps = bytes_per_step * args.steps / elapsed / 1e9
-    _log(f"bench: {args.steps} steps in {elapsed:.3f}s -> {gbps:.1f} GB/s")
-
-    # Dominant kernel: one encode launch per call (16 in -> 4 out).  Time each
-    # launch with HIP events on the codec's stream.
-    stream = torch.cuda.ExternalStream(stream_ptr)
-    nlaunch = min(nst, 128)
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-           for _ in range(nlaunch)]
-    for s in range(nlaunch):
-        evs[s][0].record(stream)
-        check(lib.rsx_encode(h, ptrs[s], full_lens, n))
-        evs[s][1].record(stream)
-    check(lib.rsx_synchronize(h))
-    durs = [a.elapsed_time(b) * 1e-3 for a, b in evs]
-    avg_launch = float(np.mean(durs))
-    enc_bytes = n * S
-    achieved = enc_bytes / avg_launch / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
-                "kernel": f"gf8_pass_vec16<{m}> (RS({k},{m}) encode, {enc_bytes} B per launch)",
-                "avg_launch_us": round(avg_launch * 1e6, 2)}
-
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(k, m, S, args.cpu_seconds)
-
+    head = headline(args, torch, rs)
+    mix = leo = None
+    if not args.quick:
+        mix = mixed(args, torch, rs)
+        leo = leopard(args, torch, rs)
+    cpu = None if args.no_cpu else cpu_baseline(args.cpu_seconds, 4 << 20)
     out = {
         "metric": METRIC,
-        "value": round(gbps, 2),
+        "value": head["batched_gbps"],
         "unit": "GB/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "ms_per_step": head["ms_batched"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u8",
         "data": "synthetic",
         "config": {
-            "workload": f"plain GPU RS({k},{m}) GF(2^8): {nst} stripes x {k}x{S // (1 << 20)} MiB "
-                        f"data shards; encode all, then reconstruct each after 0-{m} seeded erasures",
-            "k": k, "m": m, "shard_bytes": S, "stripes": nst,
-            "path": "plain (one C-ABI call per stripe, device-resident shards)",
-            "bytes_per_step": bytes_per_step,
+            "workload": f"RS(16,4) GF(2^8): {args.stripes} stripes x 16 x {args.shard_mib:g} MiB data "
+                        "shards; encode all, then reconstruct each after 0-4 seeded erasures",
+            "k": 16, "m": 4, "shard_bytes": int(args.shard_mib * (1 << 20)) // 64 * 64,
+            "stripes": args.stripes, "path": "batched (planned work list, HIP-graph replay)",
+            "bytes_per_step": head["bytes_per_step"],
         },
-        "pct_hbm_peak": round(100.0 * gbps / HBM_PEAK_GBPS, 2),
-        "parity_check_vs_oracle": parity_ok,
-        "roofline": roofline,
+        "pct_hbm_peak": round(100.0 * head["batched_gbps"] / HBM_PEAK_GBPS, 2),
+        "plain": {"gbps": head["plain_gbps"], "ms_per_step": head["ms_plain"],
+                  "pct_hbm_peak": round(100.0 * head["plain_gbps"] / HBM_PEAK_GBPS, 2),
+                  "path": "one C-ABI call per stripe"},
+        "batched_over_plain": round(head["batched_gbps"] / head["plain_gbps"], 2),
+        "parity_check_vs_oracle": head["parity_ok"],
+        "roofline": head["roofline"],
+        "plain_encode_launch": head["plain_encode_launch"],
+        "tuning": head["tuning"],
+        "mixed_workload": mix,
+        "leopard16": leo,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
     return 0
-
-
-def cpu_baseline(k: int, m: int, S: int, seconds: float):
-    """The CPU oracle (scalar restatement, one thread) on a bounded sample:
-    repeated RS(k,m) encodes of one stripe of 1 MiB shards."""
-    try:
-        from oracle import rsref
-    except Exception as ex:
-        _log(f"bench: cpu baseline skipped: {ex}")
-        return None
-    size = min(S, 1 << 20)
-    rng = np.random.default_rng(1)
-    shards = [rng.integers(0, 256, size, dtype=np.uint8) for _ in range(k)] + \
-             [np.zeros(size, np.uint8) for _ in range(m)]
-    o = rsref.Classic(k, m)
-    t0 = time.perf_counter()
-    reps = 0
-    while True:
-        o.encode(shards)
-        reps += 1
-        el = time.perf_counter() - t0
-        if el >= seconds:
-            break
-    gbps = reps * (k + m) * size / el / 1e9
-    return {"value": round(gbps, 3), "unit": "GB/s", "cores": 1, "kind": "port",
-            "sample": f"{reps} RS({k},{m}) encodes of one stripe of {size} B shards "
-                      f"({el:.1f} s, scalar oracle, 1 thread)"}
 
 
 if __name__ == "__main__":

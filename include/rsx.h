@@ -157,6 +157,8 @@ void rsx_batch_free(rsx_batch *b);
 int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int nitems, int *status,
                    uint8_t *filled);
 int rsx_batch_launch(rsx_batch *b);       /* stream-ordered on rsx_batch_stream(b) */
+/* Replays the plan on another stream (e.g. to order two plans). */
+int rsx_batch_launch_on(rsx_batch *b, void *hip_stream);
 int rsx_batch_synchronize(rsx_batch *b);
 void *rsx_batch_stream(rsx_batch *b);
 int rsx_batch_num_tasks(const rsx_batch *b);

@@ -77,6 +77,12 @@ int rsref_reconstruct(const uint8_t *matrix, int k, int m, uint8_t *const *shard
                       const size_t *lens, int nshards, int data_only, const uint8_t *required,
                       int nrequired, uint8_t *filled);
 
+/* Fast path of the classic product for the CPU baseline (rsref_fast.c):
+ * isa 0 scalar, 1 AVX2 PSHUFB, 2 AVX-512+GFNI, -1 = best available. */
+int rsref_fast_isa(void);
+void rsref_code_some_shards_fast(const uint8_t *rows, int nout, int nin, const uint8_t *const *in,
+                                 uint8_t *const *out, size_t len, int threads, int isa);
+
 /* ---- Leopard GF(2^16) (leopard.go) ---- */
 void rsref_leo16_tables(uint16_t *exp_lut, uint16_t *log_lut, uint16_t *fft_skew /*65535*/,
                         uint16_t *log_walsh);

@@ -134,6 +134,20 @@ def matrix_invert(m: np.ndarray):
     return err, out
 
 
+def fast_isa() -> int:
+    return lib().rsref_fast_isa()
+
+
+def code_some_shards_fast(rows: np.ndarray, inputs, outputs, threads: int = 1, isa: int = -1) -> None:
+    """The reference's SIMD path (rsref_fast.c): out = rows * in over `threads`."""
+    rows = np.ascontiguousarray(rows, np.uint8)
+    ins = _Shards(inputs)
+    outs = _Shards(outputs)
+    lib().rsref_code_some_shards_fast(_ptr(rows), rows.shape[0], rows.shape[1], ins.ptrs, outs.ptrs,
+                                      ctypes.c_size_t(len(inputs[0])), ctypes.c_int(threads),
+                                      ctypes.c_int(isa))
+
+
 def codec_matrix(k: int, m: int, kind: int = MATRIX_DEFAULT, custom: np.ndarray | None = None):
     out = np.zeros((k + m, k), np.uint8)
     if custom is not None:

@@ -26,7 +26,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "librsx.so")
 
 if not os.path.exists(LIB_PATH):
-    raise ImportError(f"{LIB_PATH} is missing: build it with `python -m reedsolomon_amd.build`")
+    raise ImportError(f"{LIB_PATH} is missing: build it with `python reedsolomon_amd/build.py`")
 
 # PyTorch-ROCm ships its own libamdhip64 (soname libamdhip64.so.7).  Loading
 # torch first makes librsx.so bind to that same runtime, so torch tensors and
@@ -363,6 +363,7 @@ lib.rsx_batch_free.argtypes = [_P]
 lib.rsx_batch_plan.argtypes = [_P, ctypes.POINTER(_BatchItem), ctypes.c_int,
                                ctypes.POINTER(ctypes.c_int), _U8]
 lib.rsx_batch_launch.argtypes = [_P]
+lib.rsx_batch_launch_on.argtypes = [_P, _P]
 lib.rsx_batch_synchronize.argtypes = [_P]
 lib.rsx_batch_stream.argtypes = [_P]
 lib.rsx_batch_stream.restype = _P
@@ -444,8 +445,11 @@ class Batch:
         self._keep = [keep, arr, items]
         return [status[i] for i in range(n)]
 
-    def launch(self) -> None:
-        _check(lib.rsx_batch_launch(self._h))
+    def launch(self, stream=None) -> None:
+        if stream is None:
+            _check(lib.rsx_batch_launch(self._h))
+        else:
+            _check(lib.rsx_batch_launch_on(self._h, stream))
 
     def synchronize(self) -> None:
         _check(lib.rsx_batch_synchronize(self._h))

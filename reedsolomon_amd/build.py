@@ -1,6 +1,6 @@
 """Builds reedsolomon_amd/librsx.so (HIP for gfx950) in-tree with hipcc.
 
-    python -m reedsolomon_amd.build
+    python reedsolomon_amd/build.py
 
 Every .hip / .cpp file under csrc/ is compiled to an object for gfx950 and
 linked into one shared library exporting the C ABI of include/rsx.h.

@@ -372,6 +372,13 @@ int rsx_batch_launch(rsx_batch *b) {
     return RSX_OK;
 }
 
+int rsx_batch_launch_on(rsx_batch *b, void *hip_stream) {
+    if (!b->planned) return fail(RSX_ERR_INVALID_INPUT, "batch has no plan");
+    (void)hipSetDevice(b->device);
+    RSX_HIP(hipGraphLaunch(b->exec, hip_stream ? static_cast<hipStream_t>(hip_stream) : b->stream));
+    return RSX_OK;
+}
+
 int rsx_batch_synchronize(rsx_batch *b) {
     (void)hipSetDevice(b->device);
     RSX_HIP(hipStreamSynchronize(b->stream));

@@ -215,3 +215,21 @@ def test_encode_idx_equals_encode():
         assert codec.encode_idx(shards[idx], idx, parity) == 0
     for i in range(m):
         assert (parity[i] == shards[k + i]).all()
+
+
+@pytest.mark.parametrize("isa", [0, 1, 2])
+def test_fast_cpu_path_equals_scalar(isa):
+    # The SIMD restatement used for the CPU baseline must give the scalar
+    # oracle's bytes for every tile shape (k, m up to and beyond 10 x 10).
+    if isa > rsref.fast_isa():
+        pytest.skip("ISA not available on this host")
+    rng = np.random.default_rng(isa)
+    for k, m, size in [(1, 1, 31), (10, 4, 4097), (16, 4, 65536 + 77), (17, 3, 999), (32, 12, 2048)]:
+        rows = rng.integers(0, 256, (m, k), dtype=np.uint8)
+        data = [rng.integers(0, 256, size, dtype=np.uint8) for _ in range(k)]
+        want = [np.zeros(size, np.uint8) for _ in range(m)]
+        rsref.code_some_shards(rows, data, want)
+        got = [np.zeros(size, np.uint8) for _ in range(m)]
+        rsref.code_some_shards_fast(rows, data, got, threads=3, isa=isa)
+        for i in range(m):
+            assert np.array_equal(got[i], want[i]), (k, m, size, i)
