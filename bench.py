@@ -52,6 +52,10 @@ def _parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline workload only")
+    ap.add_argument("--tile", type=int, default=0, help="batched tile bytes (0 = autotune)")
+    ap.add_argument("--bpc", type=int, default=0, help="batched blocks per CU (0 = autotune)")
+    ap.add_argument("--encode-only", action="store_true",
+                    help="time only the batched RS(16,4) encode launches (for kernel profiling)")
     return ap.parse_args()
 
 
@@ -157,10 +161,23 @@ def headline(args, torch, rs):
     # batched: one plan for the encodes, one for the reconstructs, replayed in
     # order on the encode plan's stream.
     benc, brec = rs.Batch(0), rs.Batch(0)
+    if args.tile and args.bpc:
+        benc.set_tuning(args.tile, args.bpc)
+        brec.set_tuning(args.tile, args.bpc)
     plan(rs, benc, batch_items(rs, st, {(k, m): enc}, [rs.OP_ENCODE] * nst, st.full), nst)
     rec_ops = [rs.OP_RECONSTRUCT] * nst
     plan(rs, brec, batch_items(rs, st, {(k, m): enc}, rec_ops, [e[0] for e in erase]), nst)
     sbat = benc.stream
+
+    if args.encode_only:
+        for _ in range(max(args.warmup, 1)):
+            benc.launch()
+        benc.synchronize()
+        avg = event_time(torch, sbat, benc.launch, args.steps)
+        launch_bytes = nst * n * S
+        return dict(encode_only=True, avg_launch_us=round(avg * 1e6, 2),
+                    achieved_gbps=round(launch_bytes / avg / 1e9, 1), launch_bytes=launch_bytes,
+                    tuning=dict(zip(("tile_bytes", "blocks_per_cu"), benc.tuning())))
 
     def batched_step():
         benc.launch()
@@ -396,6 +413,9 @@ def main() -> int:
 
     torch.cuda.set_device(0)
     head = headline(args, torch, rs)
+    if args.encode_only:
+        print(json.dumps({"kernel": "batch_pass<4>", **head}), flush=True)
+        return 0
     mix = leo = None
     if not args.quick:
         mix = mixed(args, torch, rs)

@@ -10,6 +10,7 @@
 // complete sub-problem (the product is column-separable), so tiles need no
 // ordering.
 #include "batch.hpp"
+#include "gf8_device.hpp"
 
 namespace rsx {
 
@@ -17,62 +18,7 @@ namespace {
 
 constexpr int kBlock = 256;
 
-struct Split3 {
-    uint32_t g0, g1, g2;
-};
-__device__ __forceinline__ Split3 split(uint32_t x) {
-    return {x & 0x07070707u, (x >> 3) & 0x07070707u, (x >> 6) & 0x03030303u};
-}
-__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
-    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
-}
-__device__ __forceinline__ uint32_t sel(uint32_t hi, uint32_t lo, uint32_t idx) {
-    return __builtin_amdgcn_perm(hi, lo, idx);
-}
-
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-
-template <int M>
-__device__ __forceinline__ void mac_pair(uint32_t (&acc)[M][4], const uint4 &xa, const uint4 &xb,
-                                         const uint4 *lo, const uint32_t *hi, int ja, int jb) {
-    const Split3 a[4] = {split(xa.x), split(xa.y), split(xa.z), split(xa.w)};
-    const Split3 b[4] = {split(xb.x), split(xb.y), split(xb.z), split(xb.w)};
-#pragma unroll
-    for (int i = 0; i < M; ++i) {
-        const uint4 ta = lo[ja * M + i], tb = lo[jb * M + i];
-        const uint32_t ha = hi[ja * M + i], hb = hi[jb * M + i];
-#pragma unroll
-        for (int d = 0; d < 4; ++d) {
-            uint32_t r = acc[i][d];
-            r = xor3(r, sel(ta.y, ta.x, a[d].g0), sel(ta.w, ta.z, a[d].g1));
-            r = xor3(r, sel(ha, ha, a[d].g2), sel(tb.y, tb.x, b[d].g0));
-            acc[i][d] = xor3(r, sel(tb.w, tb.z, b[d].g1), sel(hb, hb, b[d].g2));
-        }
-    }
-}
-
-template <int M>
-__device__ __forceinline__ void mac_one(uint32_t (&acc)[M][4], const uint4 &xa, const uint4 *lo,
-                                        const uint32_t *hi, int ja) {
-    const Split3 a[4] = {split(xa.x), split(xa.y), split(xa.z), split(xa.w)};
-#pragma unroll
-    for (int i = 0; i < M; ++i) {
-        const uint4 ta = lo[ja * M + i];
-        const uint32_t ha = hi[ja * M + i];
-#pragma unroll
-        for (int d = 0; d < 4; ++d) {
-            const uint32_t r = xor3(acc[i][d], sel(ta.y, ta.x, a[d].g0), sel(ta.w, ta.z, a[d].g1));
-            acc[i][d] = r ^ sel(ha, ha, a[d].g2);
-        }
-    }
-}
-
-__device__ __forceinline__ uint32_t mul_byte(uint32_t x, const uint4 &t, uint32_t h) {
-    const Split3 s = split(x);
-    return sel(t.y, t.x, s.g0) ^ sel(t.w, t.z, s.g1) ^ sel(h, h, s.g2);
-}
-
-constexpr int kGroup = 8;
+using namespace gf8dev;
 
 // Upper bound search: last task whose first tile is <= t.
 __device__ __forceinline__ int find_task(const BatchTask *tasks, int ntasks, uint32_t t) {
@@ -87,9 +33,13 @@ __device__ __forceinline__ int find_task(const BatchTask *tasks, int ntasks, uin
 
 template <int M>
 __global__ __launch_bounds__(kBlock) void batch_pass(const BatchLaunch L) {
+    constexpr int R = Shape<M>::R;
+    constexpr uint64_t per_chunk = static_cast<uint64_t>(R) * 64;  // vectors per wave-chunk
     extern __shared__ uint4 lds[];
     uint4 *lo = lds;
     uint32_t *hi = reinterpret_cast<uint32_t *>(lds + L.maxk * M);
+    const Tables tab{lo, hi};
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int cached = -1;
     for (uint32_t t = blockIdx.x; t < L.ntiles; t += gridDim.x) {
         const int ti = find_task(L.tasks, L.ntasks, t);
@@ -97,63 +47,31 @@ __global__ __launch_bounds__(kBlock) void batch_pass(const BatchLaunch L) {
         const int k = task.k;
         if (ti != cached) {
             __syncthreads();
-            for (int q = threadIdx.x; q < k * M; q += kBlock) {
-                const int j = q / M, i = q % M;
-                const uint32_t *src = task.tables + (static_cast<size_t>(j) * task.mtot + task.o0 + i) * 5;
-                lo[q] = i < task.nout ? make_uint4(src[0], src[1], src[2], src[3]) : make_uint4(0, 0, 0, 0);
-                hi[q] = i < task.nout ? src[4] : 0u;
-            }
+            stage_tables<M>(lo, hi, task.tables, k, task.mtot, task.o0, task.nout);
             __syncthreads();
             cached = ti;
         }
         const uint8_t *const *in = L.ptrs + task.in_off;
         uint8_t *const *out = (uint8_t *const *)(L.ptrs + task.out_off);
+        const bool acc = task.accumulate != 0;
         const uint64_t b0 = static_cast<uint64_t>(t - task.tile0) * L.tile;
         const uint64_t b1 = b0 + L.tile < task.len ? b0 + L.tile : task.len;
-        const uint64_t vend = task.aligned ? (b1 >> 4) : (b0 >> 4);
-        for (uint64_t v = (b0 >> 4) + threadIdx.x; v < vend; v += kBlock) {
-            uint32_t acc[M][4];
-#pragma unroll
-            for (int i = 0; i < M; ++i) {
-                if (task.accumulate && i < task.nout) {
-                    const uint4 o = reinterpret_cast<const uint4 *>(out[i])[v];
-                    acc[i][0] = o.x;
-                    acc[i][1] = o.y;
-                    acc[i][2] = o.z;
-                    acc[i][3] = o.w;
-                } else {
-                    acc[i][0] = acc[i][1] = acc[i][2] = acc[i][3] = 0;
-                }
-            }
-            uint4 cur[kGroup], nxt[kGroup];
-#pragma unroll
-            for (int u = 0; u < kGroup; ++u)
-                if (u < k) cur[u] = reinterpret_cast<const uint4 *>(in[u])[v];
-            for (int j0 = 0; j0 < k; j0 += kGroup) {
-                const int jn = j0 + kGroup;
-#pragma unroll
-                for (int u = 0; u < kGroup; ++u)
-                    if (jn + u < k) nxt[u] = reinterpret_cast<const uint4 *>(in[jn + u])[v];
-#pragma unroll
-                for (int u = 0; u < kGroup; u += 2) {
-                    if (j0 + u + 1 < k) mac_pair<M>(acc, cur[u], cur[u + 1], lo, hi, j0 + u, j0 + u + 1);
-                    else if (j0 + u < k) mac_one<M>(acc, cur[u], lo, hi, j0 + u);
-                }
-#pragma unroll
-                for (int u = 0; u < kGroup; ++u) cur[u] = nxt[u];
-            }
-#pragma unroll
-            for (int i = 0; i < M; ++i)
-                if (i < task.nout)
-                    __builtin_nontemporal_store(u32x4{acc[i][0], acc[i][1], acc[i][2], acc[i][3]},
-                                                reinterpret_cast<u32x4 *>(out[i]) + v);
+        uint64_t vend = b0 >> 4;  // first vector not done by the vector paths
+        if (task.aligned) {
+            const uint64_t v0 = b0 >> 4, v1 = b1 >> 4;
+            const uint64_t nchunk = (v1 - v0) / per_chunk;
+            for (uint64_t c = wave; c < nchunk; c += kBlock / 64)
+                chunk<M, R, Shape<M>::G>(in, out, k, task.nout, acc, tab, v0 + c * per_chunk, lane);
+            for (uint64_t v = v0 + nchunk * per_chunk + threadIdx.x; v < v1; v += kBlock)
+                vector1<M>(in, out, k, task.nout, acc, tab, v);
+            vend = v1;
         }
-        // Bytes past the last full vector (or every byte of an unaligned task).
+        // Bytes past the last whole vector (every byte of an unaligned task).
         for (uint64_t b = (vend << 4) + threadIdx.x; b < b1; b += kBlock) {
             for (int i = 0; i < task.nout; ++i) {
-                uint32_t acc = task.accumulate ? out[i][b] : 0u;
-                for (int j = 0; j < k; ++j) acc ^= mul_byte(in[j][b], lo[j * M + i], hi[j * M + i]);
-                out[i][b] = static_cast<uint8_t>(acc);
+                uint32_t a = acc ? out[i][b] : 0u;
+                for (int j = 0; j < k; ++j) a ^= mul_byte(in[j][b], lo[j * M + i], hi[j * M + i]);
+                out[i][b] = static_cast<uint8_t>(a);
             }
         }
     }

@@ -12,6 +12,10 @@
 // (GF(2)-linearity), XORed into the accumulator.
 #include "kernels.hpp"
 
+#include "gf8_device.hpp"
+
+#include <algorithm>
+
 namespace rsx {
 
 namespace {
@@ -33,121 +37,29 @@ __device__ __forceinline__ uint32_t mul4(const Split3 &s, uint32_t t0, uint32_t 
            __builtin_amdgcn_perm(t4, t4, s.g2);
 }
 
-// XOR of three words in one VALU op (gfx950 v_bitop3_b32, truth table 0x96).
-__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
-    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
-}
-
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ uint4 load16(const uint8_t *base, size_t v) {
-    return reinterpret_cast<const uint4 *>(base)[v];
-}
-
-// Coefficient tables staged in LDS for the whole block: per (input j,
-// output i) one 16-byte record {c*(0..3), c*(4..7), c*(0..3)<<3, c*(4..7)<<3}
-// read with one ds_read_b128 straight into VGPRs (v_perm_b32 can read only
-// one SGPR, so SGPR-resident tables would cost two v_mov per coefficient),
-// plus the 4-entry table of c*(x & 0xC0) in a second array.
-struct LdsTables {
-    const uint4 *lo;
-    const uint32_t *hi;
-};
-
-// acc[i] ^= c[i][a]*xa ^ c[i][b]*xb for 4 dwords: 6 byte selects folded into
-// the accumulator with 3 three-input XORs.
-template <int M>
-__device__ __forceinline__ void mac_pair(uint32_t (&acc)[M][4], const uint4 &xa, const uint4 &xb,
-                                         const LdsTables &t, int ja, int jb) {
-    const Split3 a[4] = {split(xa.x), split(xa.y), split(xa.z), split(xa.w)};
-    const Split3 b[4] = {split(xb.x), split(xb.y), split(xb.z), split(xb.w)};
-#pragma unroll
-    for (int i = 0; i < M; ++i) {
-        const uint4 ta = t.lo[ja * M + i], tb = t.lo[jb * M + i];
-        const uint32_t ha = t.hi[ja * M + i], hb = t.hi[jb * M + i];
-#pragma unroll
-        for (int d = 0; d < 4; ++d) {
-            uint32_t r = acc[i][d];
-            r = xor3(r, __builtin_amdgcn_perm(ta.y, ta.x, a[d].g0), __builtin_amdgcn_perm(ta.w, ta.z, a[d].g1));
-            r = xor3(r, __builtin_amdgcn_perm(ha, ha, a[d].g2), __builtin_amdgcn_perm(tb.y, tb.x, b[d].g0));
-            acc[i][d] = xor3(r, __builtin_amdgcn_perm(tb.w, tb.z, b[d].g1), __builtin_amdgcn_perm(hb, hb, b[d].g2));
-        }
-    }
-}
-
-template <int M>
-__device__ __forceinline__ void mac_one(uint32_t (&acc)[M][4], const uint4 &xa, const LdsTables &t, int ja) {
-    const Split3 a[4] = {split(xa.x), split(xa.y), split(xa.z), split(xa.w)};
-#pragma unroll
-    for (int i = 0; i < M; ++i) {
-        const uint4 ta = t.lo[ja * M + i];
-        const uint32_t ha = t.hi[ja * M + i];
-#pragma unroll
-        for (int d = 0; d < 4; ++d) {
-            const uint32_t r = xor3(acc[i][d], __builtin_amdgcn_perm(ta.y, ta.x, a[d].g0),
-                                    __builtin_amdgcn_perm(ta.w, ta.z, a[d].g1));
-            acc[i][d] = r ^ __builtin_amdgcn_perm(ha, ha, a[d].g2);
-        }
-    }
-}
-
-// Inputs are fetched in groups of kGroup 16-byte loads per lane, and the next
-// group is issued before the current one is consumed, so each wave keeps up
-// to 2*kGroup HBM requests in flight instead of one per input.
-constexpr int kGroup = 8;
-
-template <int M>
-__global__ __launch_bounds__(kBlock) void gf8_pass_vec16(const Gf8Pass p) {
+// Streaming pass: each wave owns R consecutive KiB-wide column chunks and
+// walks all k inputs (gf8_device.hpp); vectors past the last whole chunk (all
+// of them when R == 0) are done one per lane.
+template <int M, int R>
+__global__ __launch_bounds__(kBlock) void gf8_pass_stream(const Gf8Pass p) {
     extern __shared__ uint4 lds[];
-    const int k = p.k;
     uint4 *lo = lds;
-    uint32_t *hi = reinterpret_cast<uint32_t *>(lds + k * M);
-    for (int t = threadIdx.x; t < k * M; t += kBlock) {
-        const uint32_t *src = p.tables + (static_cast<size_t>(t / M) * p.mtot + p.o0 + t % M) * 5;
-        lo[t] = make_uint4(src[0], src[1], src[2], src[3]);
-        hi[t] = src[4];
-    }
+    uint32_t *hi = reinterpret_cast<uint32_t *>(lds + p.k * M);
+    gf8dev::stage_tables<M>(lo, hi, p.tables, p.k, p.mtot, p.o0, p.nout);
     __syncthreads();
-    const LdsTables tab{lo, hi};
-
-    const size_t nvec = p.len >> 4;
-    const size_t stride = static_cast<size_t>(gridDim.x) * kBlock;
-    for (size_t v = static_cast<size_t>(blockIdx.x) * kBlock + threadIdx.x; v < nvec; v += stride) {
-        uint32_t acc[M][4];
-#pragma unroll
-        for (int i = 0; i < M; ++i) {
-            if (p.accumulate) {
-                const uint4 o = load16(p.out[i], v);
-                acc[i][0] = o.x;
-                acc[i][1] = o.y;
-                acc[i][2] = o.z;
-                acc[i][3] = o.w;
-            } else {
-                acc[i][0] = acc[i][1] = acc[i][2] = acc[i][3] = 0;
-            }
-        }
-        uint4 cur[kGroup], nxt[kGroup];
-#pragma unroll
-        for (int u = 0; u < kGroup; ++u)
-            if (u < k) cur[u] = load16(p.in[u], v);
-        for (int j0 = 0; j0 < k; j0 += kGroup) {
-            const int jn = j0 + kGroup;
-#pragma unroll
-            for (int u = 0; u < kGroup; ++u)
-                if (jn + u < k) nxt[u] = load16(p.in[jn + u], v);
-#pragma unroll
-            for (int u = 0; u < kGroup; u += 2) {
-                if (j0 + u + 1 < k) mac_pair<M>(acc, cur[u], cur[u + 1], tab, j0 + u, j0 + u + 1);
-                else if (j0 + u < k) mac_one<M>(acc, cur[u], tab, j0 + u);
-            }
-#pragma unroll
-            for (int u = 0; u < kGroup; ++u) cur[u] = nxt[u];
-        }
-#pragma unroll
-        for (int i = 0; i < M; ++i)
-            __builtin_nontemporal_store(u32x4{acc[i][0], acc[i][1], acc[i][2], acc[i][3]},
-                                        reinterpret_cast<u32x4 *>(p.out[i]) + v);
-    }
+    const gf8dev::Tables t{lo, hi};
+    const uint64_t nvec = p.len >> 4;
+    const uint64_t per_chunk = static_cast<uint64_t>(R > 0 ? R : 1) * 64;
+    const uint64_t nchunk = R > 0 ? nvec / per_chunk : 0;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const bool acc = p.accumulate != 0;
+    for (uint64_t c = static_cast<uint64_t>(blockIdx.x) * (kBlock / 64) + wave; c < nchunk;
+         c += static_cast<uint64_t>(gridDim.x) * (kBlock / 64))
+        gf8dev::chunk<M, (R > 0 ? R : 1), gf8dev::Shape<M>::G>(p.in, p.out, p.k, p.nout, acc, t, c * per_chunk,
+                                                               lane);
+    for (uint64_t v = nchunk * per_chunk + static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; v < nvec;
+         v += static_cast<uint64_t>(gridDim.x) * kBlock)
+        gf8dev::vector1<M>(p.in, p.out, p.k, p.nout, acc, t, v);
 }
 
 // Byte-granular form for the tail of a shard (len % 16) and for shards whose
@@ -205,17 +117,31 @@ hipError_t launch_gf8_pass(const Gf8Pass &p, hipStream_t stream) {
         const size_t nvec = p.len >> 4;
         tail_start = nvec << 4;
         if (nvec) {
-            const dim3 grid(grid_for(nvec)), block(kBlock);
+            // Chunks per wave by shard length: one 16-byte vector per lane
+            // below 4 MiB (latency-bound: as many waves as possible), two
+            // 1 KiB chunks per wave up to 16 MiB, four beyond (with at most
+            // four outputs, so the accumulators fit).
+            const int R = p.len < (4u << 20) ? 1 : (p.nout <= 4 && p.len >= (16u << 20)) ? 4 : 2;
+            const size_t per_block = static_cast<size_t>(R) * 64 * (kBlock / 64);
+            const size_t nchunk_blocks = R == 1 ? 0 : nvec / per_block;
+            const size_t rest = nvec - nchunk_blocks * per_block;
+            size_t g = std::max<size_t>(nchunk_blocks, (rest + kBlock - 1) / kBlock);
+            g = std::max<size_t>(1, std::min<size_t>(g, 65535u * 16));
+            const dim3 grid(static_cast<unsigned>(g)), block(kBlock);
             const size_t lds_bytes = static_cast<size_t>(p.k) * p.nout * 20;
+#define RSX_CASE(M)                                                                                  \
+    case M:                                                                                          \
+        if (R == 4) hipLaunchKernelGGL((gf8_pass_stream<M, 4>), grid, block, lds_bytes, stream, p);  \
+        else if (R == 2) hipLaunchKernelGGL((gf8_pass_stream<M, 2>), grid, block, lds_bytes, stream, p); \
+        else hipLaunchKernelGGL((gf8_pass_stream<M, 0>), grid, block, lds_bytes, stream, p);         \
+        break;
             switch (p.nout) {
-#define RSX_CASE(M) \
-    case M: hipLaunchKernelGGL(gf8_pass_vec16<M>, grid, block, lds_bytes, stream, p); break;
                 RSX_CASE(1) RSX_CASE(2) RSX_CASE(3) RSX_CASE(4)
                 RSX_CASE(5) RSX_CASE(6) RSX_CASE(7) RSX_CASE(8)
-#undef RSX_CASE
             default:
                 return hipErrorInvalidValue;
             }
+#undef RSX_CASE
             hipError_t e = hipGetLastError();
             if (e != hipSuccess) return e;
         }

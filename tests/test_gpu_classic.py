@@ -282,3 +282,26 @@ def test_rs16_4_4mib_stripes_round_trip():
         enc.Reconstruct(work)
         for i in lost:
             assert torch.equal(work[i], orig[i]), (stripe, i)
+
+
+@pytest.mark.parametrize("k,m", [(5, 3), (16, 4), (9, 7)])
+def test_long_shards_stream_path(k, m):
+    # Shards of 16 MiB and more take the four-chunks-per-wave streaming path;
+    # a ragged length also exercises the per-vector and per-byte tails.
+    rng = np.random.default_rng(k * 31 + m)
+    size = (16 << 20) + 1040 + 3
+    data = [rng.integers(0, 256, size, dtype=np.uint8) for _ in range(k)]
+    rows = rsref.codec_matrix(k, m)[k:]
+    want = [np.zeros(size, np.uint8) for _ in range(m)]
+    rsref.code_some_shards_fast(rows, data, want, threads=8)
+    enc = rs.New(k, m)
+    shards = [dev(d) for d in data] + [torch.zeros(size, dtype=torch.uint8, device="cuda") for _ in range(m)]
+    enc.Encode(shards)
+    for i in range(m):
+        assert np.array_equal(host(shards[k + i]), want[i]), i
+    lost = sorted(rng.choice(k + m, m, replace=False).tolist())
+    orig = {i: shards[i].clone() for i in lost}
+    work = [None if i in lost else shards[i] for i in range(k + m)]
+    enc.Reconstruct(work)
+    for i in lost:
+        assert torch.equal(work[i], orig[i]), i
