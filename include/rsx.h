@@ -140,6 +140,7 @@ int rsx_decode_cache_size(const rsx_codec *c);
 #define RSX_OP_ENCODE 1
 #define RSX_OP_RECONSTRUCT 2
 #define RSX_OP_RECONSTRUCT_DATA 3
+#define RSX_OP_VERIFY 4           /* parity recomputed into plan scratch and compared */
 
 typedef struct rsx_batch_item {
     rsx_codec *codec;       /* classic codec from rsx_new */
@@ -162,10 +163,56 @@ int rsx_batch_launch_on(rsx_batch *b, void *hip_stream);
 int rsx_batch_synchronize(rsx_batch *b);
 void *rsx_batch_stream(rsx_batch *b);
 int rsx_batch_num_tasks(const rsx_batch *b);
+/* Result of a verify item, valid once the launch has completed. */
+int rsx_batch_verify_result(const rsx_batch *b, int item, int *ok);
+/* 1 (default): plans are captured into a HIP graph and launches replay it;
+ * 0: launches enqueue the kernels directly (no per-plan instantiate cost). */
+int rsx_batch_set_graph(rsx_batch *b, int enabled);
 /* Tile bytes per work unit and resident blocks per CU; 0, 0 = autotune at
  * the next plan (the choice is kept for later plans of this batch). */
 int rsx_batch_set_tuning(rsx_batch *b, int tile_bytes, int blocks_per_cu);
 int rsx_batch_get_tuning(const rsx_batch *b, int *tile_bytes, int *blocks_per_cu);
+
+/* ---- Serving ------------------------------------------------------------
+ * A request loop with continuous batching: requests (Encode, Verify,
+ * Reconstruct, ReconstructData of any codec, shards in device memory) are
+ * admitted in arrival order while the bytes in flight stay within an HBM
+ * budget; a scheduler thread plans whatever has been admitted into a new
+ * batch whenever one of its launch slots (each with its own stream) is free,
+ * so batches overlap on the device, and Leopard codecs run on a worker
+ * thread.  Latency is measured from arrival to completion.  Requests in
+ * flight at the same time must not share shards.  No reference counterpart:
+ * each request is the call of reedsolomon.go it names, with that call's
+ * validation and result. */
+typedef struct rsx_request {
+    rsx_codec *codec;
+    int op;                 /* RSX_OP_* */
+    int nshards;
+    uint8_t *const *shards; /* copied at submit; buffers stay the caller's */
+    const size_t *lens;
+} rsx_request;
+
+typedef struct rsx_server rsx_server;
+
+/* hbm_budget: bytes of shards in flight (a larger request is admitted alone);
+ * max_batch: requests per launch (0 = 4096). */
+int rsx_server_create(int device, size_t hbm_budget, int max_batch, rsx_server **out);
+void rsx_server_free(rsx_server *s); /* drains first */
+int rsx_server_submit(rsx_server *s, const rsx_request *req, uint64_t *ticket);
+/* *done = 1 once complete; then *status, *verify_ok and *latency_ns are set
+ * and the ticket is released (report each ticket once: poll or wait). */
+int rsx_server_poll(rsx_server *s, uint64_t ticket, int *done, int *status, int *verify_ok,
+                    uint64_t *latency_ns);
+int rsx_server_wait(rsx_server *s, uint64_t ticket, int *status, int *verify_ok,
+                    uint64_t *latency_ns);
+int rsx_server_drain(rsx_server *s);
+/* Replays a trace: request i arrives arrival_ns[i] after the start (sorted);
+ * fills status / verify_ok / latency_ns per request and the makespan. */
+int rsx_server_run_trace(rsx_server *s, const rsx_request *reqs, const uint64_t *arrival_ns, int n,
+                         int *status, int *verify_ok, uint64_t *latency_ns, uint64_t *makespan_ns);
+/* Counters: batched launches, requests in them, highest shard bytes in flight. */
+int rsx_server_stats(const rsx_server *s, uint64_t *batches, uint64_t *requests,
+                     uint64_t *max_inflight_bytes);
 
 /* Message of the last error on the calling thread. */
 const char *rsx_last_error(void);

@@ -350,7 +350,7 @@ def New(data_shards: int, parity_shards: int, *opts) -> Encoder:
 
 
 # --- batched path (include/rsx.h rsx_batch_*) ---
-OP_ENCODE, OP_RECONSTRUCT, OP_RECONSTRUCT_DATA = 1, 2, 3
+OP_ENCODE, OP_RECONSTRUCT, OP_RECONSTRUCT_DATA, OP_VERIFY = 1, 2, 3, 4
 
 
 class _BatchItem(ctypes.Structure):
@@ -370,6 +370,35 @@ lib.rsx_batch_stream.restype = _P
 lib.rsx_batch_num_tasks.argtypes = [_P]
 lib.rsx_batch_set_tuning.argtypes = [_P, ctypes.c_int, ctypes.c_int]
 lib.rsx_batch_get_tuning.argtypes = [_P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+lib.rsx_batch_set_graph.argtypes = [_P, ctypes.c_int]
+lib.rsx_batch_verify_result.argtypes = [_P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+
+
+def _marshal_item(enc, op, shards, item):
+    """Fills one rsx_batch_item / rsx_request.  Absent shards of reconstruct
+    items get output tensors (returned in `bufs`); the library still sees
+    them as absent (length 0)."""
+    import torch
+    bufs = list(shards)
+    size, like = 0, None
+    for s in shards:
+        if s is not None and s.numel():
+            size, like = s.numel(), s
+            break
+    if op in (OP_RECONSTRUCT, OP_RECONSTRUCT_DATA) and size:
+        for i, s in enumerate(shards):
+            if s is None or s.numel() == 0:
+                bufs[i] = torch.empty(size, dtype=torch.uint8, device=like.device)
+    mm = _Marshal(bufs)
+    for i, s in enumerate(shards):
+        if s is None or s.numel() == 0:
+            mm.lens[i] = 0
+    item.codec = enc.handle
+    item.op = op
+    item.nshards = len(shards)
+    item.shards = ctypes.cast(mm.ptrs, ctypes.c_void_p)
+    item.lens = ctypes.cast(mm.lens, ctypes.c_void_p)
+    return mm, bufs
 
 
 class Batch:
@@ -405,34 +434,19 @@ class Batch:
     def stream(self):
         return lib.rsx_batch_stream(self._h)
 
+    def set_graph(self, enabled: bool) -> None:
+        """Capture the plan in a HIP graph (default) or enqueue it directly."""
+        _check(lib.rsx_batch_set_graph(self._h, 1 if enabled else 0))
+
     def plan(self, items):
         """Returns the per-item status codes (0 = ok)."""
-        import torch
         n = len(items)
         arr = (_BatchItem * max(n, 1))()
         keep, fills = [], []
         total_flags = 0
         for idx, (enc, op, shards) in enumerate(items):
-            bufs = list(shards)
-            size, like = 0, None
-            for s in shards:
-                if s is not None and s.numel():
-                    size, like = s.numel(), s
-                    break
-            if op != OP_ENCODE and size:
-                for i, s in enumerate(shards):
-                    if s is None or s.numel() == 0:
-                        bufs[i] = torch.empty(size, dtype=torch.uint8, device=like.device)
-            mm = _Marshal(bufs)
-            for i, s in enumerate(shards):
-                if s is None or s.numel() == 0:
-                    mm.lens[i] = 0
+            mm, bufs = _marshal_item(enc, op, shards, arr[idx])
             keep.append(mm)
-            arr[idx].codec = enc.handle
-            arr[idx].op = op
-            arr[idx].nshards = len(shards)
-            arr[idx].shards = ctypes.cast(mm.ptrs, ctypes.c_void_p)
-            arr[idx].lens = ctypes.cast(mm.lens, ctypes.c_void_p)
             fills.append((total_flags, shards, bufs))
             total_flags += len(shards)
         status = (ctypes.c_int * max(n, 1))()
@@ -444,6 +458,12 @@ class Batch:
                     shards[i] = bufs[i]
         self._keep = [keep, arr, items]
         return [status[i] for i in range(n)]
+
+    def verify_result(self, item: int) -> bool:
+        """Outcome of Verify item `item` after the launch has completed."""
+        ok = ctypes.c_int()
+        _check(lib.rsx_batch_verify_result(self._h, item, ctypes.byref(ok)))
+        return bool(ok.value)
 
     def launch(self, stream=None) -> None:
         if stream is None:
@@ -461,6 +481,101 @@ class Batch:
         return st
 
 
+
+# --- request loop (include/rsx.h rsx_server_*) ---
+_Request = _BatchItem  # same layout: {codec, op, nshards, shards, lens}
+_U64 = ctypes.POINTER(ctypes.c_uint64)
+_I = ctypes.POINTER(ctypes.c_int)
+lib.rsx_server_create.argtypes = [ctypes.c_int, ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(_P)]
+lib.rsx_server_free.argtypes = [_P]
+lib.rsx_server_free.restype = None
+lib.rsx_server_submit.argtypes = [_P, ctypes.POINTER(_Request), _U64]
+lib.rsx_server_poll.argtypes = [_P, ctypes.c_uint64, _I, _I, _I, _U64]
+lib.rsx_server_wait.argtypes = [_P, ctypes.c_uint64, _I, _I, _U64]
+lib.rsx_server_drain.argtypes = [_P]
+lib.rsx_server_run_trace.argtypes = [_P, ctypes.POINTER(_Request), _U64, ctypes.c_int, _I, _I, _U64,
+                                     _U64]
+lib.rsx_server_stats.argtypes = [_P, _U64, _U64, _U64]
+
+
+def _fill_outputs(op, k, shards, bufs):
+    for i, s in enumerate(shards):
+        if (s is None or s.numel() == 0) and bufs[i] is not None and (
+                op == OP_RECONSTRUCT or (op == OP_RECONSTRUCT_DATA and i < k)):
+            shards[i] = bufs[i]
+
+
+class Server:
+    """A request loop on one GPU: requests (encoder, op, shards) with shards
+    in device memory are admitted in arrival order while the shard bytes in
+    flight fit `hbm_budget`, and batched into launches as they arrive
+    (continuous batching).  Absent shards of reconstruct requests are
+    replaced by the rebuilt tensors when the request completes."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0, max_batch: int = 0):
+        h = ctypes.c_void_p()
+        _check(lib.rsx_server_create(device, hbm_budget, max_batch, ctypes.byref(h)))
+        self._h = h.value
+        self._pending = {}
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib.rsx_server_free(h)
+            self._h = None
+            self._pending.clear()
+
+    def __del__(self):
+        self.close()
+
+    def submit(self, enc, op, shards) -> int:
+        req = _Request()
+        mm, bufs = _marshal_item(enc, op, shards, req)
+        t = ctypes.c_uint64()
+        _check(lib.rsx_server_submit(self._h, ctypes.byref(req), ctypes.byref(t)))
+        self._pending[t.value] = (enc, op, shards, bufs, mm)
+        return t.value
+
+    def wait(self, ticket: int):
+        """Returns (status, verify_ok, latency_ns)."""
+        st, ok, lat = ctypes.c_int(), ctypes.c_int(), ctypes.c_uint64()
+        _check(lib.rsx_server_wait(self._h, ticket, ctypes.byref(st), ctypes.byref(ok), ctypes.byref(lat)))
+        enc, op, shards, bufs, _ = self._pending.pop(ticket)
+        if st.value == 0:
+            _fill_outputs(op, enc.DataShards(), shards, bufs)
+        return st.value, bool(ok.value), lat.value
+
+    def run_trace(self, requests, arrival_ns):
+        """Submits requests[i] = (encoder, op, shards) at arrival_ns[i] after
+        the start and waits for all.  Returns (status, verify_ok, latency_ns,
+        makespan_ns); latency runs from the scheduled arrival."""
+        n = len(requests)
+        arr = (_Request * max(n, 1))()
+        keep = []
+        for i, (enc, op, shards) in enumerate(requests):
+            keep.append(_marshal_item(enc, op, shards, arr[i]))
+        arrivals = (ctypes.c_uint64 * max(n, 1))(*[int(a) for a in arrival_ns])
+        st = (ctypes.c_int * max(n, 1))()
+        ok = (ctypes.c_int * max(n, 1))()
+        lat = (ctypes.c_uint64 * max(n, 1))()
+        span = ctypes.c_uint64()
+        _check(lib.rsx_server_run_trace(self._h, arr, arrivals, n, st, ok, lat, ctypes.byref(span)))
+        for i, (enc, op, shards) in enumerate(requests):
+            if st[i] == 0:
+                _fill_outputs(op, enc.DataShards(), shards, keep[i][1])
+        return ([st[i] for i in range(n)], [bool(ok[i]) for i in range(n)],
+                [lat[i] for i in range(n)], span.value)
+
+    def drain(self) -> None:
+        _check(lib.rsx_server_drain(self._h))
+
+    def stats(self):
+        """(batched launches, requests in them, max shard bytes in flight)."""
+        a, b, c = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        _check(lib.rsx_server_stats(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+
 __all__ = [n for n in dir() if n.startswith(("With", "Err", "New", "Encoder", "RSError"))] + [
-    "device_count", "lib", "LIB_PATH", "Batch", "OP_ENCODE", "OP_RECONSTRUCT",
-    "OP_RECONSTRUCT_DATA"]
+    "device_count", "lib", "LIB_PATH", "Batch", "Server", "OP_ENCODE", "OP_RECONSTRUCT",
+    "OP_RECONSTRUCT_DATA", "OP_VERIFY"]

@@ -12,6 +12,7 @@
 
 #include "batch.hpp"
 #include "codec.hpp"
+#include "kernels.hpp"
 
 namespace rsx {
 int check_shards(const size_t *lens, int n, bool nilok);
@@ -47,6 +48,16 @@ struct rsx_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool planned = false;
+    bool use_graph = true;
+
+    // Verify items: recomputed parity goes to `scratch`, compared per item
+    // into `d_flags`, which is copied to the pinned `h_flags` at the end.
+    std::vector<CompareEntry> cmp;
+    DeviceBuffer d_cmp, d_flags, scratch;
+    uint32_t *h_flags = nullptr;
+    size_t h_flags_cap = 0;
+    int nitems = 0;
+    std::vector<uint8_t> is_verify;
 };
 
 namespace {
@@ -109,9 +120,18 @@ int build_launches(rsx_batch *b, uint32_t tile) {
 
 int enqueue(rsx_batch *b, hipStream_t s) {
     const unsigned grid = static_cast<unsigned>(std::max(1, b->cus * b->blocks_per_cu));
+    const bool verify = !b->cmp.empty();
+    if (verify)
+        RSX_HIP(launch_fill_u32(b->d_flags.as<uint32_t>(), static_cast<size_t>(b->nitems), 0u, s));
     for (size_t i = 0; i < b->launches.size(); ++i) {
         const unsigned g = std::min<unsigned>(grid, std::max<uint32_t>(b->launches[i].ntiles, 1));
         RSX_HIP(launch_batch_pass(b->launches[i], b->classes[i], g, s));
+    }
+    if (verify) {
+        RSX_HIP(launch_batch_compare(b->d_cmp.as<CompareEntry>(), static_cast<int>(b->cmp.size()),
+                                     b->d_flags.as<uint32_t>(), s));
+        RSX_HIP(hipMemcpyAsync(b->h_flags, b->d_flags.ptr, sizeof(uint32_t) * b->nitems,
+                               hipMemcpyDeviceToHost, s));
     }
     return RSX_OK;
 }
@@ -202,6 +222,7 @@ void rsx_batch_free(rsx_batch *b) {
     (void)hipSetDevice(b->device);
     (void)hipStreamSynchronize(b->stream);
     destroy_graph(b);
+    if (b->h_flags) (void)hipHostFree(b->h_flags);
     (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -230,6 +251,18 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
     for (auto &v : b->tasks) v.clear();
     b->pool.clear();
     b->planned = false;
+    b->cmp.clear();
+    b->nitems = n;
+    b->is_verify.assign(static_cast<size_t>(std::max(n, 0)), 0);
+    struct VerifyOut {
+        size_t item;
+        std::vector<size_t> slots;  // scratch offsets of the recomputed parity rows
+        std::vector<const uint8_t *> parity;
+        size_t len;
+    };
+    std::vector<VerifyOut> vouts;
+    std::vector<std::pair<size_t, size_t>> scratch_patch;  // (pool index, scratch offset)
+    size_t scratch_bytes = 0;
     std::map<const void *, size_t> table_off;  // source tables -> offset in batch copy (dwords)
     std::vector<std::pair<const uint32_t *, size_t>> table_src;  // (source, dwords)
     size_t table_dwords = 0;
@@ -259,8 +292,9 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
         std::vector<uint8_t *> out;
         const uint32_t *tables = nullptr;
         int mtot = 0;
-        if (item.op == RSX_OP_ENCODE) {
-            // reedsolomon.go:686-702
+        int err_item = RSX_OK;
+        if (item.op == RSX_OP_ENCODE || item.op == RSX_OP_VERIFY) {
+            // reedsolomon.go:686-702 (Encode) / 857-871 (Verify)
             int err = RSX_OK;
             if (item.nshards != total) err = fail(RSX_ERR_TOO_FEW_SHARDS, "too few shards given");
             else err = check_shards(lens, total, false);
@@ -268,9 +302,28 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
                 status[it] = err;
                 continue;
             }
-            if (c->m == 0) continue;
+            if (c->m == 0) {
+                if (item.op == RSX_OP_VERIFY) b->is_verify[it] = 2;  // no parity: trivially true
+                continue;
+            }
             for (int j = 0; j < k; ++j) in.push_back(item.shards[j]);
-            for (int i = k; i < total; ++i) out.push_back(item.shards[i]);
+            if (item.op == RSX_OP_VERIFY) {
+                // Parity is recomputed into plan scratch (pointers patched once
+                // the scratch buffer is sized) and compared after the passes.
+                const size_t len = lens[0];
+                const size_t pitch = (len + 255) / 256 * 256;
+                VerifyOut vo{static_cast<size_t>(it), {}, {}, len};
+                for (int i = k; i < total; ++i) {
+                    vo.slots.push_back(scratch_bytes);
+                    vo.parity.push_back(item.shards[i]);
+                    out.push_back(nullptr);  // patched to scratch once it is sized
+                    scratch_bytes += pitch;
+                }
+                vouts.push_back(std::move(vo));
+                b->is_verify[it] = 1;
+            } else {
+                for (int i = k; i < total; ++i) out.push_back(item.shards[i]);
+            }
             tables = c->parity_tables.as<uint32_t>();
             mtot = c->m;
         } else if (item.op == RSX_OP_RECONSTRUCT || item.op == RSX_OP_RECONSTRUCT_DATA) {
@@ -314,6 +367,37 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
             status[it] = fail(RSX_ERR_INVALID_INPUT, "unknown batch op");
             continue;
         }
+        {
+            // Batched items run on device memory only; a missing or host
+            // buffer becomes this item's status, never a kernel access.
+            std::vector<const void *> ptrs(in.begin(), in.end());
+            bool missing = false;
+            for (const uint8_t *p : in) missing |= p == nullptr;
+            for (size_t i = 0; i < out.size(); ++i) {
+                if (b->is_verify[it]) continue;
+                missing |= out[i] == nullptr;
+                ptrs.push_back(out[i]);
+            }
+            if (item.op == RSX_OP_VERIFY)
+                for (int i = k; i < total; ++i) {
+                    missing |= item.shards[i] == nullptr;
+                    ptrs.push_back(item.shards[i]);
+                }
+            MemKind kind = MemKind::Host;
+            if (missing) err_item = fail(RSX_ERR_INVALID_INPUT, "no buffer for a shard of a batched item");
+            else if ((err_item = classify_all(ptrs, kind)) == RSX_OK && kind != MemKind::Device)
+                err_item = fail(RSX_ERR_MIXED_MEMORY, "batched items need shards in device memory");
+            if (err_item) {
+                if (b->is_verify[it] == 1) {
+                    scratch_bytes -= vouts.back().slots.size() * ((lens[0] + 255) / 256 * 256);
+                    vouts.pop_back();
+                    b->is_verify[it] = 0;
+                }
+                if (fl) std::memset(fl, 0, c->total);
+                status[it] = err_item;
+                continue;
+            }
+        }
         const size_t len = shard_size(lens, total);
         uintptr_t bits = 0;
         for (auto *p : in) bits |= reinterpret_cast<uintptr_t>(p);
@@ -329,7 +413,10 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
         for (int o0 = 0; o0 < mtot; o0 += 8) {
             const int nout = std::min(8, mtot - o0);
             const uint32_t out_off = static_cast<uint32_t>(b->pool.size());
-            for (int i = 0; i < nout; ++i) b->pool.push_back(out[o0 + i]);
+            for (int i = 0; i < nout; ++i) {
+                if (b->is_verify[it]) scratch_patch.emplace_back(b->pool.size(), vouts.back().slots[o0 + i]);
+                b->pool.push_back(out[o0 + i]);
+            }
             rsx_batch::Task t{};
             t.tables = nullptr;
             t.len = len;
@@ -342,6 +429,27 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
             t.aligned = (bits & 15) == 0;
             b->tasks[nout].push_back(t);
             pend.push_back({nout, b->tasks[nout].size() - 1, tables});
+        }
+    }
+    // Scratch for verify items; patch the placeholder output pointers.
+    if (!vouts.empty()) {
+        int e2 = b->scratch.reserve(std::max<size_t>(scratch_bytes, 256));
+        if (e2) return e2;
+        uint8_t *base = b->scratch.as<uint8_t>();
+        for (auto &pp : scratch_patch) b->pool[pp.first] = base + pp.second;
+        for (auto &vo : vouts)
+            for (size_t r = 0; r < vo.slots.size(); ++r)
+                b->cmp.push_back({base + vo.slots[r], vo.parity[r], vo.len, static_cast<uint32_t>(vo.item), 0});
+        int e3 = b->d_cmp.reserve(b->cmp.size() * sizeof(CompareEntry));
+        if (e3) return e3;
+        RSX_HIP(hipMemcpyAsync(b->d_cmp.ptr, b->cmp.data(), b->cmp.size() * sizeof(CompareEntry),
+                               hipMemcpyHostToDevice, b->stream));
+        if ((e3 = b->d_flags.reserve(sizeof(uint32_t) * std::max(n, 1)))) return e3;
+        if (b->h_flags_cap < static_cast<size_t>(n)) {
+            if (b->h_flags) (void)hipHostFree(b->h_flags);
+            b->h_flags = nullptr;
+            RSX_HIP(hipHostMalloc(reinterpret_cast<void **>(&b->h_flags), sizeof(uint32_t) * std::max(n, 1)));
+            b->h_flags_cap = static_cast<size_t>(n);
         }
     }
     // Batch-owned copy of every table the plan reads.
@@ -360,22 +468,33 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
     } else {
         if ((err = autotune(b))) return err;
     }
-    if ((err = capture(b))) return err;
+    if (b->use_graph && (err = capture(b))) return err;
     b->planned = true;
-    return RSX_OK;
-}
-
-int rsx_batch_launch(rsx_batch *b) {
-    if (!b->planned) return fail(RSX_ERR_INVALID_INPUT, "batch has no plan");
-    (void)hipSetDevice(b->device);
-    RSX_HIP(hipGraphLaunch(b->exec, b->stream));
     return RSX_OK;
 }
 
 int rsx_batch_launch_on(rsx_batch *b, void *hip_stream) {
     if (!b->planned) return fail(RSX_ERR_INVALID_INPUT, "batch has no plan");
     (void)hipSetDevice(b->device);
-    RSX_HIP(hipGraphLaunch(b->exec, hip_stream ? static_cast<hipStream_t>(hip_stream) : b->stream));
+    hipStream_t s = hip_stream ? static_cast<hipStream_t>(hip_stream) : b->stream;
+    if (b->use_graph) RSX_HIP(hipGraphLaunch(b->exec, s));
+    else return enqueue(b, s);
+    return RSX_OK;
+}
+
+int rsx_batch_launch(rsx_batch *b) { return rsx_batch_launch_on(b, nullptr); }
+
+int rsx_batch_set_graph(rsx_batch *b, int enabled) {
+    b->use_graph = enabled != 0;
+    b->planned = false;
+    return RSX_OK;
+}
+
+int rsx_batch_verify_result(const rsx_batch *b, int item, int *ok) {
+    *ok = 0;
+    if (item < 0 || item >= b->nitems || !b->is_verify[item])
+        return fail(RSX_ERR_INVALID_INPUT, "item is not a verify item of this plan");
+    *ok = b->is_verify[item] == 2 || b->h_flags[item] == 0;
     return RSX_OK;
 }
 

@@ -32,6 +32,18 @@ struct BatchLaunch {
     uint32_t tile;  // bytes per tile (multiple of 16)
 };
 
+// Verify items: recomputed parity `a` vs the item's parity `b`.
+struct CompareEntry {
+    const uint8_t *a;
+    const uint8_t *b;
+    uint64_t len;
+    uint32_t item;
+    uint32_t pad;
+};
+
+// flags[e.item] = 1 for every entry whose bytes differ.
+hipError_t launch_batch_compare(const CompareEntry *entries, int n, uint32_t *flags, hipStream_t s);
+
 // Launches the persistent pass for tasks of exactly m_class outputs.
 hipError_t launch_batch_pass(const BatchLaunch &L, int m_class, unsigned grid, hipStream_t s);
 

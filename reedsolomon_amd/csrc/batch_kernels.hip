@@ -12,6 +12,8 @@
 #include "batch.hpp"
 #include "gf8_device.hpp"
 
+#include <algorithm>
+
 namespace rsx {
 
 namespace {
@@ -77,7 +79,35 @@ __global__ __launch_bounds__(kBlock) void batch_pass(const BatchLaunch L) {
     }
 }
 
+__global__ __launch_bounds__(kBlock) void batch_compare(const CompareEntry *entries, int n,
+                                                        uint32_t *flags) {
+    for (int e = blockIdx.x; e < n; e += gridDim.x) {
+        const CompareEntry c = entries[e];
+        const bool aligned = ((reinterpret_cast<uintptr_t>(c.a) | reinterpret_cast<uintptr_t>(c.b)) & 15) == 0;
+        uint32_t diff = 0;
+        uint64_t b = 0;
+        if (aligned) {
+            const uint64_t nv = c.len >> 4;
+            for (uint64_t v = threadIdx.x; v < nv; v += kBlock) {
+                const uint4 x = reinterpret_cast<const uint4 *>(c.a)[v];
+                const uint4 y = reinterpret_cast<const uint4 *>(c.b)[v];
+                diff |= (x.x ^ y.x) | (x.y ^ y.y) | (x.z ^ y.z) | (x.w ^ y.w);
+            }
+            b = nv << 4;
+        }
+        for (uint64_t i = b + threadIdx.x; i < c.len; i += kBlock) diff |= c.a[i] ^ c.b[i];
+        if (diff) flags[c.item] = 1;
+    }
+}
+
 }  // namespace
+
+hipError_t launch_batch_compare(const CompareEntry *entries, int n, uint32_t *flags, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(batch_compare, dim3(static_cast<unsigned>(std::min(n, 65535))), dim3(kBlock), 0, s,
+                       entries, n, flags);
+    return hipGetLastError();
+}
 
 hipError_t launch_batch_pass(const BatchLaunch &L, int m_class, unsigned grid, hipStream_t s) {
     if (L.ntiles == 0) return hipSuccess;

@@ -49,6 +49,7 @@ struct DecodePlan {
 
 // Memory class of one call's shards.
 enum class MemKind { Device, Host };
+int classify_all(const std::vector<const void *> &ptrs, MemKind &kind);
 
 struct LeopardState;  // leopard.cpp
 struct LeopardDeleter {

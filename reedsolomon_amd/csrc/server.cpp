@@ -1,0 +1,338 @@
+// server.cpp — request loop with continuous batching (include/rsx.h,
+// "Serving").
+//
+// Submit copies the request and appends it to a FIFO.  The scheduler thread
+// loops: (1) retire launch slots whose completion event has fired, (2) when a
+// slot is free, admit queued requests in order while the shard bytes in
+// flight stay within the HBM budget (a request larger than the budget is
+// admitted alone), plan the admitted classic requests into that slot's batch
+// and launch it on the slot's stream, (3) hand Leopard requests to a worker
+// thread.  Slots have their own streams, so a batch planned from requests
+// that arrived while another batch runs overlaps with it on the device.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <thread>
+#include <unordered_map>
+
+#include "codec.hpp"
+
+using Clock = std::chrono::steady_clock;
+
+namespace {
+
+constexpr int kSlots = 4;
+
+struct Req {
+    rsx_request r{};
+    std::vector<uint8_t *> shards;
+    std::vector<size_t> lens;
+    size_t bytes = 0;
+    Clock::time_point arrival;
+    bool leopard = false;
+    bool done = false;
+    int status = 0;
+    int verify_ok = 0;
+    uint64_t latency_ns = 0;
+};
+
+struct Slot {
+    rsx_batch *b = nullptr;
+    hipEvent_t ev = nullptr;
+    std::vector<std::pair<uint64_t, int>> tickets;  // (ticket, item index in the plan)
+    bool busy = false;
+};
+
+}  // namespace
+
+struct rsx_server {
+    int device = 0;
+    size_t budget = 0;
+    int max_batch = 4096;
+    mutable std::mutex mu;
+    std::condition_variable cv;       // scheduler wake-up
+    std::condition_variable done_cv;  // completions
+    std::condition_variable work_cv;  // worker wake-up
+    std::unordered_map<uint64_t, Req> reqs;
+    std::deque<uint64_t> queue, work;
+    uint64_t next = 1;
+    size_t inflight = 0, max_inflight = 0;
+    size_t outstanding = 0;  // submitted and not completed
+    uint64_t batches = 0, batched_requests = 0;
+    Slot slots[kSlots];
+    bool stop = false;
+    std::thread sched, worker;
+};
+
+namespace {
+
+// Caller holds s->mu.
+void complete(rsx_server *s, uint64_t t, int status, int verify_ok) {
+    Req &q = s->reqs[t];
+    q.done = true;
+    q.status = status;
+    q.verify_ok = verify_ok;
+    q.latency_ns = static_cast<uint64_t>(
+        std::chrono::duration_cast<std::chrono::nanoseconds>(Clock::now() - q.arrival).count());
+    s->inflight -= q.bytes;
+    --s->outstanding;
+    s->done_cv.notify_all();
+    s->cv.notify_one();  // budget freed: the scheduler may admit more
+}
+
+void worker_loop(rsx_server *s) {
+    (void)hipSetDevice(s->device);
+    std::unique_lock<std::mutex> lock(s->mu);
+    for (;;) {
+        s->work_cv.wait(lock, [&] { return s->stop || !s->work.empty(); });
+        if (s->work.empty()) {
+            if (s->stop) return;
+            continue;
+        }
+        const uint64_t t = s->work.front();
+        s->work.pop_front();
+        Req &q = s->reqs[t];
+        rsx_codec *c = q.r.codec;
+        std::vector<uint8_t *> sh = q.shards;
+        std::vector<size_t> ln = q.lens;
+        const int op = q.r.op, n = q.r.nshards;
+        lock.unlock();
+        int rc = RSX_OK, ok = 0;
+        std::vector<uint8_t> filled(static_cast<size_t>(std::max(n, 1)));
+        switch (op) {
+        case RSX_OP_ENCODE: rc = rsx_encode(c, sh.data(), ln.data(), n); break;
+        case RSX_OP_VERIFY: rc = rsx_verify(c, sh.data(), ln.data(), n, &ok); break;
+        case RSX_OP_RECONSTRUCT: rc = rsx_reconstruct(c, sh.data(), ln.data(), n, filled.data()); break;
+        case RSX_OP_RECONSTRUCT_DATA:
+            rc = rsx_reconstruct_data(c, sh.data(), ln.data(), n, filled.data());
+            break;
+        default: rc = rsx::fail(RSX_ERR_INVALID_INPUT, "unknown request op");
+        }
+        if (rc == RSX_OK) rc = rsx_synchronize(c);
+        lock.lock();
+        complete(s, t, rc, ok);
+    }
+}
+
+void scheduler_loop(rsx_server *s) {
+    (void)hipSetDevice(s->device);
+    std::unique_lock<std::mutex> lock(s->mu);
+    for (;;) {
+        // (1) retire finished slots
+        for (Slot &sl : s->slots) {
+            if (!sl.busy || hipEventQuery(sl.ev) != hipSuccess) continue;
+            for (const auto &ti : sl.tickets) {
+                int ok = 0;
+                if (s->reqs[ti.first].r.op == RSX_OP_VERIFY) rsx_batch_verify_result(sl.b, ti.second, &ok);
+                complete(s, ti.first, RSX_OK, ok);
+            }
+            sl.tickets.clear();
+            sl.busy = false;
+        }
+        // (2) admit into a free slot
+        Slot *free_slot = nullptr;
+        for (Slot &sl : s->slots)
+            if (!sl.busy) {
+                free_slot = &sl;
+                break;
+            }
+        std::vector<uint64_t> admitted;
+        while (free_slot && !s->queue.empty() && static_cast<int>(admitted.size()) < s->max_batch) {
+            const uint64_t t = s->queue.front();
+            Req &q = s->reqs[t];
+            if (s->inflight > 0 && s->inflight + q.bytes > s->budget) break;  // FIFO: wait for room
+            s->queue.pop_front();
+            s->inflight += q.bytes;
+            s->max_inflight = std::max(s->max_inflight, s->inflight);
+            if (q.leopard) {
+                s->work.push_back(t);
+                s->work_cv.notify_one();
+            } else {
+                admitted.push_back(t);
+            }
+        }
+        if (!admitted.empty()) {
+            std::vector<rsx_batch_item> items(admitted.size());
+            for (size_t i = 0; i < admitted.size(); ++i) {
+                const Req &q = s->reqs[admitted[i]];
+                items[i] = {q.r.codec, q.r.op, q.r.nshards, q.shards.data(), q.lens.data()};
+            }
+            std::vector<int> status(admitted.size());
+            size_t nflags = 0;
+            for (auto &it : items) nflags += static_cast<size_t>(std::max(it.nshards, 0));
+            std::vector<uint8_t> filled(std::max<size_t>(nflags, 1));
+            lock.unlock();
+            int rc = rsx_batch_plan(free_slot->b, items.data(), static_cast<int>(items.size()), status.data(),
+                                    filled.data());
+            if (rc == RSX_OK) rc = rsx_batch_launch(free_slot->b);
+            if (rc == RSX_OK) {
+                const hipError_t e =
+                    hipEventRecord(free_slot->ev, static_cast<hipStream_t>(rsx_batch_stream(free_slot->b)));
+                if (e != hipSuccess) rc = rsx::hip_fail(e, "hipEventRecord");
+            }
+            lock.lock();
+            ++s->batches;
+            s->batched_requests += admitted.size();
+            for (size_t i = 0; i < admitted.size(); ++i) {
+                if (rc != RSX_OK || status[i] != RSX_OK) complete(s, admitted[i], rc != RSX_OK ? rc : status[i], 0);
+                else free_slot->tickets.emplace_back(admitted[i], static_cast<int>(i));
+            }
+            free_slot->busy = !free_slot->tickets.empty();
+            continue;  // look for more work right away
+        }
+        bool busy = false;
+        for (Slot &sl : s->slots) busy |= sl.busy;
+        if (s->stop && s->queue.empty() && !busy) return;
+        if (busy) {
+            lock.unlock();
+            std::this_thread::sleep_for(std::chrono::microseconds(5));
+            lock.lock();
+        } else {
+            s->cv.wait_for(lock, std::chrono::milliseconds(1));
+        }
+    }
+}
+
+int submit_at(rsx_server *s, const rsx_request *req, Clock::time_point arrival, uint64_t *ticket) {
+    if (!req->codec || req->nshards != req->codec->total)
+        return rsx::fail(RSX_ERR_TOO_FEW_SHARDS, "too few shards given");
+    Req q;
+    q.r = *req;
+    q.shards.assign(req->shards, req->shards + req->nshards);
+    q.lens.assign(req->lens, req->lens + req->nshards);
+    size_t size = 0;
+    for (size_t l : q.lens) size = std::max(size, l);
+    q.bytes = size * static_cast<size_t>(req->nshards);
+    q.arrival = arrival;
+    q.leopard = req->codec->engine != 0;
+    std::lock_guard<std::mutex> lock(s->mu);
+    const uint64_t t = s->next++;
+    s->reqs.emplace(t, std::move(q));
+    s->queue.push_back(t);
+    ++s->outstanding;
+    *ticket = t;
+    s->cv.notify_one();
+    return RSX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rsx_server_create(int device, size_t hbm_budget, int max_batch, rsx_server **out) {
+    *out = nullptr;
+    auto s = std::make_unique<rsx_server>();
+    s->device = device;
+    s->budget = hbm_budget ? hbm_budget : (size_t{4} << 30);
+    s->max_batch = max_batch > 0 ? max_batch : 4096;
+    for (Slot &sl : s->slots) {
+        int rc = rsx_batch_create(device, &sl.b);
+        if (rc) return rc;
+        rsx_batch_set_graph(sl.b, 0);
+        rsx_batch_set_tuning(sl.b, 16 << 10, 8);
+        if (hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming) != hipSuccess)
+            return rsx::fail(RSX_ERR_DEVICE, "event creation failed");
+    }
+    rsx_server *p = s.release();
+    p->sched = std::thread(scheduler_loop, p);
+    p->worker = std::thread(worker_loop, p);
+    *out = p;
+    return RSX_OK;
+}
+
+int rsx_server_drain(rsx_server *s) {
+    std::unique_lock<std::mutex> lock(s->mu);
+    s->done_cv.wait(lock, [&] { return s->outstanding == 0; });
+    return RSX_OK;
+}
+
+void rsx_server_free(rsx_server *s) {
+    if (!s) return;
+    rsx_server_drain(s);
+    {
+        std::lock_guard<std::mutex> lock(s->mu);
+        s->stop = true;
+    }
+    s->cv.notify_all();
+    s->work_cv.notify_all();
+    s->sched.join();
+    s->worker.join();
+    for (Slot &sl : s->slots) {
+        if (sl.ev) (void)hipEventDestroy(sl.ev);
+        rsx_batch_free(sl.b);
+    }
+    delete s;
+}
+
+int rsx_server_submit(rsx_server *s, const rsx_request *req, uint64_t *ticket) {
+    return submit_at(s, req, Clock::now(), ticket);
+}
+
+int rsx_server_poll(rsx_server *s, uint64_t ticket, int *done, int *status, int *verify_ok,
+                    uint64_t *latency_ns) {
+    std::lock_guard<std::mutex> lock(s->mu);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) return rsx::fail(RSX_ERR_INVALID_INPUT, "unknown ticket");
+    *done = it->second.done;
+    if (it->second.done) {
+        *status = it->second.status;
+        *verify_ok = it->second.verify_ok;
+        *latency_ns = it->second.latency_ns;
+        s->reqs.erase(it);  // a ticket is reported done once
+    }
+    return RSX_OK;
+}
+
+int rsx_server_wait(rsx_server *s, uint64_t ticket, int *status, int *verify_ok, uint64_t *latency_ns) {
+    std::unique_lock<std::mutex> lock(s->mu);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) return rsx::fail(RSX_ERR_INVALID_INPUT, "unknown ticket");
+    s->done_cv.wait(lock, [&] { return it->second.done; });
+    *status = it->second.status;
+    *verify_ok = it->second.verify_ok;
+    *latency_ns = it->second.latency_ns;
+    s->reqs.erase(it);
+    return RSX_OK;
+}
+
+int rsx_server_run_trace(rsx_server *s, const rsx_request *reqs, const uint64_t *arrival_ns, int n,
+                         int *status, int *verify_ok, uint64_t *latency_ns, uint64_t *makespan_ns) {
+    std::vector<uint64_t> tickets(static_cast<size_t>(std::max(n, 0)));
+    const Clock::time_point t0 = Clock::now();
+    for (int i = 0; i < n; ++i) {
+        const Clock::time_point at = t0 + std::chrono::nanoseconds(arrival_ns[i]);
+        while (Clock::now() < at) {
+            const auto left = at - Clock::now();
+            if (left > std::chrono::microseconds(200)) std::this_thread::sleep_for(left / 2);
+        }
+        const int rc = submit_at(s, &reqs[i], at, &tickets[i]);
+        if (rc) return rc;
+    }
+    rsx_server_drain(s);
+    Clock::time_point last = t0;
+    {
+        std::lock_guard<std::mutex> lock(s->mu);
+        for (int i = 0; i < n; ++i) {
+            const Req &q = s->reqs[tickets[i]];
+            status[i] = q.status;
+            verify_ok[i] = q.verify_ok;
+            latency_ns[i] = q.latency_ns;
+            last = std::max(last, q.arrival + std::chrono::nanoseconds(q.latency_ns));
+            s->reqs.erase(tickets[i]);
+        }
+    }
+    *makespan_ns = static_cast<uint64_t>(std::chrono::duration_cast<std::chrono::nanoseconds>(last - t0).count());
+    return RSX_OK;
+}
+
+int rsx_server_stats(const rsx_server *s, uint64_t *batches, uint64_t *requests, uint64_t *max_inflight_bytes) {
+    std::lock_guard<std::mutex> lock(s->mu);
+    *batches = s->batches;
+    *requests = s->batched_requests;
+    *max_inflight_bytes = s->max_inflight;
+    return RSX_OK;
+}
+
+}  // extern "C"
