@@ -19,6 +19,12 @@ Bytes per operation follow the reference's benchmarks
 Encode and for each Reconstruct that has work to do (a stripe with no erasure
 returns immediately and is not counted).
 
+Served stream (unless --quick): 4096 requests, 60/10/30 encode / verify /
+reconstruct with the mixed-workload shapes and sizes plus 5% Leopard GF(2^8)
+(192,56) 64 KiB requests, through the request loop (continuous batching,
+HBM-budget admission): saturation rate with all arrivals at t=0, then Poisson
+arrivals at 0.7x that rate; p50/p99 latency from arrival to completion.
+
     python bench.py [--steps K] [--warmup W] [--quick]
 
 Prints one JSON line.  `roofline` is for the dominant kernel (the batched
@@ -54,6 +60,11 @@ def _parse():
     ap.add_argument("--quick", action="store_true", help="headline workload only")
     ap.add_argument("--tile", type=int, default=0, help="batched tile bytes (0 = autotune)")
     ap.add_argument("--bpc", type=int, default=0, help="batched blocks per CU (0 = autotune)")
+    ap.add_argument("--requests", type=int, default=4096, help="served stream: requests per trace")
+    ap.add_argument("--load", type=float, default=0.7,
+                    help="served stream: Poisson rate as a fraction of the measured saturation rate")
+    ap.add_argument("--hbm-budget-gib", type=float, default=16.0,
+                    help="served stream: shard bytes admitted in flight")
     ap.add_argument("--encode-only", action="store_true",
                     help="time only the batched RS(16,4) encode launches (for kernel profiling)")
     return ap.parse_args()
@@ -361,6 +372,113 @@ def leopard(args, torch, rs):
     return out
 
 
+def served(args, torch, rs):
+    """Served request stream: encode / verify / reconstruct requests (60/10/30)
+    of the mixed-workload shapes and sizes plus 5% Leopard GF(2^8) (192,56)
+    64 KiB requests, replayed through the request loop.  First every request
+    arrives at t=0 (saturation rate), then with Poisson arrivals at
+    `load` x that rate; latency runs from scheduled arrival to completion."""
+    lib = rs.lib
+    shapes_all = [(4, 2), (6, 3), (10, 4), (17, 3), (32, 8)]
+    rng = np.random.default_rng(SEED + 4)
+    n = args.requests
+    leo_shape = (192, 56)
+    is_leo = rng.random(n) < 0.05
+    shapes = [leo_shape if l else shapes_all[int(rng.integers(0, 5))] for l in is_leo]
+    sizes = [65536 if l else
+             max(64, int(round(np.exp(rng.uniform(np.log(4096), np.log(1 << 20))) / 64)) * 64)
+             for l in is_leo]
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(SEED + 4)
+    st = Stripes(shapes, sizes, torch, gen)
+    codecs = {s: rs.New(*s) for s in shapes_all}
+    codecs[leo_shape] = rs.New(*leo_shape, rs.WithLeopardGF(True))
+    # every stripe starts as a valid codeword (verify / reconstruct need one)
+    classic = [i for i in range(n) if not is_leo[i]]
+    b0 = rs.Batch(0)
+    items = batch_items(rs, st, codecs, [rs.OP_ENCODE] * n, st.full)
+    sub = (rs._BatchItem * len(classic))(*[items[i] for i in classic])
+    plan(rs, b0, sub, len(classic))
+    b0.launch()
+    b0.synchronize()
+    del b0
+    for i in range(n):
+        if is_leo[i]:
+            _check(lib, lib.rsx_encode(codecs[leo_shape].handle, st.ptrs[i], st.full[i], sum(leo_shape)))
+    codecs[leo_shape].Synchronize()
+    ops, lens, total = [], [], 0
+    for i, ((k, m), s) in enumerate(zip(shapes, sizes)):
+        nsh = k + m
+        total += nsh * s
+        r = rng.random()
+        if r < 0.6:
+            ops.append(rs.OP_ENCODE)
+            lens.append(st.full[i])
+        elif r < 0.7:
+            ops.append(rs.OP_VERIFY)
+            lens.append(st.full[i])
+        else:
+            lost = set(rng.choice(nsh, int(rng.integers(1, m + 1)), replace=False).tolist())
+            ops.append(rs.OP_RECONSTRUCT)
+            lens.append((ctypes.c_size_t * nsh)(*[0 if j in lost else s for j in range(nsh)]))
+    reqs = batch_items(rs, st, codecs, ops, lens)  # rsx_request has rsx_batch_item's layout
+    srv = rs.Server(0, hbm_budget=int(args.hbm_budget_gib * (1 << 30)))
+    st_arr = (ctypes.c_int * n)()
+    ok_arr = (ctypes.c_int * n)()
+    lat_arr = (ctypes.c_uint64 * n)()
+    span = ctypes.c_uint64()
+
+    def replay(arrivals):
+        arr = (ctypes.c_uint64 * n)(*[int(a) for a in arrivals])
+        _check(lib, lib.rsx_server_run_trace(srv._h, reqs, arr, n, st_arr, ok_arr, lat_arr, ctypes.byref(span)))
+        bad = [st_arr[i] for i in range(n) if st_arr[i] != 0]
+        if bad:
+            raise RuntimeError(f"served request errors: {bad[:5]}")
+        if not all(ok_arr[i] for i in range(n) if ops[i] == rs.OP_VERIFY):
+            raise RuntimeError("a served Verify of a valid codeword failed")
+        lat = np.array([lat_arr[i] for i in range(n)], dtype=np.float64) * 1e-6  # ms
+        return span.value * 1e-9, lat
+
+    replay(np.zeros(n))  # warm: decode matrices and error locators cached on the device
+    b_before = srv.stats()
+    t_sat, lat_sat = replay(np.zeros(n))
+    b_after = srv.stats()
+    rate = args.load * n / t_sat
+    arrivals = np.cumsum(rng.exponential(1e9 / rate, n))
+    t_poi, lat_poi = replay(arrivals)
+    b_end = srv.stats()
+    srv.close()
+    leo_lat = lat_poi[is_leo]
+    cls_lat = lat_poi[~is_leo]
+    _log(f"bench: served saturation {total / t_sat / 1e9:.1f} GB/s; Poisson {rate:.0f} req/s "
+         f"{total / t_poi / 1e9:.1f} GB/s p50 {np.percentile(lat_poi, 50):.3f} ms "
+         f"p99 {np.percentile(lat_poi, 99):.3f} ms")
+    out = dict(
+        requests=n, bytes_per_trace=total, leopard_requests=int(is_leo.sum()),
+        op_mix={"encode": ops.count(rs.OP_ENCODE), "verify": ops.count(rs.OP_VERIFY),
+                "reconstruct": ops.count(rs.OP_RECONSTRUCT)},
+        hbm_budget_bytes=int(args.hbm_budget_gib * (1 << 30)),
+        saturation={"gbps": round(total / t_sat / 1e9, 1), "makespan_ms": round(t_sat * 1e3, 2),
+                    "p50_ms": round(float(np.percentile(lat_sat, 50)), 3),
+                    "p99_ms": round(float(np.percentile(lat_sat, 99)), 3),
+                    "batches": b_after[0] - b_before[0],
+                    "requests_batched": b_after[1] - b_before[1]},
+        poisson={"load": args.load, "rate_req_per_s": round(rate, 1),
+                 "gbps": round(total / t_poi / 1e9, 1), "makespan_ms": round(t_poi * 1e3, 2),
+                 "p50_ms": round(float(np.percentile(lat_poi, 50)), 3),
+                 "p99_ms": round(float(np.percentile(lat_poi, 99)), 3),
+                 "mean_ms": round(float(lat_poi.mean()), 3),
+                 "classic_p50_ms": round(float(np.percentile(cls_lat, 50)), 3),
+                 "classic_p99_ms": round(float(np.percentile(cls_lat, 99)), 3),
+                 "leopard_p50_ms": round(float(np.percentile(leo_lat, 50)), 3) if len(leo_lat) else None,
+                 "leopard_p99_ms": round(float(np.percentile(leo_lat, 99)), 3) if len(leo_lat) else None,
+                 "batches": b_end[0] - b_after[0], "requests_batched": b_end[1] - b_after[1]},
+        max_inflight_bytes=b_end[2])
+    del st
+    torch.cuda.empty_cache()
+    return out
+
+
 def cpu_baseline(seconds: float, shard_bytes: int):
     """The oracle's SIMD restatement of the reference CPU path on the host
     cores: RS(16,4) encode (16 -> 4) and an equal-cost reconstruct product
@@ -416,10 +534,11 @@ def main() -> int:
     if args.encode_only:
         print(json.dumps({"kernel": "batch_pass<4>", **head}), flush=True)
         return 0
-    mix = leo = None
+    mix = leo = srv = None
     if not args.quick:
         mix = mixed(args, torch, rs)
         leo = leopard(args, torch, rs)
+        srv = served(args, torch, rs)
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_seconds, 4 << 20)
     out = {
         "metric": METRIC,
@@ -452,6 +571,7 @@ def main() -> int:
         "tuning": head["tuning"],
         "mixed_workload": mix,
         "leopard16": leo,
+        "served": srv,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -154,7 +154,10 @@ typedef struct rsx_batch rsx_batch;
 
 int rsx_batch_create(int device, rsx_batch **out);
 void rsx_batch_free(rsx_batch *b);
-/* filled receives, item after item, TotalShards flags per item. */
+/* filled receives, item after item, TotalShards flags per item.  Re-planning
+ * waits for launches on rsx_batch_stream(b); launches made with
+ * rsx_batch_launch_on must have completed before rsx_batch_plan or
+ * rsx_batch_free is called. */
 int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int nitems, int *status,
                    uint8_t *filled);
 int rsx_batch_launch(rsx_batch *b);       /* stream-ordered on rsx_batch_stream(b) */

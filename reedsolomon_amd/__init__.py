@@ -231,10 +231,10 @@ class Encoder:
         self._h = handle
         self._k, self._m = k, m
 
-    def __del__(self):
+    def __del__(self, _free=lib.rsx_free):
         h = getattr(self, "_h", None)
         if h:
-            lib.rsx_free(h)
+            _free(h)
             self._h = None
 
     # Extensions (reedsolomon.go:137-165)
@@ -413,10 +413,10 @@ class Batch:
         self._h = h.value
         self._keep = []
 
-    def __del__(self):
+    def __del__(self, _free=lib.rsx_batch_free):
         h = getattr(self, "_h", None)
         if h:
-            lib.rsx_batch_free(h)
+            _free(h)
             self._h = None
 
     def set_tuning(self, tile_bytes: int = 0, blocks_per_cu: int = 0) -> None:
@@ -518,10 +518,10 @@ class Server:
         self._h = h.value
         self._pending = {}
 
-    def close(self):
+    def close(self, _free=lib.rsx_server_free):
         h = getattr(self, "_h", None)
         if h:
-            lib.rsx_server_free(h)
+            _free(h)
             self._h = None
             self._pending.clear()
 

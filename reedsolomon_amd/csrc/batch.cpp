@@ -9,6 +9,7 @@
 #include <chrono>
 #include <cstring>
 #include <map>
+#include <tuple>
 
 #include "batch.hpp"
 #include "codec.hpp"
@@ -263,17 +264,20 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
     std::vector<VerifyOut> vouts;
     std::vector<std::pair<size_t, size_t>> scratch_patch;  // (pool index, scratch offset)
     size_t scratch_bytes = 0;
-    std::map<const void *, size_t> table_off;  // source tables -> offset in batch copy (dwords)
-    std::vector<std::pair<const uint32_t *, size_t>> table_src;  // (source, dwords)
-    size_t table_dwords = 0;
+    // Every table the plan reads is copied into `htab` (host) when first
+    // seen and uploaded once; keyed by (host table, codec, decode-cache
+    // epoch) so a table freed by a cache flush is never matched again.
+    std::map<std::tuple<const void *, const rsx_codec *, uint64_t>, size_t> table_off;
+    std::vector<uint32_t> htab;
     size_t fill_off = 0;
 
     struct Pending {
         int mc;
         size_t idx;
-        const uint32_t *src;
+        size_t toff;  // dword offset of the task's tables in htab
     };
     std::vector<Pending> pend;
+    DeviceRanges ranges;
 
     for (int it = 0; it < n; ++it) {
         const rsx_batch_item &item = items[it];
@@ -290,7 +294,8 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
         const size_t *lens = item.lens;
         std::vector<const uint8_t *> in;
         std::vector<uint8_t *> out;
-        const uint32_t *tables = nullptr;
+        const uint32_t *tables = nullptr;  // host copy of the item's tables
+        uint64_t epoch = ~uint64_t{0};
         int mtot = 0;
         int err_item = RSX_OK;
         if (item.op == RSX_OP_ENCODE || item.op == RSX_OP_VERIFY) {
@@ -324,7 +329,7 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
             } else {
                 for (int i = k; i < total; ++i) out.push_back(item.shards[i]);
             }
-            tables = c->parity_tables.as<uint32_t>();
+            tables = c->parity_tables_host.data();
             mtot = c->m;
         } else if (item.op == RSX_OP_RECONSTRUCT || item.op == RSX_OP_RECONSTRUCT_DATA) {
             // reedsolomon.go:1670-1709
@@ -351,6 +356,7 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
             {
                 std::lock_guard<std::mutex> lock(c->mu);
                 err = decode_plan(c, lens, data_only, nullptr, &plan);
+                epoch = c->decode_epoch;
             }
             if (err) {
                 status[it] = err;
@@ -361,7 +367,7 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
                 out.push_back(item.shards[i]);
                 if (fl) fl[i] = 1;
             }
-            tables = plan->tables.as<uint32_t>();
+            tables = plan->host_tables.data();
             mtot = static_cast<int>(plan->outputs.size());
         } else {
             status[it] = fail(RSX_ERR_INVALID_INPUT, "unknown batch op");
@@ -383,10 +389,11 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
                     missing |= item.shards[i] == nullptr;
                     ptrs.push_back(item.shards[i]);
                 }
-            MemKind kind = MemKind::Host;
+            bool host = false;
+            if (!missing)
+                for (const void *p : ptrs) host |= !ranges.is_device(p);
             if (missing) err_item = fail(RSX_ERR_INVALID_INPUT, "no buffer for a shard of a batched item");
-            else if ((err_item = classify_all(ptrs, kind)) == RSX_OK && kind != MemKind::Device)
-                err_item = fail(RSX_ERR_MIXED_MEMORY, "batched items need shards in device memory");
+            else if (host) err_item = fail(RSX_ERR_MIXED_MEMORY, "batched items need shards in device memory");
             if (err_item) {
                 if (b->is_verify[it] == 1) {
                     scratch_bytes -= vouts.back().slots.size() * ((lens[0] + 255) / 256 * 256);
@@ -402,12 +409,13 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
         uintptr_t bits = 0;
         for (auto *p : in) bits |= reinterpret_cast<uintptr_t>(p);
         for (auto *p : out) bits |= reinterpret_cast<uintptr_t>(p);
-        if (table_off.find(tables) == table_off.end()) {
-            table_off[tables] = table_dwords;
-            const size_t dw = static_cast<size_t>(k) * mtot * 5;
-            table_src.emplace_back(tables, dw);
-            table_dwords += dw;
+        const auto tkey = std::make_tuple(static_cast<const void *>(tables), static_cast<const rsx_codec *>(c), epoch);
+        auto tf = table_off.find(tkey);
+        if (tf == table_off.end()) {
+            tf = table_off.emplace(tkey, htab.size()).first;
+            htab.insert(htab.end(), tables, tables + static_cast<size_t>(k) * mtot * 5);
         }
+        const size_t toff = tf->second;
         const uint32_t in_off = static_cast<uint32_t>(b->pool.size());
         b->pool.insert(b->pool.end(), in.begin(), in.end());
         for (int o0 = 0; o0 < mtot; o0 += 8) {
@@ -428,7 +436,7 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
             t.o0 = static_cast<uint16_t>(o0);
             t.aligned = (bits & 15) == 0;
             b->tasks[nout].push_back(t);
-            pend.push_back({nout, b->tasks[nout].size() - 1, tables});
+            pend.push_back({nout, b->tasks[nout].size() - 1, toff});
         }
     }
     // Scratch for verify items; patch the placeholder output pointers.
@@ -453,12 +461,11 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
         }
     }
     // Batch-owned copy of every table the plan reads.
-    int err = b->d_tables.reserve(std::max<size_t>(table_dwords, 1) * 4);
+    int err = b->d_tables.reserve(std::max<size_t>(htab.size(), 1) * 4);
     if (err) return err;
-    for (auto &ts : table_src)
-        RSX_HIP(hipMemcpyAsync(b->d_tables.as<uint32_t>() + table_off[ts.first], ts.first, ts.second * 4,
-                               hipMemcpyDeviceToDevice, b->stream));
-    for (auto &p : pend) b->tasks[p.mc][p.idx].tables = b->d_tables.as<uint32_t>() + table_off[p.src];
+    if (!htab.empty())
+        RSX_HIP(hipMemcpyAsync(b->d_tables.ptr, htab.data(), htab.size() * 4, hipMemcpyHostToDevice, b->stream));
+    for (auto &p : pend) b->tasks[p.mc][p.idx].tables = b->d_tables.as<uint32_t>() + p.toff;
     if ((err = b->d_ptrs.reserve(std::max<size_t>(b->pool.size(), 1) * sizeof(void *)))) return err;
     RSX_HIP(hipMemcpyAsync(b->d_ptrs.ptr, b->pool.data(), b->pool.size() * sizeof(void *),
                            hipMemcpyHostToDevice, b->stream));

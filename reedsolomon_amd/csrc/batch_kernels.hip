@@ -81,21 +81,26 @@ __global__ __launch_bounds__(kBlock) void batch_pass(const BatchLaunch L) {
 
 __global__ __launch_bounds__(kBlock) void batch_compare(const CompareEntry *entries, int n,
                                                         uint32_t *flags) {
-    for (int e = blockIdx.x; e < n; e += gridDim.x) {
+    // blockIdx.y strides over entries, blockIdx.x over 64 KiB pieces of one.
+    constexpr uint64_t kPiece = 64u << 10;
+    for (int e = blockIdx.y; e < n; e += gridDim.y) {
         const CompareEntry c = entries[e];
         const bool aligned = ((reinterpret_cast<uintptr_t>(c.a) | reinterpret_cast<uintptr_t>(c.b)) & 15) == 0;
         uint32_t diff = 0;
-        uint64_t b = 0;
-        if (aligned) {
-            const uint64_t nv = c.len >> 4;
-            for (uint64_t v = threadIdx.x; v < nv; v += kBlock) {
-                const uint4 x = reinterpret_cast<const uint4 *>(c.a)[v];
-                const uint4 y = reinterpret_cast<const uint4 *>(c.b)[v];
-                diff |= (x.x ^ y.x) | (x.y ^ y.y) | (x.z ^ y.z) | (x.w ^ y.w);
+        for (uint64_t p0 = blockIdx.x * kPiece; p0 < c.len; p0 += gridDim.x * kPiece) {
+            const uint64_t p1 = p0 + kPiece < c.len ? p0 + kPiece : c.len;
+            uint64_t b = p0;
+            if (aligned) {
+                const uint64_t v0 = p0 >> 4, v1 = p1 >> 4;
+                for (uint64_t v = v0 + threadIdx.x; v < v1; v += kBlock) {
+                    const uint4 x = reinterpret_cast<const uint4 *>(c.a)[v];
+                    const uint4 y = reinterpret_cast<const uint4 *>(c.b)[v];
+                    diff |= (x.x ^ y.x) | (x.y ^ y.y) | (x.z ^ y.z) | (x.w ^ y.w);
+                }
+                b = v1 << 4;
             }
-            b = nv << 4;
+            for (uint64_t i = b + threadIdx.x; i < p1; i += kBlock) diff |= c.a[i] ^ c.b[i];
         }
-        for (uint64_t i = b + threadIdx.x; i < c.len; i += kBlock) diff |= c.a[i] ^ c.b[i];
         if (diff) flags[c.item] = 1;
     }
 }
@@ -104,7 +109,7 @@ __global__ __launch_bounds__(kBlock) void batch_compare(const CompareEntry *entr
 
 hipError_t launch_batch_compare(const CompareEntry *entries, int n, uint32_t *flags, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(batch_compare, dim3(static_cast<unsigned>(std::min(n, 65535))), dim3(kBlock), 0, s,
+    hipLaunchKernelGGL(batch_compare, dim3(16, static_cast<unsigned>(std::min(n, 65535))), dim3(kBlock), 0, s,
                        entries, n, flags);
     return hipGetLastError();
 }

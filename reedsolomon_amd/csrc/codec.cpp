@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cstring>
 
+#include "batch.hpp"
 #include "kernels.hpp"
 #include "leopard.hpp"
 
@@ -85,6 +86,27 @@ MemKind classify(const void *p) {
     return MemKind::Host;
 }
 
+bool DeviceRanges::is_device(const void *p) {
+    const uintptr_t a = reinterpret_cast<uintptr_t>(p);
+    if (last_ < ranges_.size() && a >= ranges_[last_].first && a < ranges_[last_].second) return true;
+    for (size_t i = 0; i < ranges_.size(); ++i)
+        if (a >= ranges_[i].first && a < ranges_[i].second) {
+            last_ = i;
+            return true;
+        }
+    if (classify(p) != MemKind::Device) return false;
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, const_cast<void *>(p)) == hipSuccess && size > 0) {
+        const uintptr_t b = reinterpret_cast<uintptr_t>(base);
+        ranges_.emplace_back(b, b + size);
+        last_ = ranges_.size() - 1;
+    } else {
+        (void)hipGetLastError();
+    }
+    return true;
+}
+
 // All pointers of one call must share a memory class.
 int classify_all(const std::vector<const void *> &ptrs, MemKind &kind) {
     bool have = false;
@@ -126,11 +148,11 @@ int run_matmul(rsx_codec *c, const uint32_t *tables, const std::vector<const uin
 }
 
 // Uploads coefficient tables for `rows` (nrows x nin) into a [nin][nrows][5]
-// device buffer.
+// device buffer and leaves the same tables in `host`.
 int upload_tables(const std::vector<const uint8_t *> &rows, int nin, DeviceBuffer &dst,
-                  hipStream_t stream) {
+                  std::vector<uint32_t> &host, hipStream_t stream) {
     const int nrows = static_cast<int>(rows.size());
-    std::vector<uint32_t> host(static_cast<size_t>(nin) * nrows * kTableDwords);
+    host.assign(static_cast<size_t>(nin) * nrows * kTableDwords, 0);
     for (int j = 0; j < nin; ++j)
         for (int i = 0; i < nrows; ++i)
             coefficient_table(rows[i][j], &host[(static_cast<size_t>(j) * nrows + i) * kTableDwords]);
@@ -231,7 +253,7 @@ int rsx_new(int k, int m, const rsx_options *opt, rsx_codec **out) {
         if (err) return fail(err, "coding matrix construction failed");
         std::vector<const uint8_t *> rows(m);
         for (int i = 0; i < m; ++i) rows[i] = c->matrix.row(k + i);
-        err = upload_tables(rows, k, c->parity_tables, c->stream);
+        err = upload_tables(rows, k, c->parity_tables, c->parity_tables_host, c->stream);
         if (err) return err;
     }
     *out = c.release();
@@ -390,10 +412,14 @@ int rsx_verify(rsx_codec *c, uint8_t *const *shards, const size_t *lens, int nsh
     std::vector<uint8_t *> out(c->m);
     for (int i = 0; i < c->m; ++i) out[i] = c->scratch.as<uint8_t>() + pitch * i;
     if ((err = run_matmul(c, c->parity_tables.as<uint32_t>(), in, out, len, false))) return err;
+    std::vector<CompareEntry> cmp(c->m);
+    for (int i = 0; i < c->m; ++i) cmp[i] = {out[i], dev[c->k + i], len, 0, 0};
+    if ((err = c->cmp.reserve(cmp.size() * sizeof(CompareEntry)))) return err;
+    RSX_HIP(hipMemcpyAsync(c->cmp.ptr, cmp.data(), cmp.size() * sizeof(CompareEntry), hipMemcpyHostToDevice,
+                           c->stream));
     RSX_HIP(launch_fill_u32(c->flag.as<uint32_t>(), 1, 0u, c->stream));
-    for (int i = 0; i < c->m; ++i)
-        RSX_HIP(launch_compare(out[i], dev[c->k + i], len, c->flag.as<int>(), c->stream));
-    int mismatch = 0;
+    RSX_HIP(launch_batch_compare(c->cmp.as<CompareEntry>(), c->m, c->flag.as<uint32_t>(), c->stream));
+    uint32_t mismatch = 0;
     RSX_HIP(hipMemcpyAsync(&mismatch, c->flag.ptr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     RSX_HIP(hipStreamSynchronize(c->stream));
     *ok = mismatch == 0;
@@ -425,6 +451,7 @@ int decode_plan(rsx_codec *c, const size_t *lens, bool data_only, const uint8_t 
     if (c->decode_cache.size() >= kDecodeCacheCap) {
         RSX_HIP(hipStreamSynchronize(c->stream));
         c->decode_cache.clear();
+        ++c->decode_epoch;
     }
     auto p = std::make_unique<DecodePlan>();
     // reedsolomon.go:1718-1730: the first k present rows.
@@ -455,7 +482,7 @@ int decode_plan(rsx_codec *c, const size_t *lens, bool data_only, const uint8_t 
     std::vector<const uint8_t *> rp;
     for (auto &r : rows) rp.push_back(r.data());
     if (!rp.empty()) {
-        int err = upload_tables(rp, k, p->tables, c->stream);
+        int err = upload_tables(rp, k, p->tables, p->host_tables, c->stream);
         if (err) return err;
     }
     *plan = p.get();

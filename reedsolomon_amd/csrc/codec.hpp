@@ -45,11 +45,25 @@ struct DecodePlan {
     std::vector<int> inputs;   // k surviving shard indices (first k present)
     std::vector<int> outputs;  // shard indices written, in row order
     DeviceBuffer tables;       // [k][outputs.size()][5]
+    std::vector<uint32_t> host_tables;  // the same tables in host memory
 };
 
 // Memory class of one call's shards.
 enum class MemKind { Device, Host };
 int classify_all(const std::vector<const void *> &ptrs, MemKind &kind);
+
+// Classifies many pointers with one runtime query per device allocation:
+// the address range of each device allocation seen is remembered, so a
+// pointer inside a known range needs no query.  Valid while the caller
+// frees nothing, i.e. for the span of one call.
+class DeviceRanges {
+  public:
+    bool is_device(const void *p);
+
+  private:
+    std::vector<std::pair<uintptr_t, uintptr_t>> ranges_;  // [base, end)
+    size_t last_ = 0;
+};
 
 struct LeopardState;  // leopard.cpp
 struct LeopardDeleter {
@@ -69,12 +83,15 @@ struct rsx_codec {
     // classic engine
     rsx::Matrix matrix;              // (k+m) x k
     rsx::DeviceBuffer parity_tables; // [k][m][5]
+    std::vector<uint32_t> parity_tables_host;
 
     std::mutex mu;  // guards the caches and the staging buffers below
     std::unordered_map<std::string, std::unique_ptr<rsx::DecodePlan>> decode_cache;
+    uint64_t decode_epoch = 0;  // incremented whenever decode_cache is flushed
     rsx::DeviceBuffer staging;  // host-memory calls: shards staged in HBM
     rsx::DeviceBuffer scratch;  // Verify: recomputed parity
     rsx::DeviceBuffer flag;     // Verify: mismatch flag
+    rsx::DeviceBuffer cmp;      // Verify: compare list
 
     std::unique_ptr<rsx::LeopardState, rsx::LeopardDeleter> leo;
 };

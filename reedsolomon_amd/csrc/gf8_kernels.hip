@@ -80,15 +80,6 @@ __global__ __launch_bounds__(kBlock) void gf8_pass_bytes(const Gf8Pass p, size_t
     }
 }
 
-__global__ __launch_bounds__(kBlock) void compare_bytes(const uint8_t *a, const uint8_t *b,
-                                                        size_t n, int *flag) {
-    const size_t stride = static_cast<size_t>(gridDim.x) * kBlock;
-    int diff = 0;
-    for (size_t i = static_cast<size_t>(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride)
-        diff |= a[i] != b[i];
-    if (diff) *flag = 1;
-}
-
 __global__ __launch_bounds__(kBlock) void fill_u32(uint32_t *p, size_t n, uint32_t v) {
     const size_t stride = static_cast<size_t>(gridDim.x) * kBlock;
     for (size_t i = static_cast<size_t>(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride) p[i] = v;
@@ -159,16 +150,6 @@ hipError_t launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t stream
     size_t g = (n + kBlock - 1) / kBlock;
     if (g > 4096) g = 4096;
     hipLaunchKernelGGL(fill_u32, dim3(static_cast<unsigned>(g)), dim3(kBlock), 0, stream, p, n, v);
-    return hipGetLastError();
-}
-
-hipError_t launch_compare(const uint8_t *a, const uint8_t *b, size_t n, int *flag,
-                          hipStream_t stream) {
-    if (n == 0) return hipSuccess;
-    size_t g = (n + kBlock * 16 - 1) / (kBlock * 16);
-    if (g > 4096) g = 4096;
-    hipLaunchKernelGGL(compare_bytes, dim3(static_cast<unsigned>(g)), dim3(kBlock), 0, stream, a, b,
-                       n, flag);
     return hipGetLastError();
 }
 

@@ -32,8 +32,4 @@ hipError_t launch_gf8_pass(const Gf8Pass &p, hipStream_t stream);
 // plain kernel node if a caller captures it in a HIP graph).
 hipError_t launch_fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t stream);
 
-// *flag = 1 if any of the n bytes of a and b differ (Verify).
-hipError_t launch_compare(const uint8_t *a, const uint8_t *b, size_t n, int *flag,
-                          hipStream_t stream);
-
 }  // namespace rsx

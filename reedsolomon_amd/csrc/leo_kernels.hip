@@ -3,10 +3,9 @@
 // Thread layout: blockIdx.y picks the butterfly unit (so its skews, and the
 // "skew == modulus => xor only" branch, are uniform across the block) and the
 // lanes of the block spread over the column chunk: one lane owns one
-// 16-element column unit of every row it touches.  GF(2^16) products use the
-// split lo/hi byte layout directly: 12 v_perm_b32 byte selects per 4 elements
-// (3 bit groups x 2 source bytes x 2 product bytes), folded with 3-input XORs.
+// 16-element column unit of every row it touches.
 #include "leo_kernels.hpp"
+#include "leo_device.hpp"
 
 #include <algorithm>
 
@@ -16,108 +15,7 @@ namespace {
 
 constexpr int kBlock = 256;
 
-using ConstU32 = const __attribute__((address_space(4))) uint32_t;
-
-__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
-    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
-}
-__device__ __forceinline__ uint32_t sel(uint32_t hi, uint32_t lo, uint32_t idx) {
-    return __builtin_amdgcn_perm(hi, lo, idx);
-}
-__device__ __forceinline__ uint4 xor4(const uint4 &a, const uint4 &b) {
-    return make_uint4(a.x ^ b.x, a.y ^ b.y, a.z ^ b.z, a.w ^ b.w);
-}
-__device__ __forceinline__ ConstU32 *as_const(const uint32_t *p) {
-    return (ConstU32 *)(reinterpret_cast<uintptr_t>(p));
-}
-
-// ---- GF(2^16): 16 elements per unit, lo bytes and hi bytes in two uint4 ----
-struct F16 {
-    struct E {
-        uint4 lo, hi;
-    };
-    static constexpr int kUnitBytes = 32;
-    static constexpr int kTW = 20;
-    static constexpr unsigned kMod = 65535;
-
-    __device__ static size_t offset(int u) { return static_cast<size_t>(u >> 1) * 64 + (u & 1) * 16; }
-    __device__ static E load(const uint8_t *row, int u) {
-        const uint8_t *c = row + offset(u);
-        return {*reinterpret_cast<const uint4 *>(c), *reinterpret_cast<const uint4 *>(c + 32)};
-    }
-    __device__ static void store(uint8_t *row, int u, const E &e) {
-        uint8_t *c = row + offset(u);
-        *reinterpret_cast<uint4 *>(c) = e.lo;
-        *reinterpret_cast<uint4 *>(c + 32) = e.hi;
-    }
-    __device__ static E zero() { return {make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0)}; }
-    __device__ static void xor_into(E &x, const E &y) {
-        x.lo = xor4(x.lo, y.lo);
-        x.hi = xor4(x.hi, y.hi);
-    }
-    // (lo, hi) of one dword pair times the table's element, XORed into (xl, xh).
-    __device__ static void mac1(uint32_t &xl, uint32_t &xh, uint32_t L, uint32_t H, ConstU32 *t) {
-        const uint32_t a0 = L & 0x07070707u, a1 = (L >> 3) & 0x07070707u, a2 = (L >> 6) & 0x03030303u;
-        const uint32_t b0 = H & 0x07070707u, b1 = (H >> 3) & 0x07070707u, b2 = (H >> 6) & 0x03030303u;
-        uint32_t r = xor3(xl, sel(t[1], t[0], a0), sel(t[5], t[4], a1));
-        r = xor3(r, sel(t[8], t[8], a2), sel(t[11], t[10], b0));
-        xl = xor3(r, sel(t[15], t[14], b1), sel(t[18], t[18], b2));
-        r = xor3(xh, sel(t[3], t[2], a0), sel(t[7], t[6], a1));
-        r = xor3(r, sel(t[9], t[9], a2), sel(t[13], t[12], b0));
-        xh = xor3(r, sel(t[17], t[16], b1), sel(t[19], t[19], b2));
-    }
-    __device__ static void mul_add(E &x, const E &y, ConstU32 *t) {
-        mac1(x.lo.x, x.hi.x, y.lo.x, y.hi.x, t);
-        mac1(x.lo.y, x.hi.y, y.lo.y, y.hi.y, t);
-        mac1(x.lo.z, x.hi.z, y.lo.z, y.hi.z, t);
-        mac1(x.lo.w, x.hi.w, y.lo.w, y.hi.w, t);
-    }
-};
-
-// ---- GF(2^8): 16 one-byte elements per unit ----
-struct F8 {
-    typedef uint4 E;
-    static constexpr int kUnitBytes = 16;
-    static constexpr int kTW = 5;
-    static constexpr unsigned kMod = 255;
-
-    __device__ static E load(const uint8_t *row, int u) {
-        return reinterpret_cast<const uint4 *>(row)[u];
-    }
-    __device__ static void store(uint8_t *row, int u, const E &e) {
-        reinterpret_cast<uint4 *>(row)[u] = e;
-    }
-    __device__ static E zero() { return make_uint4(0, 0, 0, 0); }
-    __device__ static void xor_into(E &x, const E &y) { x = xor4(x, y); }
-    __device__ static uint32_t mac1(uint32_t x, uint32_t v, ConstU32 *t) {
-        const uint32_t g0 = v & 0x07070707u, g1 = (v >> 3) & 0x07070707u, g2 = (v >> 6) & 0x03030303u;
-        return xor3(xor3(x, sel(t[1], t[0], g0), sel(t[3], t[2], g1)), sel(t[4], t[4], g2), 0u);
-    }
-    __device__ static void mul_add(E &x, const E &y, ConstU32 *t) {
-        x.x = mac1(x.x, y.x, t);
-        x.y = mac1(x.y, y.y, t);
-        x.z = mac1(x.z, y.z, t);
-        x.w = mac1(x.w, y.w, t);
-    }
-};
-
-template <class F>
-__device__ __forceinline__ ConstU32 *table(const LeoDev &f, unsigned log_m) {
-    return as_const(f.tables) + static_cast<size_t>(log_m) * F::kTW;
-}
-
-// galois_noasm.go:94-113: fftDIT2 = mulAdd then xor, ifftDIT2 = xor then
-// mulAdd; a skew equal to the modulus means "xor only" (leopard.go:741-760).
-template <class F>
-__device__ __forceinline__ void fft2(typename F::E &x, typename F::E &y, unsigned lm, const LeoDev &f) {
-    if (lm != F::kMod) F::mul_add(x, y, table<F>(f, lm));
-    F::xor_into(y, x);
-}
-template <class F>
-__device__ __forceinline__ void ifft2(typename F::E &x, typename F::E &y, unsigned lm, const LeoDev &f) {
-    F::xor_into(y, x);
-    if (lm != F::kMod) F::mul_add(x, y, table<F>(f, lm));
-}
+using namespace leo;
 
 template <class F, bool INV>
 __global__ __launch_bounds__(kBlock) void k_step4(const LeoDev f, const LeoWork w, int dist,

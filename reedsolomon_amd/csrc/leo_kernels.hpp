@@ -56,4 +56,35 @@ hipError_t leo_derivative(const LeoWork &in, const LeoWork &out, int n, hipStrea
 hipError_t leo_store_rows(const LeoDev &f, const LeoWork &w, const int *rows, const uint16_t *logs,
                           uint8_t *const *dst, size_t col_off, int count, hipStream_t s);
 
+// ---- LDS-resident transforms (leo_fused.hip) ----
+// Encode keeps the accumulator and one group's rows of a column slice in LDS
+// (2m rows, m when k <= m); reconstruct keeps all n rows.
+constexpr size_t kLeoFusedEncodeLds = 128u << 10;
+constexpr size_t kLeoFusedDecodeLds = 64u << 10;
+
+struct LeoEncodeArgs {
+    const uint8_t *const *data;  // groups*m row pointers, group-major, null past row k
+    uint8_t *const *parity;      // p row pointers
+    size_t size;                 // bytes per shard (multiple of 64)
+    int k, p, m, groups;         // m = ceilPow2(p)
+    int slice;                   // bytes per row per workgroup (power of two >= 64)
+};
+
+struct LeoDecodeArgs {
+    const uint8_t *const *rows;  // n row pointers: parity, zeros, data, zeros (null = zero/erased)
+    const uint16_t *errlocs;     // n logs multiplied into the rows as they load
+    const int *out_rows;         // nout work rows written back ...
+    const uint16_t *out_logs;    // ... times exp(out_logs[q]) ...
+    uint8_t *const *out;         // ... to out[q]
+    size_t size;
+    int n, nout, ifft_trunc, fft_trunc;
+    int slice;
+};
+
+// Slice width for `rows` LDS rows and radix-4 steps over `step_rows` rows, or
+// 0 when even 64-byte slices exceed `lds_budget`.
+int leo_fused_slice(int bits, int rows, int step_rows, size_t lds_budget);
+hipError_t leo_encode_fused(const LeoDev &f, const LeoEncodeArgs &a, hipStream_t s);
+hipError_t leo_decode_fused(const LeoDev &f, const LeoDecodeArgs &a, hipStream_t s);
+
 }  // namespace rsx

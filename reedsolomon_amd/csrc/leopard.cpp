@@ -14,9 +14,11 @@
 #include "leopard.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 
+#include "batch.hpp"
 #include "kernels.hpp"
 #include "leo_kernels.hpp"
 #include "leopard_field.hpp"
@@ -66,9 +68,10 @@ struct LeoPattern {
 
 struct LeopardState {
     int bits = 16;
+    bool fused = true;  // LDS-resident kernels where the rows fit
     const LeoField *field = nullptr;
     const LeoDev *dev = nullptr;
-    DeviceBuffer work, work2, ptrs, rows, logs, stage, scratch, flag;
+    DeviceBuffer work, work2, ptrs, rows, logs, stage, scratch, flag, cmp;
     std::vector<uint8_t *> h_ptrs;
     std::unordered_map<std::string, std::unique_ptr<LeoPattern>> patterns;
 };
@@ -109,6 +112,10 @@ int leopard_init(rsx_codec *c) {
     std::unique_ptr<LeopardState, LeopardDeleter> s(new LeopardState);
     s->bits = c->engine == RSX_LEOPARD_GF8 ? 8 : 16;
     s->field = &leo_field(s->bits);
+    // RSX_LEOPARD_LAYERED=1 keeps every codec created afterwards on the
+    // layer-per-launch kernels (both schedules give the same bytes).
+    const char *layered = std::getenv("RSX_LEOPARD_LAYERED");
+    s->fused = !(layered && layered[0] == '1');
     int err = device_tables(c->device, s->bits, c->stream, &s->dev);
     if (err) return err;
     c->leo = std::move(s);
@@ -208,10 +215,16 @@ int encode_device(rsx_codec *c, const std::vector<uint8_t *> &data, const std::v
     if ((err = s.rows.reserve(p * sizeof(int)))) return err;
     RSX_HIP(hipMemcpyAsync(s.rows.ptr, rows.data(), p * sizeof(int), hipMemcpyHostToDevice, c->stream));
 
-    const size_t chunk = chunk_for(size, 2 * m, kEncodeBudget);
-    if ((err = s.work.reserve(chunk * 2 * m))) return err;
     const uint8_t *const *dptr = reinterpret_cast<const uint8_t *const *>(s.ptrs.ptr);
     uint8_t *const *pptr = reinterpret_cast<uint8_t *const *>(s.ptrs.as<uint8_t *>() + static_cast<size_t>(groups) * m);
+    const int slice = s.fused ? leo_fused_slice(s.bits, groups > 1 ? 2 * m : m, m, kLeoFusedEncodeLds) : 0;
+    if (slice) {
+        const LeoEncodeArgs a{dptr, pptr, size, k, p, m, groups, slice};
+        RSX_HIP(leo_encode_fused(*s.dev, a, c->stream));
+        return RSX_OK;
+    }
+    const size_t chunk = chunk_for(size, 2 * m, kEncodeBudget);
+    if ((err = s.work.reserve(chunk * 2 * m))) return err;
     for (size_t off = 0; off < size; off += chunk) {
         const size_t cols = std::min(chunk, size - off);
         LeoWork acc{s.work.as<uint8_t>(), chunk, cols};
@@ -300,12 +313,19 @@ int reconstruct_device(rsx_codec *c, const std::vector<uint8_t *> &shards, const
         RSX_HIP(hipMemcpyAsync(s.logs.ptr, logs.data(), nout * sizeof(uint16_t), hipMemcpyHostToDevice, c->stream));
     }
     const int ifft_trunc = s.bits == 16 ? pat->input_count : m + k;
+    const uint8_t *const *lptr = reinterpret_cast<const uint8_t *const *>(s.ptrs.ptr);
+    uint8_t *const *optr = reinterpret_cast<uint8_t *const *>(s.ptrs.as<uint8_t *>() + n);
+    const int slice = s.fused ? leo_fused_slice(s.bits, n, n, kLeoFusedDecodeLds) : 0;
+    if (slice) {
+        const LeoDecodeArgs a{lptr, pat->d_errlocs.as<uint16_t>(), s.rows.as<int>(), s.logs.as<uint16_t>(), optr,
+                              size, n, nout, ifft_trunc, m + k, slice};
+        if (nout) RSX_HIP(leo_decode_fused(*s.dev, a, c->stream));
+        return RSX_OK;
+    }
     const size_t chunk = chunk_for(size, 2 * n, kDecodeBudget);
     if ((err = s.work.reserve(chunk * n))) return err;
     if ((err = s.work2.reserve(chunk * n))) return err;
     const LeoDev &f = *s.dev;
-    const uint8_t *const *lptr = reinterpret_cast<const uint8_t *const *>(s.ptrs.ptr);
-    uint8_t *const *optr = reinterpret_cast<uint8_t *const *>(s.ptrs.as<uint8_t *>() + n);
     for (size_t off = 0; off < size; off += chunk) {
         const size_t cols = std::min(chunk, size - off);
         LeoWork w{s.work.as<uint8_t>(), chunk, cols};
@@ -433,9 +453,13 @@ int leopard_verify(rsx_codec *c, uint8_t *const *shards, const size_t *lens, int
     std::vector<uint8_t *> data(r.dev.begin(), r.dev.begin() + k), tmp(p);
     for (int i = 0; i < p; ++i) tmp[i] = s.scratch.as<uint8_t>() + size * i;
     if ((err = encode_device(c, data, tmp, size))) return err;
+    std::vector<CompareEntry> cmp(p);
+    for (int i = 0; i < p; ++i) cmp[i] = {tmp[i], r.dev[k + i], size, 0, 0};
+    if ((err = s.cmp.reserve(p * sizeof(CompareEntry)))) return err;
+    RSX_HIP(hipMemcpyAsync(s.cmp.ptr, cmp.data(), p * sizeof(CompareEntry), hipMemcpyHostToDevice, c->stream));
     RSX_HIP(launch_fill_u32(s.flag.as<uint32_t>(), 1, 0u, c->stream));
-    for (int i = 0; i < p; ++i) RSX_HIP(launch_compare(tmp[i], r.dev[k + i], size, s.flag.as<int>(), c->stream));
-    int mismatch = 0;
+    RSX_HIP(launch_batch_compare(s.cmp.as<CompareEntry>(), p, s.flag.as<uint32_t>(), c->stream));
+    uint32_t mismatch = 0;
     RSX_HIP(hipMemcpyAsync(&mismatch, s.flag.ptr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     RSX_HIP(hipStreamSynchronize(c->stream));
     *ok = mismatch == 0;

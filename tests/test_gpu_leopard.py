@@ -162,3 +162,35 @@ def test_leopard8_192_56_64kib():
     enc.Reconstruct(work)
     for i in lost:
         assert np.array_equal(host(work[i]), want[i]), i
+
+
+# Shapes x sizes that give several column slices and a narrower last slice,
+# run on both schedules: every work row of a slice in LDS (the default where
+# the rows fit) and one launch per butterfly layer (RSX_LEOPARD_LAYERED=1).
+SCHEDULE_CASES = [(8, 192, 56, 64 * 37), (8, 17, 3, 64 * 129), (8, 128, 128, 64 * 5), (8, 3, 2, 64 * 1000),
+                  (16, 100, 20, 64 * 41), (16, 300, 300, 64 * 3), (16, 33, 7, 64 * 300), (16, 5, 3, 64 * 777)]
+
+
+@pytest.mark.parametrize("bits,k,m,size", SCHEDULE_CASES)
+def test_lds_and_layered_schedules_match_oracle(bits, k, m, size, monkeypatch):
+    rng = np.random.default_rng(size + k)
+    data = [rng.integers(0, 256, size, dtype=np.uint8) for _ in range(k)]
+    o, want = oracle_encode(bits, k, m, data)
+    lost = set(rng.choice(k + m, m, replace=False).tolist())
+    ref = [None if i in lost else want[i].copy() for i in range(k + m)]
+    assert o.reconstruct(ref) == 0
+    for layered in (False, True):
+        if layered:
+            monkeypatch.setenv("RSX_LEOPARD_LAYERED", "1")
+        else:
+            monkeypatch.delenv("RSX_LEOPARD_LAYERED", raising=False)
+        enc = rs.New(k, m, opt(bits))
+        shards = [dev(d) for d in data] + [torch.zeros(size, dtype=torch.uint8, device="cuda") for _ in range(m)]
+        enc.Encode(shards)
+        for i in range(m):
+            assert np.array_equal(host(shards[k + i]), want[k + i]), (layered, i)
+        assert enc.Verify(shards)
+        work = [None if i in lost else shards[i] for i in range(k + m)]
+        enc.Reconstruct(work)
+        for i in range(k + m):
+            assert np.array_equal(host(work[i]), ref[i]), (layered, i)
