@@ -9,7 +9,8 @@ ways: the plain path (one C-ABI call per stripe) and the batched path (one
 planned, graph-replayed launch for all encodes, one for all reconstructs).
 `value` is the batched path; the plain path is reported beside it.
 
-Also measured (unless --quick): the mixed workload (8192 stripes, (k, m) in
+Also measured (unless --quick): RS(10,4) over 1 MiB shards (1024 stripes,
+batched encode + reconstruct), the mixed workload (8192 stripes, (k, m) in
 {(4,2),(6,3),(10,4),(17,3),(32,8)}, shards log-uniform 4 KiB-1 MiB rounded to
 64 B, 30% reconstructs with 1..m erasures) batched vs plain, and Leopard
 GF(2^16) k=4096, m=1024, 64 KiB shards (encode + reconstruct of 1024 erasures).
@@ -248,6 +249,52 @@ def headline(args, torch, rs):
                              "gbps": round(n * S / avg_plain / 1e9, 1)},
         tuning={"tile_bytes": tile, "blocks_per_cu": bpc},
     )
+    del st, benc, brec
+    torch.cuda.empty_cache()
+    return out
+
+
+def rs10_4(args, torch, rs):
+    """RS(10,4) over 1 MiB shards (the 70%-of-HBM-peak target shape): 1024
+    stripes, batched encode of all, then batched reconstruct after 0-4 seeded
+    erasures per stripe."""
+    k, m, S, nst = 10, 4, 1 << 20, 1024
+    n = k + m
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(1014)
+    st = Stripes([(k, m)] * nst, [S] * nst, torch, gen)
+    enc = rs.New(k, m)
+    rng = np.random.default_rng(1014)
+    lens, rec_bytes, phys = [], 0, 0
+    for _ in range(nst):
+        e = int(rng.integers(0, m + 1))
+        lost = set(rng.choice(n, e, replace=False).tolist()) if e else set()
+        lens.append((ctypes.c_size_t * n)(*[0 if i in lost else S for i in range(n)]))
+        rec_bytes += n * S if e else 0
+        phys += (k + e) * S if e else 0
+    benc, brec = rs.Batch(0), rs.Batch(0)
+    plan(rs, benc, batch_items(rs, st, {(k, m): enc}, [rs.OP_ENCODE] * nst, st.full), nst)
+    plan(rs, brec, batch_items(rs, st, {(k, m): enc}, [rs.OP_RECONSTRUCT] * nst, lens), nst)
+    sbat = benc.stream
+
+    def step():
+        benc.launch()
+        brec.launch(sbat)
+
+    for _ in range(max(args.warmup, 1)):
+        step()
+    benc.synchronize()
+    t = event_time(torch, sbat, step, args.steps)
+    t_enc = event_time(torch, sbat, benc.launch, args.steps)
+    step_bytes = nst * n * S + rec_bytes
+    gbps = step_bytes / t / 1e9
+    _log(f"bench: RS(10,4) 1 MiB batched {gbps:.1f} GB/s ({100 * gbps / HBM_PEAK_GBPS:.1f}% of peak)")
+    out = dict(stripes=nst, shard_bytes=S, bytes_per_step=step_bytes, ms_per_step=round(t * 1e3, 3),
+               gbps=round(gbps, 1), pct_hbm_peak=round(100 * gbps / HBM_PEAK_GBPS, 2),
+               physical_gbps=round((nst * n * S + phys) / t / 1e9, 1),
+               encode_launch_gbps=round(nst * n * S / t_enc / 1e9, 1),
+               encode_pct_hbm_peak=round(100 * nst * n * S / t_enc / 1e9 / HBM_PEAK_GBPS, 2),
+               tuning=dict(zip(("tile_bytes", "blocks_per_cu"), benc.tuning())))
     del st, benc, brec
     torch.cuda.empty_cache()
     return out
@@ -534,8 +581,9 @@ def main() -> int:
     if args.encode_only:
         print(json.dumps({"kernel": "batch_pass<4>", **head}), flush=True)
         return 0
-    mix = leo = srv = None
+    mix = leo = srv = r104 = None
     if not args.quick:
+        r104 = rs10_4(args, torch, rs)
         mix = mixed(args, torch, rs)
         leo = leopard(args, torch, rs)
         srv = served(args, torch, rs)
@@ -569,6 +617,7 @@ def main() -> int:
         "roofline": head["roofline"],
         "plain_encode_launch": head["plain_encode_launch"],
         "tuning": head["tuning"],
+        "rs10_4_1mib": r104,
         "mixed_workload": mix,
         "leopard16": leo,
         "served": srv,

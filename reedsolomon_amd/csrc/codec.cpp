@@ -110,9 +110,10 @@ bool DeviceRanges::is_device(const void *p) {
 // All pointers of one call must share a memory class.
 int classify_all(const std::vector<const void *> &ptrs, MemKind &kind) {
     bool have = false;
+    DeviceRanges ranges;
     for (const void *p : ptrs) {
         if (!p) continue;
-        MemKind k = classify(p);
+        const MemKind k = ranges.is_device(p) ? MemKind::Device : MemKind::Host;
         if (!have) {
             kind = k;
             have = true;

@@ -58,8 +58,10 @@ hipError_t leo_store_rows(const LeoDev &f, const LeoWork &w, const int *rows, co
 
 // ---- LDS-resident transforms (leo_fused.hip) ----
 // Encode keeps the accumulator and one group's rows of a column slice in LDS
-// (2m rows, m when k <= m); reconstruct keeps all n rows.
-constexpr size_t kLeoFusedEncodeLds = 128u << 10;
+// (2 regions of m rows, 1 when k <= m); reconstruct keeps all n rows.  A
+// region of R rows takes slice * (R + 1) bytes.  Reconstruct also holds
+// n * slice bytes in registers for the formal derivative.
+constexpr size_t kLeoFusedEncodeLds = 160u << 10;
 constexpr size_t kLeoFusedDecodeLds = 64u << 10;
 
 struct LeoEncodeArgs {
@@ -81,10 +83,33 @@ struct LeoDecodeArgs {
     int slice;
 };
 
-// Slice width for `rows` LDS rows and radix-4 steps over `step_rows` rows, or
-// 0 when even 64-byte slices exceed `lds_budget`.
-int leo_fused_slice(int bits, int rows, int step_rows, size_t lds_budget);
+// Slice width for `regions` LDS regions of `rows` rows, radix-4 steps over
+// `step_rows` rows and shards of `size` bytes, or 0 when even 64-byte slices
+// exceed `lds_budget`.
+int leo_fused_slice(int bits, int rows, int regions, int step_rows, size_t size, size_t lds_budget);
 hipError_t leo_encode_fused(const LeoDev &f, const LeoEncodeArgs &a, hipStream_t s);
 hipError_t leo_decode_fused(const LeoDev &f, const LeoDecodeArgs &a, hipStream_t s);
+
+// ---- GF(2^16) reconstruct in three LDS-tiled passes (leo_multipass.hip) ----
+// For n >= 2048 rows: work arrays hold `slices` 64-byte column slices of all
+// n rows, slice-major.
+struct LeoMultipassArgs {
+    const uint8_t *const *rows;  // n row pointers (null = zero / erased)
+    const uint16_t *errlocs;     // n logs multiplied into the rows as they load
+    const int *out_index;        // n: output number of each work row, or -1
+    const uint16_t *out_logs;    // nout logs of the reveal multiply
+    uint8_t *const *out;         // nout destinations
+    uint8_t *work, *work2;       // n * 64 bytes per slice each
+    size_t col0;                 // first shard byte of this column chunk
+    int slices;                  // 64-byte slices in this chunk
+    int n, ifft_trunc, fft_trunc;
+    int C, P, A;                 // tile shapes from leo_multipass_shape
+};
+
+// Tile shapes for n rows; false when n is out of the multipass range.
+bool leo_multipass_shape(int n, int *C, int *P, int *A);
+// Column chunk (bytes) whose two work arrays stay within 128 MiB.
+size_t leo_multipass_chunk(int n, size_t size);
+hipError_t leo_decode_multipass(const LeoDev &f, const LeoMultipassArgs &a, hipStream_t s);
 
 }  // namespace rsx

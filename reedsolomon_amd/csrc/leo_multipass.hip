@@ -1,0 +1,287 @@
+// leo_multipass.hip — Leopard GF(2^16) reconstruct for transforms taller than
+// one LDS tile (n >= 2048 rows).
+//
+// The n-row transform (leopard.go:421-647: scale by error locators, IFFT,
+// formal derivative, FFT, reveal) runs as three passes over 64-byte column
+// slices, each holding a tile of rows in LDS.  Let P be the largest FFT step
+// distance <= 1024 and C = 1024.
+//
+//   pass 1  tiles of C consecutive rows: load the shards scaled by the error
+//           locators, IFFT layers with distance < C.
+//   pass 2  tiles of the rows lo + j*P (j < n/P) for a run of lo: the IFFT
+//           layers >= C, then the FFT steps with distance >= P, applied to
+//           both in and y = in + H(in), where H is the part of the formal
+//           derivative over row bits >= P.  Writes u = FFT_B(y), v = FFT_B(in).
+//   pass 3  tiles of P consecutive rows: z = u + L(v), where L is the part of
+//           the derivative over row bits < P, then the FFT steps with
+//           distance < P, and the erased rows are scaled back and stored.
+//
+// The derivative out[t] = in[t] ^ XOR over zero bits b of t of in[t | b]
+// splits as D = I + L + H.  FFT_B (distances >= P) mixes rows that differ in
+// bits >= P with skews that depend on those bits only, so it commutes with L:
+// FFT_B(D in) = FFT_B(in + H in) + L(FFT_B in) = u + L(v).
+//
+// Work arrays are slice-major (row r of slice s at (s*n + r)*64) so every
+// tile reads and writes contiguous runs.
+#include <algorithm>
+
+#include "leo_tile.hpp"
+
+namespace rsx {
+
+namespace {
+
+using namespace leo;
+using namespace leo_tile;
+
+constexpr int kMB = 256;     // threads per workgroup
+constexpr int kSlice = 64;   // bytes per row per workgroup: one 64-byte chunk
+
+__device__ __forceinline__ uint8_t *wrow(uint8_t *work, int n, int s, int r) {
+    return work + (static_cast<size_t>(s) * n + r) * kSlice;
+}
+
+// Copies rows [0, nrows) of a tile from global rows rp(l) (64 bytes each),
+// several loads in flight per thread.
+template <class F, class RowPtr>
+__device__ __forceinline__ void load_tile(const Tile &w, int nrows, RowPtr rp) {
+    constexpr int kBatch = 4;
+    const int items = nrows << w.ushift;
+    const int umask = (1 << w.ushift) - 1;
+    for (int it0 = threadIdx.x; it0 < items; it0 += kBatch * kMB) {
+        typename F::E v[kBatch];
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j) {
+            const int it = it0 + j * kMB;
+            if (it < items) v[j] = F::gload(rp(it >> w.ushift), it & umask);
+        }
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j) {
+            const int it = it0 + j * kMB;
+            if (it < items) Lds<F>::store(w, it >> w.ushift, it & umask, v[j]);
+        }
+    }
+}
+
+template <class F, class RowPtr>
+__device__ __forceinline__ void store_tile(const Tile &w, int nrows, RowPtr rp) {
+    for_units(w, nrows, [&](int l, int u) { F::gstore(rp(l), u, Lds<F>::load(w, l, u)); });
+}
+
+// One radix-4 step over tiles whose local row l = j*A + a is global row
+// lo0 + a + j*P: butterflies at global distance `dist` (>= P) connect local
+// rows j, j + dj, j + 2dj, j + 3dj with dj = dist / P.  `w2`, when given, gets
+// the same butterflies (same skews).
+template <class F, bool INV>
+__device__ __forceinline__ void strided_step4(const Tile &w, const Tile *w2, int ashift, int J, int P, int dist,
+                                              int mtrunc, const LeoDev &f) {
+    const int dj = dist / P;
+    const int gcount = min(J / (4 * dj), (mtrunc + 4 * dist - 1) / (4 * dist));
+    if (gcount <= 0) return;
+    const int nq = gcount * dj;  // quads per lo
+    const int qs = log2i(nq);
+    const int A = 1 << ashift;
+    const int items = (A << qs) << w.ushift;
+    for (int it = threadIdx.x; it < items; it += kMB) {
+        const int a = it & (A - 1);
+        const int q = (it >> ashift) & ((1 << qs) - 1);
+        const int u = it >> (ashift + qs);
+        if (q >= nq || u >= w.ucount) continue;
+        const int rj = (q / dj) * 4 * dj;
+        const int j = rj + q % dj;
+        const int iend = rj * P + dist;  // global r + dist
+        const unsigned lm01 = f.skew[iend - 1], lm02 = f.skew[iend + dist - 1], lm23 = f.skew[iend + 2 * dist - 1];
+        const int l0 = (j << ashift) + a, ld = dj << ashift;
+        for (int t = 0; t < (w2 ? 2 : 1); ++t) {
+            const Tile &x = t ? *w2 : w;
+            typename F::E x0 = Lds<F>::load(x, l0, u), x1 = Lds<F>::load(x, l0 + ld, u),
+                          x2 = Lds<F>::load(x, l0 + 2 * ld, u), x3 = Lds<F>::load(x, l0 + 3 * ld, u);
+            if (INV) {
+                ifft2<F>(x0, x1, lm01, f);
+                ifft2<F>(x2, x3, lm23, f);
+                ifft2<F>(x0, x2, lm02, f);
+                ifft2<F>(x1, x3, lm02, f);
+            } else {
+                fft2<F>(x0, x2, lm02, f);
+                fft2<F>(x1, x3, lm02, f);
+                fft2<F>(x0, x1, lm01, f);
+                fft2<F>(x2, x3, lm23, f);
+            }
+            Lds<F>::store(x, l0, u, x0);
+            Lds<F>::store(x, l0 + ld, u, x1);
+            Lds<F>::store(x, l0 + 2 * ld, u, x2);
+            Lds<F>::store(x, l0 + 3 * ld, u, x3);
+        }
+    }
+    __syncthreads();
+}
+
+template <class F>
+__global__ __launch_bounds__(kMB) void k_dec_pass1(const LeoDev f, const LeoMultipassArgs a) {
+    extern __shared__ uint4 lds_raw[];
+    const int s = blockIdx.x;
+    const int base = blockIdx.y * a.C;
+    const Tile w{reinterpret_cast<uint8_t *>(lds_raw), a.C + 1, 1, 2};
+    load_rows<F>(w, a.rows + base, a.errlocs + base, a.C, a.col0 + static_cast<size_t>(s) * kSlice, f);
+    for (int dshift = 0; (4 << dshift) <= a.C; dshift += 2)
+        step4<F, true>(w, a.C, base, dshift, a.ifft_trunc, f.skew, -1, f);
+    store_tile<F>(w, a.C, [&](int l) { return wrow(a.work, a.n, s, base + l); });
+}
+
+template <class F>
+__global__ __launch_bounds__(kMB) void k_dec_pass2(const LeoDev f, const LeoMultipassArgs a) {
+    extern __shared__ uint4 lds_raw[];
+    const int s = blockIdx.x;
+    const int lo0 = blockIdx.y * a.A;
+    const int J = a.n / a.P;
+    const int ashift = log2i(a.A), jshift = log2i(J);
+    const int R = a.A * J;
+    uint8_t *lds = reinterpret_cast<uint8_t *>(lds_raw);
+    const Tile in{lds, R + 1, 1, 2};
+    const Tile y{lds + kSlice * (R + 1), R + 1, 1, 2};
+    auto grow = [&](int l) { return lo0 + (l & (a.A - 1)) + (l >> ashift) * a.P; };
+    load_tile<F>(in, R, [&](int l) { return wrow(a.work, a.n, s, grow(l)); });
+    __syncthreads();
+    // IFFT layers with distance >= C.
+    int dist = a.C;
+    for (; 4 * dist <= a.n; dist *= 4) strided_step4<F, true>(in, nullptr, ashift, J, a.P, dist, a.ifft_trunc, f);
+    if (dist < a.n) {
+        // final radix-2 layer: rows j, j + dj for j < dj, skew[dist - 1]
+        const int dj = dist / a.P;
+        const unsigned lm = f.skew[dist - 1];
+        const int items = (R / 2) << in.ushift;
+        for (int it = threadIdx.x; it < items; it += kMB) {
+            const int u = it & 1, p = it >> 1;  // p < A * dj
+            const int l0 = p, l1 = p + (dj << ashift);
+            typename F::E x0 = Lds<F>::load(in, l0, u), x1 = Lds<F>::load(in, l1, u);
+            ifft2<F>(x0, x1, lm, f);
+            Lds<F>::store(in, l0, u, x0);
+            Lds<F>::store(in, l1, u, x1);
+        }
+        __syncthreads();
+    }
+    // y = in + H(in): H over the row bits >= P, i.e. all bits of j.
+    {
+        const int slots = kSlice / 16;
+        const int items = R * slots;
+        const int rshift = ashift + jshift;
+        for (int it = threadIdx.x; it < items; it += kMB) {
+            const int l = it & (R - 1), sl = it >> rshift;
+            const int j = l >> ashift, lo = l & (a.A - 1);
+            uint4 acc = *slot(in, l, sl);
+            for (int b = 1; b < J; b <<= 1)
+                if (!(j & b)) acc = xor4(acc, *slot(in, ((j | b) << ashift) + lo, sl));
+            *slot(y, l, sl) = acc;
+        }
+        __syncthreads();
+    }
+    // FFT steps with distance >= P on both tiles.
+    for (int d = a.n >> 2; d >= a.P; d >>= 2) strided_step4<F, false>(in, &y, ashift, J, a.P, d, a.fft_trunc, f);
+    store_tile<F>(in, R, [&](int l) { return wrow(a.work2, a.n, s, grow(l)); });  // v
+    store_tile<F>(y, R, [&](int l) { return wrow(a.work, a.n, s, grow(l)); });    // u
+}
+
+template <class F>
+__global__ __launch_bounds__(kMB) void k_dec_pass3(const LeoDev f, const LeoMultipassArgs a) {
+    extern __shared__ uint4 lds_raw[];
+    const int s = blockIdx.x;
+    const int base = blockIdx.y * a.P;
+    const Tile w{reinterpret_cast<uint8_t *>(lds_raw), a.P + 1, 1, 2};
+    load_tile<F>(w, a.P, [&](int l) { return wrow(a.work2, a.n, s, base + l); });  // v
+    __syncthreads();
+    // z = u + L(v): L over the row bits < P.  Registers first, then LDS.
+    {
+        const int slots = kSlice / 16;
+        const int ps = log2i(a.P);
+        const int items = a.P * slots;
+        uint4 res[kDerivMax];
+#pragma unroll
+        for (int jj = 0; jj < kDerivMax; ++jj) {
+            const int it = threadIdx.x + jj * kMB;
+            if (it < items) {
+                const int t = it & (a.P - 1), sl = it >> ps;
+                uint4 acc = gload4(wrow(a.work, a.n, s, base + t) + sl * 16);
+                for (int b = 1; b < a.P; b <<= 1)
+                    if (!(t & b)) acc = xor4(acc, *slot(w, t | b, sl));
+                res[jj] = acc;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int jj = 0; jj < kDerivMax; ++jj) {
+            const int it = threadIdx.x + jj * kMB;
+            if (it < items) *slot(w, it & (a.P - 1), it >> ps) = res[jj];
+        }
+        __syncthreads();
+    }
+    // FFT steps with distance < P, then the final radix-2 layer.
+    int dist4 = a.n, dist = a.n >> 2;
+    for (; dist != 0; dist4 = dist, dist >>= 2)
+        if (dist < a.P) step4<F, false>(w, a.P, base, log2i(dist), a.fft_trunc, f.skew, -1, f);
+    if (dist4 == 2 && a.fft_trunc > base) {
+        const int pairs = min(a.P / 2, (a.fft_trunc - base + 1) / 2);
+        for_rows(w, pairs, [&](int q, int u) {
+            const int r = 2 * q;
+            typename F::E x = Lds<F>::load(w, r, u), yv = Lds<F>::load(w, r + 1, u);
+            fft2<F>(x, yv, f.skew[base + r], f);
+            Lds<F>::store(w, r, u, x);
+            Lds<F>::store(w, r + 1, u, yv);
+        });
+        __syncthreads();
+    }
+    // Reveal the erased rows of this tile.
+    const size_t col = a.col0 + static_cast<size_t>(s) * kSlice;
+    for_units(w, a.P, [&](int t, int u) {
+        const int q = a.out_index[base + t];
+        if (q < 0) return;
+        typename F::E v = F::zero();
+        F::mul_add(v, Lds<F>::load(w, t, u), table<F>(f, a.out_logs[q]));
+        F::gstore(a.out[q] + col, u, v);
+    });
+}
+
+}  // namespace
+
+bool leo_multipass_shape(int n, int *C, int *P, int *A) {
+    if (n < 2048) return false;
+    int p = n >> 2;
+    while (p > 1024) p >>= 2;
+    const int J = n / p;
+    if (J > 512) return false;
+    *C = 1024;
+    *P = p;
+    *A = 512 / J;
+    return true;
+}
+
+size_t leo_multipass_chunk(int n, size_t size) {
+    size_t cw = (size_t{64} << 20) / static_cast<size_t>(n);  // two work arrays within 128 MiB
+    cw -= cw % kSlice;
+    cw = std::max<size_t>(cw, kSlice);
+    return std::min(cw, size);
+}
+
+hipError_t leo_decode_multipass(const LeoDev &f, const LeoMultipassArgs &a, hipStream_t s) {
+    if (f.bits != 16 || a.slices <= 0) return hipErrorInvalidValue;
+    const size_t lds1 = static_cast<size_t>(kSlice) * (a.C + 1);
+    const size_t lds2 = 2 * static_cast<size_t>(kSlice) * (a.A * (a.n / a.P) + 1);
+    const size_t lds3 = static_cast<size_t>(kSlice) * (a.P + 1);
+    const unsigned slices = static_cast<unsigned>(a.slices);
+    hipError_t e;
+    auto allow = [](const void *k, size_t lds) {
+        return lds <= (64u << 10) ? hipSuccess
+                                   : hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                         static_cast<int>(lds));
+    };
+    if ((e = allow(reinterpret_cast<const void *>(k_dec_pass1<F16>), lds1)) != hipSuccess) return e;
+    if ((e = allow(reinterpret_cast<const void *>(k_dec_pass2<F16>), lds2)) != hipSuccess) return e;
+    if ((e = allow(reinterpret_cast<const void *>(k_dec_pass3<F16>), lds3)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_dec_pass1<F16>, dim3(slices, a.n / a.C), dim3(kMB), lds1, s, f, a);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_dec_pass2<F16>, dim3(slices, a.P / a.A), dim3(kMB), lds2, s, f, a);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_dec_pass3<F16>, dim3(slices, a.n / a.P), dim3(kMB), lds3, s, f, a);
+    return hipGetLastError();
+}
+
+}  // namespace rsx

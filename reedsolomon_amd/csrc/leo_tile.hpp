@@ -1,0 +1,259 @@
+// leo_tile.hpp — device building blocks of the LDS-resident Leopard
+// kernels: the LDS tile layout, lane-to-work mappings, radix-4 / radix-2
+// butterfly steps over a tile, row loads and stores, and the formal
+// derivative.  The butterfly schedule, skews and truncations are those of the
+// layer-per-launch kernels (leopard.go:696-897 fftDIT / ifftDIT,
+// leopard8.go likewise); only where the work rows live differs.
+#pragma once
+
+#include "leo_device.hpp"
+#include "leo_kernels.hpp"
+
+namespace rsx {
+namespace leo_tile {
+
+using namespace leo;
+
+constexpr int kFB = 256;        // threads per workgroup (512 where a step has that many items)
+constexpr int kFBMax = 512;
+constexpr int kDerivMax = kLeoFusedDecodeLds / 16 / kFB;  // uint4 per thread in the derivative
+
+// Work rows of one slice in LDS, slot-major: 16-byte slot s of row r sits at
+// base + (s * r1 + pos(r)) * 16 with r1 = rows + 1.  A GF(2^8) column unit
+// is one slot; a GF(2^16) unit is the lo slot and the hi slot two further
+// (the 64-byte chunk layout of the shard, slot by slot).
+//
+// Butterfly lanes take consecutive rows of one slot; pos() permutes the low
+// four row bits by bits 4 and 5 so that the 16 rows a quarter-wave touches in
+// any radix-4 or radix-2 step (consecutive rows, rows 4q+j, rows
+// 16a+b+4j, rows 2q) fall in 16 distinct bank groups.  Copy lanes take
+// consecutive slots of one row; the +1 row of padding keeps those distinct.
+struct Tile {
+    uint8_t *base;
+    int r1;      // rows + 1
+    int ushift;  // log2(column units in a full slice)
+    int ucount;  // column units of this slice (the last slice may be narrower)
+};
+
+__device__ __forceinline__ int log2i(int x) {
+    int s = 0;
+    while ((1 << s) < x) ++s;
+    return s;
+}
+
+__device__ __forceinline__ int pos(int r) { return r ^ (((r >> 4) & 1) * 5) ^ (((r >> 5) & 1) * 10); }
+
+__device__ __forceinline__ uint4 *slot(const Tile &t, int r, int s) {
+    return reinterpret_cast<uint4 *>(t.base + ((s * t.r1 + pos(r)) << 4));
+}
+
+template <class F>
+struct Lds;
+template <>
+struct Lds<F8> {
+    __device__ static uint4 load(const Tile &t, int r, int u) { return *slot(t, r, u); }
+    __device__ static void store(const Tile &t, int r, int u, const uint4 &e) { *slot(t, r, u) = e; }
+};
+template <>
+struct Lds<F16> {
+    __device__ static int lo(int u) { return ((u >> 1) << 2) | (u & 1); }
+    __device__ static F16::E load(const Tile &t, int r, int u) { return {*slot(t, r, lo(u)), *slot(t, r, lo(u) + 2)}; }
+    __device__ static void store(const Tile &t, int r, int u, const F16::E &e) {
+        *slot(t, r, lo(u)) = e.lo;
+        *slot(t, r, lo(u) + 2) = e.hi;
+    }
+};
+
+// fn(q, u) for q < nq, u < ucount; q varies fastest across lanes.
+template <class Fn>
+__device__ __forceinline__ void for_rows(const Tile &t, int nq, Fn fn) {
+    const int qs = log2i(nq);
+    const int items = (1 << qs) << t.ushift;
+    const int qmask = (1 << qs) - 1;
+    for (int it = threadIdx.x; it < items; it += blockDim.x) {
+        const int q = it & qmask, u = it >> qs;
+        if (q < nq && u < t.ucount) fn(q, u);
+    }
+}
+
+// fn(r, u) for r < nrows, u < ucount; u varies fastest (coalesced HBM rows).
+template <class Fn>
+__device__ __forceinline__ void for_units(const Tile &t, int nrows, Fn fn) {
+    const int items = nrows << t.ushift;
+    const int umask = (1 << t.ushift) - 1;
+    for (int it = threadIdx.x; it < items; it += blockDim.x) {
+        const int u = it & umask, r = it >> t.ushift;
+        if (u < t.ucount) fn(r, u);
+    }
+}
+
+// One radix-4 step (two layers) at distance 2^dshift over the butterfly
+// groups r < mtrunc of a tile of `rows` consecutive rows whose local row 0 is
+// row `base` of the transform (skews and truncation use the global row), as
+// in k_step4.
+template <class F, bool INV>
+__device__ __forceinline__ void step4(const Tile &w, int rows, int base, int dshift, int mtrunc,
+                                      const uint16_t *skew, int bias, const LeoDev &f) {
+    const int dist = 1 << dshift;
+    const int left = mtrunc - base;
+    if (left <= 0) return;  // uniform across the block: no barrier is skipped by a subset
+    const int groups = min(rows >> (dshift + 2), (left + 4 * dist - 1) >> (dshift + 2));
+    for_rows(w, groups << dshift, [&](int q, int u) {
+        const int r = (q >> dshift) << (dshift + 2);
+        const int i = r + (q & (dist - 1));
+        const int iend = base + r + dist;
+        const unsigned lm01 = skew[iend + bias], lm02 = skew[iend + dist + bias],
+                       lm23 = skew[iend + 2 * dist + bias];
+        typename F::E x0 = Lds<F>::load(w, i, u), x1 = Lds<F>::load(w, i + dist, u),
+                      x2 = Lds<F>::load(w, i + 2 * dist, u), x3 = Lds<F>::load(w, i + 3 * dist, u);
+        if (INV) {
+            ifft2<F>(x0, x1, lm01, f);
+            ifft2<F>(x2, x3, lm23, f);
+            ifft2<F>(x0, x2, lm02, f);
+            ifft2<F>(x1, x3, lm02, f);
+        } else {
+            fft2<F>(x0, x2, lm02, f);
+            fft2<F>(x1, x3, lm02, f);
+            fft2<F>(x0, x1, lm01, f);
+            fft2<F>(x2, x3, lm23, f);
+        }
+        Lds<F>::store(w, i, u, x0);
+        Lds<F>::store(w, i + dist, u, x1);
+        Lds<F>::store(w, i + 2 * dist, u, x2);
+        Lds<F>::store(w, i + 3 * dist, u, x3);
+    });
+    __syncthreads();
+}
+
+// IFFT over `n` rows truncated to `mtrunc`; `skew` + `bias` as in leo_step4,
+// the final radix-2 layer (odd log2 n) uses skew[dist + bias].
+template <class F>
+__device__ __forceinline__ void ifft(const Tile &w, int n, int mtrunc, const uint16_t *skew, int bias,
+                                     const LeoDev &f) {
+    int dshift = 0;
+    while ((4 << dshift) <= n) {
+        step4<F, true>(w, n, 0, dshift, mtrunc, skew, bias, f);
+        dshift += 2;
+    }
+    const int dist = 1 << dshift;
+    if (dist < n) {
+        const unsigned lm = skew[dist + bias];
+        for_rows(w, dist, [&](int i, int u) {
+            typename F::E x = Lds<F>::load(w, i, u), y = Lds<F>::load(w, i + dist, u);
+            ifft2<F>(x, y, lm, f);
+            Lds<F>::store(w, i, u, x);
+            Lds<F>::store(w, i + dist, u, y);
+        });
+        __syncthreads();
+    }
+}
+
+// FFT over `n` rows truncated to `mtrunc` (skew = fftSkew, bias -1).
+template <class F>
+__device__ __forceinline__ void fft(const Tile &w, int n, int mtrunc, const LeoDev &f) {
+    int dist4 = n, dist = n >> 2;
+    while (dist != 0) {
+        step4<F, false>(w, n, 0, log2i(dist), mtrunc, f.skew, -1, f);
+        dist4 = dist;
+        dist >>= 2;
+    }
+    if (dist4 == 2) {
+        for_rows(w, (mtrunc + 1) / 2, [&](int q, int u) {
+            const int r = 2 * q;
+            typename F::E x = Lds<F>::load(w, r, u), y = Lds<F>::load(w, r + 1, u);
+            fft2<F>(x, y, f.skew[r], f);
+            Lds<F>::store(w, r, u, x);
+            Lds<F>::store(w, r + 1, u, y);
+        });
+        __syncthreads();
+    }
+}
+
+// rows [0, nrows) of w <- src[t] (+col) times exp(logs[t]) when logs, zero
+// where src[t] is null.  Each thread issues kBatch row loads before it
+// stores any, so HBM latency overlaps across them.
+template <class F>
+__device__ __forceinline__ void load_rows(const Tile &w, const uint8_t *const *src, const uint16_t *logs,
+                                          int nrows, size_t col, const LeoDev &f) {
+    constexpr int kBatch = 32 / F::kUnitBytes * 4;  // 8 GF(2^8) / 4 GF(2^16) units in flight
+    const int items = nrows << w.ushift;
+    const int umask = (1 << w.ushift) - 1;
+    for (int it0 = threadIdx.x; it0 < items; it0 += kBatch * blockDim.x) {
+        typename F::E v[kBatch];
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j) {
+            const int it = it0 + j * blockDim.x;
+            v[j] = F::zero();
+            if (it < items && (it & umask) < w.ucount) {
+                const uint8_t *s = src[it >> w.ushift];
+                if (s) v[j] = F::gload(s + col, it & umask);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j) {
+            const int it = it0 + j * blockDim.x;
+            if (it < items && (it & umask) < w.ucount) {
+                const int t = it >> w.ushift;
+                if (logs) {
+                    typename F::E y = F::zero();
+                    F::mul_add(y, v[j], table<F>(f, logs[t]));
+                    v[j] = y;
+                }
+                Lds<F>::store(w, t, it & umask, v[j]);
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// dst[q] (+col) <- row rows[q] (or row q) times exp(logs[q]) when logs.
+template <class F>
+__device__ __forceinline__ void store_rows(const Tile &w, uint8_t *const *dst, const int *rows,
+                                           const uint16_t *logs, int count, size_t col, const LeoDev &f) {
+    for_units(w, count, [&](int q, int u) {
+        const typename F::E x = Lds<F>::load(w, rows ? rows[q] : q, u);
+        typename F::E v = F::zero();
+        if (logs) F::mul_add(v, x, table<F>(f, logs[q]));
+        else v = x;
+        F::gstore(dst[q] + col, u, v);
+    });
+}
+
+// acc ^= tmp, every 16-byte position of two equally shaped tiles.
+__device__ __forceinline__ void xor_tiles(const Tile &acc, const Tile &tmp, int positions) {
+    for (int it = threadIdx.x; it < positions; it += blockDim.x) {
+        uint4 *a = reinterpret_cast<uint4 *>(acc.base) + it;
+        *a = xor4(*a, reinterpret_cast<const uint4 *>(tmp.base)[it]);
+    }
+    __syncthreads();
+}
+
+// Formal derivative in place (leopard.go:619-623): out[t] = in[t] ^ XOR over
+// zero bits b of t (t | 2^b < n) of in[t | 2^b].  Every thread computes its
+// slots into registers, then all store after a barrier.
+__device__ __forceinline__ void derivative(const Tile &w, int n, int slots) {
+    const int ns = log2i(n);
+    const int items = n * slots;
+    uint4 res[kDerivMax];
+#pragma unroll
+    for (int j = 0; j < kDerivMax; ++j) {
+        const int it = threadIdx.x + j * blockDim.x;
+        if (it < items) {
+            const int t = it & (n - 1), s = it >> ns;
+            uint4 acc = *slot(w, t, s);
+            for (int b = 1; b < n; b <<= 1)
+                if (!(t & b)) acc = xor4(acc, *slot(w, t | b, s));
+            res[j] = acc;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kDerivMax; ++j) {
+        const int it = threadIdx.x + j * blockDim.x;
+        if (it < items) *slot(w, it & (n - 1), it >> ns) = res[j];
+    }
+    __syncthreads();
+}
+
+}  // namespace leo_tile
+}  // namespace rsx

@@ -71,7 +71,7 @@ struct LeopardState {
     bool fused = true;  // LDS-resident kernels where the rows fit
     const LeoField *field = nullptr;
     const LeoDev *dev = nullptr;
-    DeviceBuffer work, work2, ptrs, rows, logs, stage, scratch, flag, cmp;
+    DeviceBuffer work, work2, ptrs, rows, logs, stage, scratch, flag, cmp, outidx;
     std::vector<uint8_t *> h_ptrs;
     std::unordered_map<std::string, std::unique_ptr<LeoPattern>> patterns;
 };
@@ -217,7 +217,7 @@ int encode_device(rsx_codec *c, const std::vector<uint8_t *> &data, const std::v
 
     const uint8_t *const *dptr = reinterpret_cast<const uint8_t *const *>(s.ptrs.ptr);
     uint8_t *const *pptr = reinterpret_cast<uint8_t *const *>(s.ptrs.as<uint8_t *>() + static_cast<size_t>(groups) * m);
-    const int slice = s.fused ? leo_fused_slice(s.bits, groups > 1 ? 2 * m : m, m, kLeoFusedEncodeLds) : 0;
+    const int slice = s.fused ? leo_fused_slice(s.bits, m, groups > 1 ? 2 : 1, m, size, kLeoFusedEncodeLds) : 0;
     if (slice) {
         const LeoEncodeArgs a{dptr, pptr, size, k, p, m, groups, slice};
         RSX_HIP(leo_encode_fused(*s.dev, a, c->stream));
@@ -315,11 +315,34 @@ int reconstruct_device(rsx_codec *c, const std::vector<uint8_t *> &shards, const
     const int ifft_trunc = s.bits == 16 ? pat->input_count : m + k;
     const uint8_t *const *lptr = reinterpret_cast<const uint8_t *const *>(s.ptrs.ptr);
     uint8_t *const *optr = reinterpret_cast<uint8_t *const *>(s.ptrs.as<uint8_t *>() + n);
-    const int slice = s.fused ? leo_fused_slice(s.bits, n, n, kLeoFusedDecodeLds) : 0;
+    // n rows in LDS; the formal derivative also holds n * slice bytes in registers.
+    const int slice = s.fused ? leo_fused_slice(s.bits, n, 1, n, size, std::min(kLeoFusedEncodeLds,
+                                                                            kLeoFusedDecodeLds / n * (n + 1)))
+                              : 0;
     if (slice) {
         const LeoDecodeArgs a{lptr, pat->d_errlocs.as<uint16_t>(), s.rows.as<int>(), s.logs.as<uint16_t>(), optr,
                               size, n, nout, ifft_trunc, m + k, slice};
         if (nout) RSX_HIP(leo_decode_fused(*s.dev, a, c->stream));
+        return RSX_OK;
+    }
+    int mC = 0, mP = 0, mA = 0;
+    if (s.fused && s.bits == 16 && leo_multipass_shape(n, &mC, &mP, &mA)) {
+        if (!nout) return RSX_OK;
+        std::vector<int> out_index(static_cast<size_t>(n), -1);
+        for (int q = 0; q < nout; ++q) out_index[rows[q]] = q;
+        if ((err = s.outidx.reserve(out_index.size() * sizeof(int)))) return err;
+        RSX_HIP(hipMemcpyAsync(s.outidx.ptr, out_index.data(), out_index.size() * sizeof(int),
+                               hipMemcpyHostToDevice, c->stream));
+        const size_t cw = leo_multipass_chunk(n, size);
+        if ((err = s.work.reserve(cw * n))) return err;
+        if ((err = s.work2.reserve(cw * n))) return err;
+        for (size_t off = 0; off < size; off += cw) {
+            const size_t cols = std::min(cw, size - off);
+            const LeoMultipassArgs a{lptr, pat->d_errlocs.as<uint16_t>(), s.outidx.as<int>(), s.logs.as<uint16_t>(),
+                                     optr, s.work.as<uint8_t>(), s.work2.as<uint8_t>(), off,
+                                     static_cast<int>(cols / 64), n, ifft_trunc, m + k, mC, mP, mA};
+            RSX_HIP(leo_decode_multipass(*s.dev, a, c->stream));
+        }
         return RSX_OK;
     }
     const size_t chunk = chunk_for(size, 2 * n, kDecodeBudget);
@@ -356,25 +379,14 @@ struct Resolved {
     MemKind kind = MemKind::Device;
 };
 
-MemKind classify_ptr(const void *p) {
-    hipPointerAttribute_t attr;
-    if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
-        (void)hipGetLastError();
-        return MemKind::Host;
-    }
-    return (attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged ||
-            attr.type == hipMemoryTypeUnified)
-               ? MemKind::Device
-               : MemKind::Host;
-}
-
 int resolve(rsx_codec *c, uint8_t *const *shards, int total, size_t size, const std::vector<bool> &use,
             const std::vector<bool> &load, Resolved &r) {
     LeopardState &s = *c->leo;
     bool have = false;
+    DeviceRanges ranges;  // one runtime query per allocation, not per shard
     for (int i = 0; i < total; ++i) {
         if (!use[i] || !shards[i]) continue;
-        MemKind kk = classify_ptr(shards[i]);
+        const MemKind kk = ranges.is_device(shards[i]) ? MemKind::Device : MemKind::Host;
         if (!have) {
             r.kind = kk;
             have = true;
