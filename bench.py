@@ -66,6 +66,8 @@ def _parse():
                     help="served stream: Poisson rate as a fraction of the measured saturation rate")
     ap.add_argument("--hbm-budget-gib", type=float, default=16.0,
                     help="served stream: shard bytes admitted in flight")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="with --encode-only: also copy 1 GiB HBM->HBM 5 times (counter calibration)")
     ap.add_argument("--encode-only", action="store_true",
                     help="time only the batched RS(16,4) encode launches (for kernel profiling)")
     return ap.parse_args()
@@ -138,6 +140,26 @@ def event_time(torch, stream_ptr, fn, reps):
     return float(np.mean([a.elapsed_time(b) * 1e-3 for a, b in evs]))
 
 
+def measured_traffic(launch_bytes):
+    """HBM bytes per batch_pass<4> launch from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (copy-calibrated), when it was measured
+    on this same launch size; None otherwise."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "batch_pass4_rs16_4_256x4MiB_traffic.md")
+    vals = {}
+    try:
+        with open(path) as f:
+            for line in f:
+                key, _, val = line.partition(":")
+                if key.strip().endswith("_bytes_per_launch") and val.strip().isdigit():
+                    vals[key.strip()] = int(val.strip())
+    except OSError:
+        return None
+    if vals.get("algorithmic_bytes_per_launch") != launch_bytes:
+        return None
+    return vals.get("traffic_bytes_per_launch")
+
+
 def headline(args, torch, rs):
     """The RS(16,4) 256 x 16 x 4 MiB workload, plain and batched."""
     lib = rs.lib
@@ -187,6 +209,15 @@ def headline(args, torch, rs):
         benc.synchronize()
         avg = event_time(torch, sbat, benc.launch, args.steps)
         launch_bytes = nst * n * S
+        if args.copy_calib:
+            # A plain copy of known size: its FETCH_SIZE / WRITE_SIZE per
+            # dispatch calibrate the same counters read for batch_pass.
+            src = torch.empty(1 << 30, dtype=torch.uint8, device="cuda")
+            dst = torch.empty_like(src)
+            for _ in range(5):
+                dst.copy_(src)
+            torch.cuda.synchronize()
+            del src, dst
         return dict(encode_only=True, avg_launch_us=round(avg * 1e6, 2),
                     achieved_gbps=round(launch_bytes / avg / 1e9, 1), launch_bytes=launch_bytes,
                     tuning=dict(zip(("tile_bytes", "blocks_per_cu"), benc.tuning())))
@@ -242,7 +273,7 @@ def headline(args, torch, rs):
         ms_plain=round(t_plain / args.steps * 1e3, 3), ms_batched=round(t_batch / args.steps * 1e3, 3),
         parity_ok=parity_ok,
         roofline={"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                  "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                  "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": measured_traffic(launch_bytes),
                   "kernel": f"batch_pass<4> (RS(16,4) encode of {nst} stripes, {launch_bytes} B per launch)",
                   "avg_launch_us": round(avg * 1e6, 2)},
         plain_encode_launch={"kernel": "gf8_pass_vec16<4> (one stripe)", "avg_us": round(avg_plain * 1e6, 2),

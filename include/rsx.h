@@ -128,22 +128,27 @@ int rsx_synchronize(rsx_codec *c);
 int rsx_decode_cache_size(const rsx_codec *c);
 
 /* ---- Batched path -------------------------------------------------------
- * Many Encode / Reconstruct / ReconstructData calls of classic codecs (any
- * (k, m), shard size and erasure pattern) planned once into a device work
- * list and run as one persistent launch per output-count class, captured in
- * a HIP graph that rsx_batch_launch replays.  There is no single reference
- * counterpart: each item stands for one call of reedsolomon.go:686 / 1446 /
- * 1634 and is validated as that call is; status[i] receives its result.
- * Items of one plan must be independent (no item may read a shard another
- * item writes), shards must be device memory, and the codecs must outlive
- * the plan.  Outputs are overwritten, so a plan may be launched repeatedly. */
+ * Many Encode / Verify / Reconstruct / ReconstructData calls planned once
+ * into a device work list and run as persistent launches captured in a HIP
+ * graph that rsx_batch_launch replays.  Classic codecs (any (k, m), shard
+ * size and erasure pattern): one launch per output-count class.  Leopard
+ * codecs (leopard.go:139 / 393 / 421, leopard8.go likewise): one LDS-resident
+ * launch per field, for 16-byte aligned shards whose rows fit one LDS tile
+ * (every GF(2^8) shape; GF(2^16) up to m = 1024 for encode and n = 1024 work
+ * rows for reconstruct); other Leopard items get RSX_ERR_NOT_SUPPORTED and
+ * take the per-stripe call.  There is no single reference counterpart: each
+ * item stands for one call of reedsolomon.go:686 / 857 / 1446 / 1634 and is
+ * validated as that call is; status[i] receives its result.  Items of one
+ * plan must be independent (no item may read a shard another item writes),
+ * shards must be device memory, and the codecs must outlive the plan.
+ * Outputs are overwritten, so a plan may be launched repeatedly. */
 #define RSX_OP_ENCODE 1
 #define RSX_OP_RECONSTRUCT 2
 #define RSX_OP_RECONSTRUCT_DATA 3
 #define RSX_OP_VERIFY 4           /* parity recomputed into plan scratch and compared */
 
 typedef struct rsx_batch_item {
-    rsx_codec *codec;       /* classic codec from rsx_new */
+    rsx_codec *codec;       /* codec from rsx_new */
     int op;                 /* RSX_OP_* */
     int nshards;            /* entries of shards / lens (the codec's TotalShards) */
     uint8_t *const *shards; /* device memory; absent shards carry an output buffer */
@@ -182,8 +187,8 @@ int rsx_batch_get_tuning(const rsx_batch *b, int *tile_bytes, int *blocks_per_cu
  * admitted in arrival order while the bytes in flight stay within an HBM
  * budget; a scheduler thread plans whatever has been admitted into a new
  * batch whenever one of its launch slots (each with its own stream) is free,
- * so batches overlap on the device, and Leopard codecs run on a worker
- * thread.  Latency is measured from arrival to completion.  Requests in
+ * so batches overlap on the device; Leopard requests the batch kernel does
+ * not take run through the per-stripe call on a worker thread.  Latency is measured from arrival to completion.  Requests in
  * flight at the same time must not share shards.  No reference counterpart:
  * each request is the call of reedsolomon.go it names, with that call's
  * validation and result. */
@@ -197,8 +202,9 @@ typedef struct rsx_request {
 
 typedef struct rsx_server rsx_server;
 
-/* hbm_budget: bytes of shards in flight (a larger request is admitted alone);
- * max_batch: requests per launch (0 = 4096). */
+/* hbm_budget: bytes of shards in flight (a larger request is admitted alone;
+ * one batch takes at most budget / 8); max_batch: requests per launch
+ * (0 = 4096). */
 int rsx_server_create(int device, size_t hbm_budget, int max_batch, rsx_server **out);
 void rsx_server_free(rsx_server *s); /* drains first */
 int rsx_server_submit(rsx_server *s, const rsx_request *req, uint64_t *ticket);

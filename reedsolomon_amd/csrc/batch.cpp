@@ -14,6 +14,7 @@
 #include "batch.hpp"
 #include "codec.hpp"
 #include "kernels.hpp"
+#include "leopard.hpp"
 
 namespace rsx {
 int check_shards(const size_t *lens, int n, bool nilok);
@@ -27,6 +28,10 @@ using namespace rsx;
 struct rsx_batch {
     int device = 0;
     hipStream_t stream = nullptr;
+    // Leopard launches run on `side`, forked from and joined back to the
+    // launch stream, so they overlap the classic passes (also inside graphs).
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     int cus = 0;
     // tuning: 0 = autotune on first launch
     uint32_t tile = 0;
@@ -59,6 +64,14 @@ struct rsx_batch {
     size_t h_flags_cap = 0;
     int nitems = 0;
     std::vector<uint8_t> is_verify;
+
+    // Leopard items, per field (0: GF(2^8), 1: GF(2^16)), run by the
+    // LDS-resident batch kernel after the classic passes.
+    std::vector<rsx::LeoItemPlan> leo[2];
+    std::vector<rsx::LeoBatchTask> leo_tasks[2];
+    DeviceBuffer d_leo_tasks[2], d_leo_aux;
+    size_t leo_lds[2] = {0, 0};
+    uint32_t leo_slices[2] = {0, 0};
 };
 
 namespace {
@@ -124,10 +137,28 @@ int enqueue(rsx_batch *b, hipStream_t s) {
     const bool verify = !b->cmp.empty();
     if (verify)
         RSX_HIP(launch_fill_u32(b->d_flags.as<uint32_t>(), static_cast<size_t>(b->nitems), 0u, s));
+    const bool leo = !b->leo_tasks[0].empty() || !b->leo_tasks[1].empty();
+    const bool fork = leo && !b->launches.empty();
+    hipStream_t ls = s;
+    if (fork) {
+        RSX_HIP(hipEventRecord(b->fork, s));
+        RSX_HIP(hipStreamWaitEvent(b->side, b->fork, 0));
+        ls = b->side;
+    }
+    for (int fi = 0; fi < 2; ++fi) {
+        if (b->leo_tasks[fi].empty()) continue;
+        const unsigned g = std::min<unsigned>(static_cast<unsigned>(std::max(1, b->cus * 8)), b->leo_slices[fi]);
+        RSX_HIP(leo_batch_launch(*b->leo[fi][0].dev, b->d_leo_tasks[fi].as<LeoBatchTask>(),
+                                 static_cast<int>(b->leo_tasks[fi].size()), b->leo_slices[fi], b->leo_lds[fi], g,
+                                 ls));
+    }
+    if (fork) RSX_HIP(hipEventRecord(b->join, b->side));
     for (size_t i = 0; i < b->launches.size(); ++i) {
         const unsigned g = std::min<unsigned>(grid, std::max<uint32_t>(b->launches[i].ntiles, 1));
         RSX_HIP(launch_batch_pass(b->launches[i], b->classes[i], g, s));
     }
+    if (fork) RSX_HIP(hipStreamWaitEvent(s, b->join, 0));
+
     if (verify) {
         RSX_HIP(launch_batch_compare(b->d_cmp.as<CompareEntry>(), static_cast<int>(b->cmp.size()),
                                      b->d_flags.as<uint32_t>(), s));
@@ -210,8 +241,11 @@ int rsx_batch_create(int device, rsx_batch **out) {
     auto *b = new rsx_batch;
     b->device = device;
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->join, hipEventDisableTiming) != hipSuccess ||
         hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) {
-        delete b;
+        rsx_batch_free(b);
         return fail(RSX_ERR_DEVICE, "stream creation failed");
     }
     *out = b;
@@ -221,10 +255,14 @@ int rsx_batch_create(int device, rsx_batch **out) {
 void rsx_batch_free(rsx_batch *b) {
     if (!b) return;
     (void)hipSetDevice(b->device);
-    (void)hipStreamSynchronize(b->stream);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->side) (void)hipStreamSynchronize(b->side);
     destroy_graph(b);
     if (b->h_flags) (void)hipHostFree(b->h_flags);
-    (void)hipStreamDestroy(b->stream);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->join) (void)hipEventDestroy(b->join);
+    if (b->side) (void)hipStreamDestroy(b->side);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
 
@@ -255,6 +293,13 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
     b->cmp.clear();
     b->nitems = n;
     b->is_verify.assign(static_cast<size_t>(std::max(n, 0)), 0);
+    for (int fi = 0; fi < 2; ++fi) {
+        b->leo[fi].clear();
+        b->leo_tasks[fi].clear();
+        b->leo_lds[fi] = 0;
+        b->leo_slices[fi] = 0;
+    }
+    std::vector<uint8_t> laux;  // Leopard error locators / output rows / logs
     struct VerifyOut {
         size_t item;
         std::vector<size_t> slots;  // scratch offsets of the recomputed parity rows
@@ -285,11 +330,42 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
         status[it] = RSX_OK;
         uint8_t *fl = filled ? filled + fill_off : nullptr;
         if (c) fill_off += c->total;
-        if (!c || c->engine) {
-            status[it] = fail(RSX_ERR_NOT_SUPPORTED, "batched path covers classic codecs");
+        if (!c) {
+            status[it] = fail(RSX_ERR_INVALID_INPUT, "batch item without a codec");
             continue;
         }
         if (fl) std::memset(fl, 0, c->total);
+        if (c->engine) {
+            // Leopard: one task of the LDS-resident batch kernel; Verify is
+            // an encode into plan scratch plus a compare.
+            if (item.op != RSX_OP_ENCODE && item.op != RSX_OP_VERIFY && item.op != RSX_OP_RECONSTRUCT &&
+                item.op != RSX_OP_RECONSTRUCT_DATA) {
+                status[it] = fail(RSX_ERR_INVALID_INPUT, "unknown batch op");
+                continue;
+            }
+            LeoItemPlan lp;
+            const int e = leopard_plan_item(c, item.op, item.shards, item.lens, item.nshards, b->pool, laux, lp, fl);
+            if (e) {
+                if (fl) std::memset(fl, 0, c->total);
+                status[it] = e;
+                continue;
+            }
+            if (item.op == RSX_OP_VERIFY) {
+                const size_t len = lp.task.enc.size;
+                const size_t pitch = (len + 255) / 256 * 256;
+                VerifyOut vo{static_cast<size_t>(it), {}, {}, len};
+                for (int i = 0; i < c->m; ++i) {
+                    vo.slots.push_back(scratch_bytes);
+                    vo.parity.push_back(item.shards[c->k + i]);
+                    scratch_patch.emplace_back(lp.parity_pool + i, scratch_bytes);
+                    scratch_bytes += pitch;
+                }
+                vouts.push_back(std::move(vo));
+                b->is_verify[it] = 1;
+            }
+            if (!lp.empty) b->leo[lp.bits == 16 ? 1 : 0].push_back(lp);
+            continue;
+        }
         const int k = c->k, total = c->total;
         const size_t *lens = item.lens;
         std::vector<const uint8_t *> in;
@@ -469,6 +545,40 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
     if ((err = b->d_ptrs.reserve(std::max<size_t>(b->pool.size(), 1) * sizeof(void *)))) return err;
     RSX_HIP(hipMemcpyAsync(b->d_ptrs.ptr, b->pool.data(), b->pool.size() * sizeof(void *),
                            hipMemcpyHostToDevice, b->stream));
+    // Leopard tasks: resolve pool / aux offsets to device addresses.
+    if (!laux.empty()) {
+        if ((err = b->d_leo_aux.reserve(laux.size()))) return err;
+        RSX_HIP(hipMemcpyAsync(b->d_leo_aux.ptr, laux.data(), laux.size(), hipMemcpyHostToDevice, b->stream));
+    }
+    for (int fi = 0; fi < 2; ++fi) {
+        uint32_t slice0 = 0;
+        for (const LeoItemPlan &lp : b->leo[fi]) {
+            LeoBatchTask t = lp.task;
+            t.slice0 = slice0;
+            uint8_t *const *pp = b->d_ptrs.as<uint8_t *>() + lp.ptr_off;
+            if (t.op == 0) {
+                t.enc.data = pp;
+                t.enc.parity = b->d_ptrs.as<uint8_t *>() + lp.parity_pool;
+            } else {
+                t.dec.rows = pp;
+                t.dec.out = pp + t.dec.n;
+                t.dec.errlocs = reinterpret_cast<const uint16_t *>(b->d_leo_aux.as<uint8_t>() + lp.errl_off);
+                t.dec.out_rows = reinterpret_cast<const int *>(b->d_leo_aux.as<uint8_t>() + lp.rows_off);
+                t.dec.out_logs = reinterpret_cast<const uint16_t *>(b->d_leo_aux.as<uint8_t>() + lp.logs_off);
+            }
+            if (static_cast<uint64_t>(slice0) + lp.slices > 0xFFFFFFFFull)
+                return fail(RSX_ERR_INVALID_INPUT, "batch too large for one plan");
+            slice0 += lp.slices;
+            b->leo_lds[fi] = std::max(b->leo_lds[fi], lp.lds);
+            b->leo_tasks[fi].push_back(t);
+        }
+        b->leo_slices[fi] = slice0;
+        if (b->leo_tasks[fi].empty()) continue;
+        const size_t bytes = b->leo_tasks[fi].size() * sizeof(LeoBatchTask);
+        if ((err = b->d_leo_tasks[fi].reserve(bytes))) return err;
+        RSX_HIP(hipMemcpyAsync(b->d_leo_tasks[fi].ptr, b->leo_tasks[fi].data(), bytes, hipMemcpyHostToDevice,
+                               b->stream));
+    }
     RSX_HIP(hipStreamSynchronize(b->stream));
     if (b->tuned) {
         if ((err = build_launches(b, b->tile))) return err;
@@ -514,6 +624,7 @@ int rsx_batch_synchronize(rsx_batch *b) {
 int rsx_batch_num_tasks(const rsx_batch *b) {
     size_t n = 0;
     for (auto &v : b->tasks) n += v.size();
+    for (auto &v : b->leo_tasks) n += v.size();
     return static_cast<int>(n);
 }
 

@@ -16,45 +16,78 @@ namespace {
 using namespace leo;
 using namespace leo_tile;
 
+// One column slice (bytes [col, col + slice) of every shard) of an encode.
+template <class F>
+__device__ __forceinline__ void encode_slice(const LeoDev &f, const LeoEncodeArgs &a, uint8_t *lds, size_t col) {
+    const int ushift = log2i(a.slice / F::kUnitBytes);
+    const int region = a.slice * (a.m + 1);
+    const int cols = static_cast<int>(a.size - col < static_cast<size_t>(a.slice) ? a.size - col : a.slice);
+    const Tile acc{lds, a.m + 1, ushift, cols / F::kUnitBytes};
+    const Tile tmp{lds + region, a.m + 1, ushift, cols / F::kUnitBytes};
+    for (int g = 0; g < a.groups; ++g) {
+        const Tile &w = g == 0 ? acc : tmp;
+        const int mtrunc = g == 0 ? (a.k < a.m ? a.k : a.m) : (a.k - g * a.m < a.m ? a.k - g * a.m : a.m);
+        load_rows<F>(w, a.data + static_cast<size_t>(g) * a.m, nullptr, a.m, col, f);
+        ifft<F>(w, a.m, mtrunc, f.skew + a.m - 1 + g * a.m, 0, f);
+        if (g > 0) xor_tiles(acc, tmp, region / 16);
+    }
+    fft<F>(acc, a.m, a.p, f);
+    store_rows<F>(acc, a.parity, nullptr, nullptr, a.p, col, f);
+    __syncthreads();  // the next slice reuses the rows
+}
+
+// One column slice of a reconstruct.
+template <class F>
+__device__ __forceinline__ void decode_slice(const LeoDev &f, const LeoDecodeArgs &a, uint8_t *lds, size_t col) {
+    const int ushift = log2i(a.slice / F::kUnitBytes);
+    const int cols = static_cast<int>(a.size - col < static_cast<size_t>(a.slice) ? a.size - col : a.slice);
+    const Tile w{lds, a.n + 1, ushift, cols / F::kUnitBytes};
+    load_rows<F>(w, a.rows, a.errlocs, a.n, col, f);
+    ifft<F>(w, a.n, a.ifft_trunc, f.skew, -1, f);
+    derivative(w, a.n, cols / 16);
+    fft<F>(w, a.n, a.fft_trunc, f);
+    store_rows<F>(w, a.out, a.out_rows, a.out_logs, a.nout, col, f);
+    __syncthreads();
+}
+
 template <class F>
 __global__ __launch_bounds__(kFBMax) void k_encode_fused(const LeoDev f, const LeoEncodeArgs a) {
     extern __shared__ uint4 lds_raw[];
-    uint8_t *lds = reinterpret_cast<uint8_t *>(lds_raw);
-    const int ushift = log2i(a.slice / F::kUnitBytes);
-    const int region = a.slice * (a.m + 1);
     for (size_t col = static_cast<size_t>(blockIdx.x) * a.slice; col < a.size;
-         col += static_cast<size_t>(gridDim.x) * a.slice) {
-        const int cols = static_cast<int>(a.size - col < static_cast<size_t>(a.slice) ? a.size - col : a.slice);
-        const Tile acc{lds, a.m + 1, ushift, cols / F::kUnitBytes};
-        const Tile tmp{lds + region, a.m + 1, ushift, cols / F::kUnitBytes};
-        for (int g = 0; g < a.groups; ++g) {
-            const Tile &w = g == 0 ? acc : tmp;
-            const int mtrunc = g == 0 ? (a.k < a.m ? a.k : a.m) : (a.k - g * a.m < a.m ? a.k - g * a.m : a.m);
-            load_rows<F>(w, a.data + static_cast<size_t>(g) * a.m, nullptr, a.m, col, f);
-            ifft<F>(w, a.m, mtrunc, f.skew + a.m - 1 + g * a.m, 0, f);
-            if (g > 0) xor_tiles(acc, tmp, region / 16);
-        }
-        fft<F>(acc, a.m, a.p, f);
-        store_rows<F>(acc, a.parity, nullptr, nullptr, a.p, col, f);
-        __syncthreads();  // the next slice reuses the rows
-    }
+         col += static_cast<size_t>(gridDim.x) * a.slice)
+        encode_slice<F>(f, a, reinterpret_cast<uint8_t *>(lds_raw), col);
 }
 
 template <class F>
 __global__ __launch_bounds__(kFBMax) void k_decode_fused(const LeoDev f, const LeoDecodeArgs a) {
     extern __shared__ uint4 lds_raw[];
-    uint8_t *lds = reinterpret_cast<uint8_t *>(lds_raw);
-    const int ushift = log2i(a.slice / F::kUnitBytes);
     for (size_t col = static_cast<size_t>(blockIdx.x) * a.slice; col < a.size;
-         col += static_cast<size_t>(gridDim.x) * a.slice) {
-        const int cols = static_cast<int>(a.size - col < static_cast<size_t>(a.slice) ? a.size - col : a.slice);
-        const Tile w{lds, a.n + 1, ushift, cols / F::kUnitBytes};
-        load_rows<F>(w, a.rows, a.errlocs, a.n, col, f);
-        ifft<F>(w, a.n, a.ifft_trunc, f.skew, -1, f);
-        derivative(w, a.n, cols / 16);
-        fft<F>(w, a.n, a.fft_trunc, f);
-        store_rows<F>(w, a.out, a.out_rows, a.out_logs, a.nout, col, f);
-        __syncthreads();
+         col += static_cast<size_t>(gridDim.x) * a.slice)
+        decode_slice<F>(f, a, reinterpret_cast<uint8_t *>(lds_raw), col);
+}
+
+// Batched: blocks stride over the global slice index of all tasks, find
+// their task by binary search over the first-slice offsets, and run that
+// task's slice.  The task is uniform per block.
+template <class F>
+__global__ __launch_bounds__(kFB) void k_leo_batch(const LeoDev f, const LeoBatchTask *tasks, int ntasks,
+                                                   uint32_t nslices) {
+    extern __shared__ uint4 lds_raw[];
+    uint8_t *lds = reinterpret_cast<uint8_t *>(lds_raw);
+    for (uint32_t g = blockIdx.x; g < nslices; g += gridDim.x) {
+        int lo = 0, hi = ntasks - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (tasks[mid].slice0 <= g) lo = mid;
+            else hi = mid - 1;
+        }
+        if (tasks[lo].op == 0) {
+            const LeoEncodeArgs a = tasks[lo].enc;
+            encode_slice<F>(f, a, lds, static_cast<size_t>(g - tasks[lo].slice0) * a.slice);
+        } else {
+            const LeoDecodeArgs a = tasks[lo].dec;
+            decode_slice<F>(f, a, lds, static_cast<size_t>(g - tasks[lo].slice0) * a.slice);
+        }
     }
 }
 
@@ -103,6 +136,20 @@ hipError_t leo_encode_fused(const LeoDev &f, const LeoEncodeArgs &a, hipStream_t
     } else {
         if ((e = allow_lds(k_encode_fused<F8>, lds)) != hipSuccess) return e;
         hipLaunchKernelGGL(k_encode_fused<F8>, dim3(grid_for(a.size, a.slice)), dim3(threads), lds, s, f, a);
+    }
+    return hipGetLastError();
+}
+
+hipError_t leo_batch_launch(const LeoDev &f, const LeoBatchTask *tasks, int ntasks, uint32_t nslices, size_t lds,
+                            unsigned grid, hipStream_t s) {
+    if (ntasks <= 0 || nslices == 0) return hipSuccess;
+    hipError_t e;
+    if (f.bits == 16) {
+        if ((e = allow_lds(k_leo_batch<F16>, lds)) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_leo_batch<F16>, dim3(grid), dim3(kFB), lds, s, f, tasks, ntasks, nslices);
+    } else {
+        if ((e = allow_lds(k_leo_batch<F8>, lds)) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_leo_batch<F8>, dim3(grid), dim3(kFB), lds, s, f, tasks, ntasks, nslices);
     }
     return hipGetLastError();
 }

@@ -83,6 +83,20 @@ struct LeoDecodeArgs {
     int slice;
 };
 
+// One Leopard item of a batched launch: op 0 = encode (enc), 1 = reconstruct
+// (dec); its column slices are global slices [slice0, slice0 + slices).
+struct LeoBatchTask {
+    LeoEncodeArgs enc;
+    LeoDecodeArgs dec;
+    uint32_t slice0;
+    uint32_t op;
+};
+
+// Runs every task's slices (tasks sorted by slice0, all of one field) with
+// `grid` persistent blocks of 256 threads and `lds` bytes (the largest task).
+hipError_t leo_batch_launch(const LeoDev &f, const LeoBatchTask *tasks, int ntasks, uint32_t nslices, size_t lds,
+                            unsigned grid, hipStream_t s);
+
 // Slice width for `regions` LDS regions of `rows` rows, radix-4 steps over
 // `step_rows` rows and shards of `size` bytes, or 0 when even 64-byte slices
 // exceed `lds_budget`.

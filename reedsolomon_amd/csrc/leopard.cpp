@@ -516,4 +516,122 @@ int leopard_reconstruct(rsx_codec *c, uint8_t *const *shards, const size_t *lens
     return RSX_OK;
 }
 
+namespace {
+
+// Shard size handed to the slice chooser for batched items: large, so the
+// slice is sized for full workgroups rather than for slice count.
+constexpr size_t kBatchedSize = size_t{1} << 40;
+
+size_t aux_append(std::vector<uint8_t> &aux, const void *src, size_t bytes) {
+    const size_t off = (aux.size() + 15) / 16 * 16;
+    aux.resize(off + bytes);
+    std::memcpy(aux.data() + off, src, bytes);
+    return off;
+}
+
+}  // namespace
+
+int leopard_plan_item(rsx_codec *c, int op, uint8_t *const *shards, const size_t *lens, int nshards,
+                      std::vector<const uint8_t *> &pool, std::vector<uint8_t> &aux, LeoItemPlan &out,
+                      uint8_t *filled) {
+    const int k = c->k, p = c->m, total = c->total;
+    LeopardState &s = *c->leo;
+    out = LeoItemPlan{};
+    out.bits = s.bits;
+    out.dev = s.dev;
+    if (nshards != total) return fail(RSX_ERR_TOO_FEW_SHARDS, "too few shards given");
+    const bool rec = op == RSX_OP_RECONSTRUCT || op == RSX_OP_RECONSTRUCT_DATA;
+    int err = check(lens, nshards, rec);
+    if (err) return err;
+    const size_t size = shard_size_of(lens, nshards);
+    const int m = ceil_pow2(p), n = ceil_pow2(m + k);
+    std::vector<uint8_t> erased(total, 0);
+    int present = 0, data_present = 0;
+    for (int i = 0; i < total; ++i) {
+        erased[i] = lens[i] == 0;
+        if (!erased[i]) {
+            ++present;
+            if (i < k) ++data_present;
+        }
+    }
+    if (rec) {
+        // leopard.go:421-440: nothing to do, or too few shards.
+        if (present == total || (op == RSX_OP_RECONSTRUCT_DATA && data_present == k)) {
+            out.empty = true;
+            return RSX_OK;
+        }
+        if (present < k) return fail(RSX_ERR_TOO_FEW_SHARDS, "too few shards given");
+    }
+    if (size % 64) return fail(RSX_ERR_INVALID_SHARD_SIZE, "shard size must be a multiple of 64");
+    const bool recover_all = op != RSX_OP_RECONSTRUCT_DATA;
+    const int end = recover_all ? total : k;
+    // Device memory, 16-byte aligned, a buffer for every output.
+    DeviceRanges ranges;
+    uintptr_t bits = 0;
+    for (int i = 0; i < total; ++i) {
+        const bool used = !erased[i] || (rec && i < end) || op == RSX_OP_ENCODE;
+        if (!used) continue;
+        if (!shards[i]) return fail(RSX_ERR_INVALID_INPUT, "no buffer for a shard of a batched item");
+        if (!ranges.is_device(shards[i]))
+            return fail(RSX_ERR_MIXED_MEMORY, "batched items need shards in device memory");
+        bits |= reinterpret_cast<uintptr_t>(shards[i]);
+    }
+    if (bits & 15) return fail(RSX_ERR_NOT_SUPPORTED, "batched Leopard items need 16-byte aligned shards");
+    const LeoDev &f = *s.dev;
+    if (!rec) {
+        const int groups = (k + m - 1) / m;
+        // Other items fill the device: size the slice for full workgroups.
+        const int slice = leo_fused_slice(s.bits, m, groups > 1 ? 2 : 1, m, kBatchedSize, kLeoFusedEncodeLds);
+        if (!slice) return fail(RSX_ERR_NOT_SUPPORTED, "Leopard shape too tall for the batched kernel");
+        out.ptr_off = pool.size();
+        for (int g = 0; g < groups; ++g)
+            for (int j = 0; j < m; ++j) {
+                const int r = g * m + j;
+                pool.push_back(r < k ? shards[r] : nullptr);
+            }
+        out.parity_pool = pool.size();
+        for (int i = 0; i < p; ++i) pool.push_back(shards[k + i]);
+        out.task.op = 0;
+        out.task.enc = LeoEncodeArgs{nullptr, nullptr, size, k, p, m, groups, slice};
+        out.lds = static_cast<size_t>(groups > 1 ? 2 : 1) * slice * (m + 1);
+        out.slices = static_cast<uint32_t>((size + slice - 1) / slice);
+        (void)f;
+        return RSX_OK;
+    }
+    const int slice = leo_fused_slice(s.bits, n, 1, n, kBatchedSize, std::min(kLeoFusedEncodeLds,
+                                                                               kLeoFusedDecodeLds / n * (n + 1)));
+    if (!slice) return fail(RSX_ERR_NOT_SUPPORTED, "Leopard shape too tall for the batched kernel");
+    LeoPattern *pat = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(c->mu);
+        if ((err = pattern_for(c, erased.data(), recover_all, &pat))) return err;
+    }
+    out.ptr_off = pool.size();
+    std::vector<const uint8_t *> tab(static_cast<size_t>(n), nullptr);
+    for (int i = 0; i < p; ++i) tab[i] = erased[k + i] ? nullptr : shards[k + i];
+    for (int i = 0; i < k; ++i) tab[m + i] = erased[i] ? nullptr : shards[i];
+    std::vector<int> rows;
+    std::vector<uint16_t> logs;
+    for (int i = 0; i < end; ++i) {
+        if (!erased[i]) continue;
+        const int pos = i >= k ? i - k : i + m;
+        rows.push_back(pos);
+        logs.push_back(static_cast<uint16_t>(s.field->modulus - pat->errlocs[pos]));
+        tab.push_back(shards[i]);
+        if (filled) filled[i] = 1;
+    }
+    pool.insert(pool.end(), tab.begin(), tab.end());
+    out.errl_off = aux_append(aux, pat->errlocs.data(), pat->errlocs.size() * sizeof(uint16_t));
+    out.rows_off = aux_append(aux, rows.data(), rows.size() * sizeof(int));
+    out.logs_off = aux_append(aux, logs.data(), logs.size() * sizeof(uint16_t));
+    const int nout = static_cast<int>(rows.size());
+    out.task.op = 1;
+    out.task.dec = LeoDecodeArgs{nullptr, nullptr, nullptr, nullptr, nullptr, size, n, nout,
+                                 s.bits == 16 ? pat->input_count : m + k, m + k, slice};
+    out.lds = static_cast<size_t>(slice) * (n + 1);
+    out.slices = static_cast<uint32_t>((size + slice - 1) / slice);
+    out.empty = nout == 0;
+    return RSX_OK;
+}
+
 }  // namespace rsx

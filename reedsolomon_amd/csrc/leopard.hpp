@@ -4,7 +4,10 @@
 #include <cstddef>
 #include <cstdint>
 
+#include <vector>
+
 #include "codec.hpp"
+#include "leo_kernels.hpp"
 
 namespace rsx {
 
@@ -17,5 +20,31 @@ int leopard_encode(rsx_codec *c, uint8_t *const *shards, const size_t *lens, int
 int leopard_verify(rsx_codec *c, uint8_t *const *shards, const size_t *lens, int nshards, int *ok);
 int leopard_reconstruct(rsx_codec *c, uint8_t *const *shards, const size_t *lens, int nshards,
                         bool recover_all, uint8_t *filled);
+
+// One Leopard item of a batched plan.  Pointer fields of `task` are left
+// null; the offsets say where their contents were appended: `ptr_off` into
+// the plan's pointer pool (encode: groups*m data rows then p parity rows;
+// reconstruct: n work rows then nout outputs), `errl_off` / `rows_off` /
+// `logs_off` (bytes) into the plan's auxiliary buffer.
+struct LeoItemPlan {
+    int bits = 0;
+    const LeoDev *dev = nullptr;
+    LeoBatchTask task{};
+    size_t lds = 0;
+    uint32_t slices = 0;
+    size_t ptr_off = 0, errl_off = 0, rows_off = 0, logs_off = 0;
+    size_t parity_pool = 0;  // encode: pool index of the first parity pointer
+    bool empty = false;      // nothing to do (no erasure to fill)
+};
+
+// Validates one batched Leopard item as the per-stripe method would (op:
+// RSX_OP_ENCODE / _VERIFY (planned as an encode) / _RECONSTRUCT /
+// _RECONSTRUCT_DATA) and plans it for the LDS-resident batch kernel.
+// Returns the item's status; RSX_ERR_NOT_SUPPORTED where the batched form
+// does not apply (shards not 16-byte aligned, or rows that do not fit one
+// LDS tile).
+int leopard_plan_item(rsx_codec *c, int op, uint8_t *const *shards, const size_t *lens, int nshards,
+                      std::vector<const uint8_t *> &pool, std::vector<uint8_t> &aux, LeoItemPlan &out,
+                      uint8_t *filled);
 
 }  // namespace rsx

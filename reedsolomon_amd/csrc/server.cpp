@@ -5,10 +5,12 @@
 // loops: (1) retire launch slots whose completion event has fired, (2) when a
 // slot is free, admit queued requests in order while the shard bytes in
 // flight stay within the HBM budget (a request larger than the budget is
-// admitted alone), plan the admitted classic requests into that slot's batch
-// and launch it on the slot's stream, (3) hand Leopard requests to a worker
-// thread.  Slots have their own streams, so a batch planned from requests
-// that arrived while another batch runs overlaps with it on the device.
+// admitted alone), plan the admitted requests (classic and Leopard) into that
+// slot's batch and launch it on the slot's stream, (3) hand the Leopard
+// requests the batch kernel does not take (unaligned shards, rows too tall
+// for one LDS tile) to a worker thread that runs the per-stripe call.
+// Slots have their own streams, so a batch planned from requests that
+// arrived while another batch runs overlaps with it on the device.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -139,19 +141,19 @@ void scheduler_loop(rsx_server *s) {
                 break;
             }
         std::vector<uint64_t> admitted;
+        size_t batch_bytes = 0;
         while (free_slot && !s->queue.empty() && static_cast<int>(admitted.size()) < s->max_batch) {
             const uint64_t t = s->queue.front();
             Req &q = s->reqs[t];
             if (s->inflight > 0 && s->inflight + q.bytes > s->budget) break;  // FIFO: wait for room
+            // A batch takes at most an eighth of the budget, so under load the
+            // next batch is planned while earlier ones run.
+            if (!admitted.empty() && batch_bytes + q.bytes > s->budget / 8) break;
+            batch_bytes += q.bytes;
             s->queue.pop_front();
             s->inflight += q.bytes;
             s->max_inflight = std::max(s->max_inflight, s->inflight);
-            if (q.leopard) {
-                s->work.push_back(t);
-                s->work_cv.notify_one();
-            } else {
-                admitted.push_back(t);
-            }
+            admitted.push_back(t);
         }
         if (!admitted.empty()) {
             std::vector<rsx_batch_item> items(admitted.size());
@@ -176,8 +178,16 @@ void scheduler_loop(rsx_server *s) {
             ++s->batches;
             s->batched_requests += admitted.size();
             for (size_t i = 0; i < admitted.size(); ++i) {
-                if (rc != RSX_OK || status[i] != RSX_OK) complete(s, admitted[i], rc != RSX_OK ? rc : status[i], 0);
-                else free_slot->tickets.emplace_back(admitted[i], static_cast<int>(i));
+                if (rc == RSX_OK && status[i] == RSX_ERR_NOT_SUPPORTED && s->reqs[admitted[i]].leopard) {
+                    // Leopard shapes / alignments the batch kernel does not
+                    // take run through the per-stripe call on the worker.
+                    s->work.push_back(admitted[i]);
+                    s->work_cv.notify_one();
+                } else if (rc != RSX_OK || status[i] != RSX_OK) {
+                    complete(s, admitted[i], rc != RSX_OK ? rc : status[i], 0);
+                } else {
+                    free_slot->tickets.emplace_back(admitted[i], static_cast<int>(i));
+                }
             }
             free_slot->busy = !free_slot->tickets.empty();
             continue;  // look for more work right away

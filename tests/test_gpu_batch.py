@@ -113,3 +113,87 @@ def test_manual_tuning_gives_same_bytes(tile, bpc):
     assert b.tuning() == (tile, bpc)
     for i in range(10, 14):
         assert np.array_equal(shards[i].cpu().numpy(), ref[i])
+
+
+def _leo_codeword(rng, bits, k, m, size):
+    data = [rng.integers(0, 256, size, dtype=np.uint8) for _ in range(k)]
+    full = data + [np.zeros(size, np.uint8) for _ in range(m)]
+    assert rsref.Leopard(k, m, bits).encode(full) == 0
+    return full
+
+
+LEO_CASES = [(8, 192, 56, 65536), (8, 10, 3, 64 * 37), (8, 64, 64, 4096), (16, 100, 20, 64 * 41),
+             (16, 300, 300, 192), (16, 33, 7, 64 * 300)]
+
+
+def test_batched_leopard_items_match_oracle_and_plain():
+    rng = np.random.default_rng(808)
+    items, checks = [], []
+    for bits, k, m, size in LEO_CASES:
+        enc = rs.New(k, m, rs.WithLeopardGF16(True) if bits == 16 else rs.WithLeopardGF(True))
+        full = _leo_codeword(rng, bits, k, m, size)
+        dev = [torch.from_numpy(x).to("cuda") for x in full]
+        # encode into stale parity
+        shards = dev[:k] + [torch.from_numpy(rng.integers(0, 256, size, dtype=np.uint8)).to("cuda") for _ in range(m)]
+        items.append((enc, rs.OP_ENCODE, shards))
+        checks.append(("enc", k, m, full, shards))
+        # verify: good and corrupted
+        good = [x.clone() for x in dev]
+        bad = [x.clone() for x in dev]
+        bad[int(rng.integers(0, k + m))][int(rng.integers(0, size))] ^= 0x21
+        items.append((enc, rs.OP_VERIFY, good))
+        checks.append(("ver", True))
+        items.append((enc, rs.OP_VERIFY, bad))
+        checks.append(("ver", False))
+        # reconstruct / reconstruct data
+        for op in (rs.OP_RECONSTRUCT, rs.OP_RECONSTRUCT_DATA):
+            lost = set(rng.choice(k + m, int(rng.integers(1, m + 1)), replace=False).tolist())
+            work = [None if i in lost else dev[i].clone() for i in range(k + m)]
+            items.append((enc, op, work))
+            checks.append(("rec", k, m, full, work, lost, op))
+    # a classic item in the same plan
+    cenc = rs.New(10, 4)
+    cdata = [rng.integers(0, 256, 5000, dtype=np.uint8) for _ in range(10)]
+    cref = cdata + [np.zeros(5000, np.uint8) for _ in range(4)]
+    rsref.Classic(10, 4).encode(cref)
+    cshards = [torch.from_numpy(x).to("cuda") for x in cdata] + [torch.zeros(5000, dtype=torch.uint8, device="cuda")
+                                                               for _ in range(4)]
+    items.append((cenc, rs.OP_ENCODE, cshards))
+    checks.append(("enc", 10, 4, cref, cshards))
+    b = rs.Batch(0)
+    for use_graph in (True, False):
+        b.set_graph(use_graph)
+        status = b.run(items)
+        assert status == [0] * len(items), status
+        for idx, c in enumerate(checks):
+            if c[0] == "enc":
+                _, k, m, full, shards = c
+                for i in range(k, k + m):
+                    assert np.array_equal(shards[i].cpu().numpy(), full[i]), (idx, i)
+            elif c[0] == "ver":
+                assert b.verify_result(idx) == c[1], idx
+            else:
+                _, k, m, full, work, lost, op = c
+                for i in range(k + m):
+                    if i in lost and (op == rs.OP_RECONSTRUCT or i < k):
+                        assert np.array_equal(work[i].cpu().numpy(), full[i]), (idx, i)
+                    elif i in lost:
+                        assert work[i] is None
+    assert b.num_tasks() > 0
+
+
+def test_batched_leopard_unsupported_items_are_reported():
+    rng = np.random.default_rng(9)
+    # GF(2^16) reconstruct with n = 2048 work rows: not one LDS tile.
+    k, m, size = 1000, 100, 128
+    enc = rs.New(k, m, rs.WithLeopardGF16(True))
+    full = _leo_codeword(rng, 16, k, m, size)
+    work = [None if i < 3 else torch.from_numpy(full[i]).to("cuda") for i in range(k + m)]
+    # unaligned GF(2^8) encode
+    enc8 = rs.New(4, 2, rs.WithLeopardGF(True))
+    pool = torch.zeros(6 * 64 + 1, dtype=torch.uint8, device="cuda")
+    unaligned = [pool[1 + 64 * i: 1 + 64 * (i + 1)] for i in range(6)]
+    short = [torch.zeros(100, dtype=torch.uint8, device="cuda") for _ in range(6)]
+    b = rs.Batch(0)
+    st = b.run([(enc, rs.OP_RECONSTRUCT, work), (enc8, rs.OP_ENCODE, unaligned), (enc8, rs.OP_ENCODE, short)])
+    assert st == [rs.ErrNotSupported.code, rs.ErrNotSupported.code, rs.ErrInvalidShardSize.code]
