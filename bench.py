@@ -66,6 +66,8 @@ def _parse():
                     help="served stream: Poisson rate as a fraction of the measured saturation rate")
     ap.add_argument("--hbm-budget-gib", type=float, default=16.0,
                     help="served stream: shard bytes admitted in flight")
+    ap.add_argument("--only", choices=["rs10_4", "mixed", "leopard16", "served"],
+                    help="run one secondary section alone and print its JSON (for profiling)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="with --encode-only: also copy 1 GiB HBM->HBM 5 times (counter calibration)")
     ap.add_argument("--encode-only", action="store_true",
@@ -608,6 +610,10 @@ def main() -> int:
     import reedsolomon_amd as rs
 
     torch.cuda.set_device(0)
+    if args.only:
+        fn = {"rs10_4": rs10_4, "mixed": mixed, "leopard16": leopard, "served": served}[args.only]
+        print(json.dumps({"section": args.only, **fn(args, torch, rs)}), flush=True)
+        return 0
     head = headline(args, torch, rs)
     if args.encode_only:
         print(json.dumps({"kernel": "batch_pass<4>", **head}), flush=True)
