@@ -116,6 +116,27 @@ int rsx_reconstruct_data(rsx_codec *c, uint8_t *const *shards, const size_t *len
 int rsx_reconstruct_some(rsx_codec *c, uint8_t *const *shards, const size_t *lens, int nshards,
                          const uint8_t *required, int nrequired, uint8_t *filled);
 
+/* Encoder.Update, reedsolomon.go:762-818 (Leopard: ErrNotSupported,
+ * leopard.go:316).  shards: old data (NULL where unchanged) then the parity,
+ * updated in place; new_data: k entries, NULL where unchanged.  A NULL
+ * pointer is Go's nil.  As in the reference, each changed old data shard is
+ * overwritten with old XOR new.  Lengths of the changed shards and of the
+ * parity must equal the shard size (the reference reads past shorter ones;
+ * here that is ErrShardSize). */
+int rsx_update(rsx_codec *c, uint8_t *const *shards, const size_t *lens, int nshards,
+               uint8_t *const *new_data, const size_t *new_lens, int nnew);
+
+/* Extensions.DecodeIdx, reedsolomon.go:1464-1617 (Leopard: ErrNotSupported).
+ * Progressive decode: dst[i] ^= (row i of the decode matrix) * input for the
+ * inputs given in this call, the decode matrix built from the first k shards
+ * marked in expect_input.  expect_input == NULL is the merge mode
+ * (dst[i] ^= input[i]).  NULL pointers are nil.  The reference's joined
+ * errors map to their sentinel (ErrInvalidInput, ErrInvalidShardSize,
+ * ErrTooFewShards). */
+int rsx_decode_idx(rsx_codec *c, uint8_t *const *dst, const size_t *dst_lens, int ndst,
+                   const uint8_t *expect_input, int nexpect, uint8_t *const *input,
+                   const size_t *input_lens, int ninput);
+
 /* Stream control for device-resident calls (no reference counterpart: the Go
  * methods are synchronous; the shim calls rsx_synchronize before returning
  * when it hands over device buffers it must read). */

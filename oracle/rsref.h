@@ -76,6 +76,12 @@ int rsref_verify(const uint8_t *matrix, int k, int m, uint8_t *const *shards, co
 int rsref_reconstruct(const uint8_t *matrix, int k, int m, uint8_t *const *shards,
                       const size_t *lens, int nshards, int data_only, const uint8_t *required,
                       int nrequired, uint8_t *filled);
+/* Update / DecodeIdx: a NULL shard pointer is Go's nil. */
+int rsref_update(const uint8_t *matrix, int k, int m, uint8_t *const *shards, const size_t *lens,
+                 int nshards, uint8_t *const *newd, const size_t *nlens, int nnew);
+int rsref_decode_idx(const uint8_t *matrix, int k, int m, uint8_t *const *dst, const size_t *dlens,
+                     int ndst, const uint8_t *expect, int nexpect, uint8_t *const *input,
+                     const size_t *ilens, int ninput);
 
 /* Fast path of the classic product for the CPU baseline (rsref_fast.c):
  * isa 0 scalar, 1 AVX2 PSHUFB, 2 AVX-512+GFNI, -1 = best available. */

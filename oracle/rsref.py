@@ -90,6 +90,27 @@ class _Shards:
         self.n = n
 
 
+class _NilShards:
+    """Pointer + length arrays where None is Go's nil (a NULL pointer)."""
+
+    def __init__(self, shards):
+        n = len(shards)
+        self.ptrs = (ctypes.c_void_p * max(n, 1))()
+        self.lens = (ctypes.c_size_t * max(n, 1))()
+        for i, s in enumerate(shards):
+            if s is None:
+                self.ptrs[i] = None
+                self.lens[i] = 0
+            else:
+                assert s.dtype == np.uint8 and s.flags["C_CONTIGUOUS"]
+                self.ptrs[i] = s.ctypes.data if len(s) else ctypes.addressof(_EMPTY)
+                self.lens[i] = len(s)
+        self.n = n
+
+
+_EMPTY = (ctypes.c_uint8 * 1)()  # backing for non-nil empty slices
+
+
 def gf8_tables():
     e = np.zeros(256, np.uint8)
     lg = np.zeros(256, np.uint8)
@@ -214,6 +235,21 @@ class Classic:
 
     def reconstruct_data(self, shards) -> int:
         return self._reconstruct(shards, True, None)
+
+    def update(self, shards, new_data) -> int:
+        """reedsolomon.go:762 Update (None = nil); modifies shards in place."""
+        s, nd = _NilShards(shards), _NilShards(new_data)
+        return lib().rsref_update(_ptr(self.matrix), self.k, self.m, s.ptrs, s.lens, s.n, nd.ptrs, nd.lens, nd.n)
+
+    def decode_idx(self, dst, expect_input, inp) -> int:
+        """reedsolomon.go:1464 DecodeIdx (None = nil; expect_input None = merge mode)."""
+        d, i = _NilShards(dst), _NilShards(inp)
+        if expect_input is None:
+            return lib().rsref_decode_idx(_ptr(self.matrix), self.k, self.m, d.ptrs, d.lens, d.n, None, 0,
+                                          i.ptrs, i.lens, i.n)
+        ex = np.asarray(expect_input, np.uint8)
+        return lib().rsref_decode_idx(_ptr(self.matrix), self.k, self.m, d.ptrs, d.lens, d.n, _ptr(ex), len(ex),
+                                      i.ptrs, i.lens, i.n)
 
     def reconstruct_some(self, shards, required) -> int:
         # reedsolomon.go:1655-1660

@@ -454,3 +454,138 @@ out:
     free(inv);
     return err;
 }
+
+/* ------------------------------------------------------------------------ */
+/* reedsolomon.go:762-818 Update / updateParityShards.  A NULL pointer is    */
+/* Go's nil.  Each changed old data shard becomes old XOR new, as there.    */
+/* ------------------------------------------------------------------------ */
+int rsref_update(const uint8_t *matrix, int k, int m, uint8_t *const *shards, const size_t *lens,
+                 int nshards, uint8_t *const *newd, const size_t *nlens, int nnew) {
+    if (nshards != k + m) return RSREF_ERR_TOO_FEW_SHARDS;
+    if (nnew != k) return RSREF_ERR_TOO_FEW_SHARDS;
+    int err = check_shards(lens, nshards, 1);
+    if (err) return err;
+    err = check_shards(nlens, nnew, 1);
+    if (err) return err;
+    for (int i = 0; i < k; i++)
+        if (newd[i] && !shards[i]) return RSREF_ERR_INVALID_INPUT;
+    for (int i = k; i < k + m; i++)
+        if (!shards[i]) return RSREF_ERR_INVALID_INPUT;
+    size_t size = shard_size(lens, nshards);
+    /* The reference slices past shorter buffers here; the restatement
+     * rejects mismatched lengths instead. */
+    for (int i = 0; i < k; i++)
+        if (newd[i] && (nlens[i] != size || lens[i] != size)) return RSREF_ERR_SHARD_SIZE;
+    for (int i = k; i < k + m; i++)
+        if (lens[i] != size) return RSREF_ERR_SHARD_SIZE;
+    for (int c = 0; c < k; c++) {
+        if (!newd[c]) continue;
+        for (size_t b = 0; b < size; b++) shards[c][b] ^= newd[c][b];
+        for (int r = 0; r < m; r++)
+            rsref_gal_mul_slice(matrix[(size_t)(k + r) * k + c], shards[c], shards[k + r], size, 1);
+    }
+    return RSREF_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* reedsolomon.go:1464-1617 DecodeIdx.  A NULL pointer is Go's nil;          */
+/* expect == NULL is the merge mode.                                         */
+/* ------------------------------------------------------------------------ */
+int rsref_decode_idx(const uint8_t *matrix, int k, int m, uint8_t *const *dst, const size_t *dlens,
+                     int ndst, const uint8_t *expect, int nexpect, uint8_t *const *input,
+                     const size_t *ilens, int ninput) {
+    int total = k + m;
+    if (!expect) {
+        if (ndst != ninput) return RSREF_ERR_INVALID_INPUT;
+        for (int i = 0; i < ndst; i++) {
+            if (!input[i]) continue;
+            if (!dst[i]) return RSREF_ERR_INVALID_INPUT;
+            if (dlens[i] != ilens[i]) return RSREF_ERR_INVALID_SHARD_SIZE;
+            for (size_t b = 0; b < ilens[i]; b++) dst[i][b] ^= input[i][b];
+        }
+        return RSREF_OK;
+    }
+    if (nexpect != total || ndst != total || ninput != total) return RSREF_ERR_INVALID_INPUT;
+    for (int i = 0; i < total; i++)
+        if (input[i] && !expect[i]) return RSREF_ERR_INVALID_INPUT;
+    for (int i = 0; i < total; i++)
+        if (expect[i] && dst[i]) return RSREF_ERR_INVALID_INPUT;
+    int *valid = malloc(sizeof(int) * total);
+    int nvalid = 0;
+    for (int i = 0; i < total; i++)
+        if (expect[i]) valid[nvalid++] = i;
+    if (nvalid < k) {
+        free(valid);
+        return RSREF_ERR_TOO_FEW_SHARDS;
+    }
+    /* getDecodeMatrix: inverse of the rows of the first k valid indices. */
+    uint8_t *subm = malloc((size_t)k * k), *inv = malloc((size_t)k * k);
+    for (int r = 0; r < k; r++) memcpy(subm + r * k, matrix + (size_t)valid[r] * k, k);
+    int err = rsref_matrix_invert(subm, k, k, inv);
+    free(subm);
+    size_t size = 0;
+    int have = 0;
+    if (!err) {
+        for (int i = 0; i < total && !err; i++) {
+            if (!input[i]) continue;
+            if (!have) {
+                size = ilens[i];
+                have = size != 0;
+            } else if (ilens[i] != size) {
+                err = RSREF_ERR_INVALID_SHARD_SIZE;
+            }
+        }
+        for (int i = 0; i < total && !err; i++)
+            if (dst[i] && dlens[i] != size) err = RSREF_ERR_INVALID_SHARD_SIZE;
+    }
+    if (err) {
+        free(valid);
+        free(inv);
+        return err;
+    }
+    /* rows: one per non-nil dst, nvalid columns (zero past column k). */
+    uint8_t *rows = calloc((size_t)total * nvalid, 1);
+    uint8_t **outs = malloc(sizeof(uint8_t *) * total);
+    int nout = 0;
+    for (int d = 0; d < total; d++) {
+        if (!dst[d]) continue;
+        uint8_t *row = rows + (size_t)nout * nvalid;
+        if (d < k) {
+            memcpy(row, inv + (size_t)d * k, k);
+        } else {
+            const uint8_t *prow = matrix + (size_t)d * k;
+            for (int col = 0; col < nvalid && col < k; col++) {
+                uint8_t v = 0;
+                for (int i = 0; i < k; i++) v ^= g_mul[prow[i]][inv[i * k + col]];
+                row[col] = v;
+            }
+        }
+        outs[nout++] = dst[d];
+    }
+    if (nout) {
+        const uint8_t **ins = malloc(sizeof(uint8_t *) * nvalid);
+        uint8_t *adj = malloc((size_t)nout * nvalid + 1);
+        int nin = 0;
+        int *cols = malloc(sizeof(int) * nvalid);
+        for (int col = 0; col < nvalid; col++)
+            if (input[valid[col]]) {
+                ins[nin] = input[valid[col]];
+                cols[nin++] = col;
+            }
+        for (int j = 0; j < nin; j++)  /* an empty input beside sized ones (a panic in the reference) */
+            if (ilens[valid[cols[j]]] != size) err = RSREF_ERR_INVALID_SHARD_SIZE;
+        if (nin && !err) {
+            for (int o = 0; o < nout; o++)
+                for (int j = 0; j < nin; j++) adj[o * nin + j] = rows[(size_t)o * nvalid + cols[j]];
+            rsref_code_some_shards(adj, nout, nin, ins, outs, size, 0);
+        }
+        free(ins);
+        free(adj);
+        free(cols);
+    }
+    free(rows);
+    free(outs);
+    free(valid);
+    free(inv);
+    return err;
+}

@@ -70,6 +70,8 @@ lib.rsx_verify.argtypes = [_P, _PP, _SZ, ctypes.c_int, ctypes.POINTER(ctypes.c_i
 lib.rsx_reconstruct.argtypes = [_P, _PP, _SZ, ctypes.c_int, _U8]
 lib.rsx_reconstruct_data.argtypes = [_P, _PP, _SZ, ctypes.c_int, _U8]
 lib.rsx_reconstruct_some.argtypes = [_P, _PP, _SZ, ctypes.c_int, _U8, ctypes.c_int, _U8]
+lib.rsx_update.argtypes = [_P, _PP, _SZ, ctypes.c_int, _PP, _SZ, ctypes.c_int]
+lib.rsx_decode_idx.argtypes = [_P, _PP, _SZ, ctypes.c_int, _U8, ctypes.c_int, _PP, _SZ, ctypes.c_int]
 lib.rsx_set_stream.argtypes = [_P, _P]
 lib.rsx_get_stream.argtypes = [_P]
 lib.rsx_get_stream.restype = _P
@@ -224,6 +226,29 @@ class _Marshal:
                 self.device = dev
 
 
+_EMPTY = (ctypes.c_uint8 * 1)()  # address for non-nil empty shards
+
+
+class _NilMarshal:
+    """Pointer + length arrays where None is Go's nil (a NULL pointer) and an
+    empty array is a non-nil empty slice."""
+
+    def __init__(self, shards):
+        n = len(shards)
+        self.n = n
+        self.ptrs = (ctypes.c_void_p * max(n, 1))()
+        self.lens = (ctypes.c_size_t * max(n, 1))()
+        self.device = None
+        for i, s in enumerate(shards):
+            if s is None:
+                continue
+            p, ln, dev = _shard_info(s)
+            self.ptrs[i] = p or ctypes.addressof(_EMPTY)
+            self.lens[i] = ln
+            if dev is not None and self.device is None:
+                self.device = dev
+
+
 class Encoder:
     """reedsolomon.Encoder + Extensions over the HIP library."""
 
@@ -326,6 +351,25 @@ class Encoder:
         for i in range(n):
             if filled[i]:
                 shards[i] = bufs[i]
+
+    def Update(self, shards, newDatashards) -> None:
+        """reedsolomon.go:762.  None = nil.  Parity is updated in place and each
+        changed old data shard becomes old ^ new, as in the reference."""
+        a, b = _NilMarshal(shards), _NilMarshal(newDatashards)
+        _check(lib.rsx_update(self._h, a.ptrs, a.lens, a.n, b.ptrs, b.lens, b.n))
+        if a.device or b.device:
+            self.Synchronize()
+
+    def DecodeIdx(self, dst, expectInput, input) -> None:
+        """reedsolomon.go:1464.  None = nil; expectInput None = merge mode."""
+        d, i = _NilMarshal(dst), _NilMarshal(input)
+        if expectInput is None:
+            _check(lib.rsx_decode_idx(self._h, d.ptrs, d.lens, d.n, None, 0, i.ptrs, i.lens, i.n))
+        else:
+            ex = (ctypes.c_uint8 * max(len(expectInput), 1))(*[1 if e else 0 for e in expectInput])
+            _check(lib.rsx_decode_idx(self._h, d.ptrs, d.lens, d.n, ex, len(expectInput), i.ptrs, i.lens, i.n))
+        if d.device or i.device:
+            self.Synchronize()
 
     def Reconstruct(self, shards) -> None:
         self._reconstruct(lib.rsx_reconstruct, shards)

@@ -164,6 +164,15 @@ int upload_tables(const std::vector<const uint8_t *> &rows, int nin, DeviceBuffe
     return RSX_OK;
 }
 
+int stage_slots(rsx_codec *c, size_t len, size_t nslots, std::vector<uint8_t *> &dev) {
+    const size_t pitch = align_up(len, kPitchAlign);
+    int err = c->staging.reserve(std::max<size_t>(pitch * nslots, 1));
+    if (err) return err;
+    dev.resize(nslots);
+    for (size_t i = 0; i < nslots; ++i) dev[i] = c->staging.as<uint8_t>() + pitch * i;
+    return RSX_OK;
+}
+
 // Stages host shards into HBM: returns device addresses per index in `idx`.
 // Caller holds c->mu.
 int stage_in(rsx_codec *c, size_t len, const std::vector<int> &load, const std::vector<int> &slots,

@@ -65,6 +65,18 @@ class DeviceRanges {
     size_t last_ = 0;
 };
 
+// Shared helpers of the classic engine (codec.cpp).
+size_t shard_size(const size_t *lens, int n);
+int check_shards(const size_t *lens, int n, bool nilok);
+// out = rows * in, or out ^= rows * in when accumulate; `tables` is [nin][nrows][5].
+int run_matmul(rsx_codec *c, const uint32_t *tables, const std::vector<const uint8_t *> &in,
+               const std::vector<uint8_t *> &out, size_t len, bool accumulate);
+// Coefficient tables of `rows` (nrows rows of nin bytes) into dst ([nin][nrows][5]).
+int upload_tables(const std::vector<const uint8_t *> &rows, int nin, DeviceBuffer &dst,
+                  std::vector<uint32_t> &host, hipStream_t stream);
+// Staging slot pointers of `len` bytes in the codec's HBM buffer (nslots slots).
+int stage_slots(rsx_codec *c, size_t len, size_t nslots, std::vector<uint8_t *> &dev);
+
 struct LeopardState;  // leopard.cpp
 struct LeopardDeleter {
     void operator()(LeopardState *s) const;
@@ -92,6 +104,8 @@ struct rsx_codec {
     rsx::DeviceBuffer scratch;  // Verify: recomputed parity
     rsx::DeviceBuffer flag;     // Verify: mismatch flag
     rsx::DeviceBuffer cmp;      // Verify: compare list
+    rsx::DeviceBuffer aux_tables;  // Update / DecodeIdx: per-call coefficient tables
+    rsx::DeviceBuffer one_table;   // the table of coefficient 1 (XOR passes)
 
     std::unique_ptr<rsx::LeopardState, rsx::LeopardDeleter> leo;
 };
