@@ -244,6 +244,23 @@ int rsx_server_run_trace(rsx_server *s, const rsx_request *reqs, const uint64_t 
 int rsx_server_stats(const rsx_server *s, uint64_t *batches, uint64_t *requests,
                      uint64_t *max_inflight_bytes);
 
+/* ---- LowLevel (mulslice.go, gf16.go) -------------------------------------
+ * Synchronous, like the Go functions.  Buffers may be device or host memory
+ * (host memory is staged through HBM). */
+/* LowLevel.GalMulSlice / GalMulSliceXor, mulslice.go:29-39:
+ * out[i] = c * in[i] (do_xor = 0) or out[i] ^= c * in[i], for i < len. */
+int rsx_gal_mul_slice(int device, uint8_t c, const uint8_t *in, uint8_t *out, size_t len, int do_xor);
+/* Inv, mulslice.go:42 (Inv(0) = 0). */
+uint8_t rsx_inv(uint8_t e);
+/* LowLevel.GF16Mul, gf16.go:6 (Leopard GF(2^16), polynomial 0x1002D). */
+uint16_t rsx_gf16_mul(uint16_t a, uint16_t b);
+/* LowLevel.GF16MulSliceXor (nout = 1) / GF16MulSliceXor8 (nout = 8),
+ * mulslice.go:45-95: outs[j] ^= scalars[j] * in over the Leopard split layout;
+ * zero scalars are skipped.  len must be a multiple of 64 and every output
+ * len bytes long: the reference panics otherwise, here ErrInvalidShardSize. */
+int rsx_gf16_mul_slice_xor(int device, const uint16_t *scalars, int nout, const uint8_t *in,
+                           uint8_t *const *outs, const size_t *out_lens, size_t len);
+
 /* Message of the last error on the calling thread. */
 const char *rsx_last_error(void);
 int rsx_device_count(void);

@@ -342,3 +342,21 @@ def leo16_errlocs(k: int, m: int, erased) -> np.ndarray:
     out = np.zeros(65536, np.uint16)
     lib().rsref_leo16_errlocs(k, m, _ptr(er), 1, _ptr(out))
     return out
+
+
+def gf16_mul_slice_xor(scalar: int, inp: np.ndarray, out: np.ndarray) -> None:
+    """mulslice.go:73-95 GF16MulSliceXor on the split layout (each 64-byte
+    chunk: 32 low bytes, then 32 high bytes): out ^= scalar * in, elementwise
+    mulLog(x, logLUT[scalar]) (leopard.go:1000-1006)."""
+    if len(inp) == 0 or scalar == 0:
+        return
+    t = leo16_tables()
+    exp, log = t["exp"].astype(np.int64), t["log"].astype(np.int64)
+    c = inp.reshape(-1, 2, 32)
+    x = c[:, 0, :].astype(np.int64) | (c[:, 1, :].astype(np.int64) << 8)
+    s = log[x] + int(log[scalar])
+    s = (s + (s >> 16)) & 0xFFFF  # addMod: partial reduction (expLUT[modulus] == expLUT[0])
+    prod = np.where(x == 0, 0, exp[s])
+    o = out.reshape(-1, 2, 32)
+    o[:, 0, :] ^= (prod & 0xFF).astype(np.uint8)
+    o[:, 1, :] ^= (prod >> 8).astype(np.uint8)

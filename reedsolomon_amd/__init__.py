@@ -72,6 +72,13 @@ lib.rsx_reconstruct_data.argtypes = [_P, _PP, _SZ, ctypes.c_int, _U8]
 lib.rsx_reconstruct_some.argtypes = [_P, _PP, _SZ, ctypes.c_int, _U8, ctypes.c_int, _U8]
 lib.rsx_update.argtypes = [_P, _PP, _SZ, ctypes.c_int, _PP, _SZ, ctypes.c_int]
 lib.rsx_decode_idx.argtypes = [_P, _PP, _SZ, ctypes.c_int, _U8, ctypes.c_int, _PP, _SZ, ctypes.c_int]
+lib.rsx_gal_mul_slice.argtypes = [ctypes.c_int, ctypes.c_uint8, _P, _P, ctypes.c_size_t, ctypes.c_int]
+lib.rsx_inv.argtypes = [ctypes.c_uint8]
+lib.rsx_inv.restype = ctypes.c_uint8
+lib.rsx_gf16_mul.argtypes = [ctypes.c_uint16, ctypes.c_uint16]
+lib.rsx_gf16_mul.restype = ctypes.c_uint16
+lib.rsx_gf16_mul_slice_xor.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_uint16), ctypes.c_int, _P, _PP, _SZ,
+                                       ctypes.c_size_t]
 lib.rsx_set_stream.argtypes = [_P, _P]
 lib.rsx_get_stream.argtypes = [_P]
 lib.rsx_get_stream.restype = _P
@@ -620,6 +627,54 @@ class Server:
         return a.value, b.value, c.value
 
 
+
+# --- LowLevel (mulslice.go, gf16.go) ---
+def Inv(e: int) -> int:
+    """mulslice.go:42: multiplicative inverse in GF(2^8) (Inv(0) = 0)."""
+    return lib.rsx_inv(e)
+
+
+class LowLevel:
+    """mulslice.go LowLevel on the GPU (`device`); slices are numpy arrays or
+    torch.uint8 CUDA tensors.  Length errors the reference panics on raise."""
+
+    def __init__(self, device: int = 0):
+        self.device = device
+
+    def WithOptions(self, *opts) -> None:
+        pass
+
+    def GalMulSlice(self, c: int, inp, out) -> None:
+        p, n, _ = _shard_info(inp)
+        q, nq, _ = _shard_info(out)
+        if nq < n:
+            raise ErrInvalidShardSize("out must be at least as long as in")
+        _check(lib.rsx_gal_mul_slice(self.device, c, p or None, q or None, n, 0))
+
+    def GalMulSliceXor(self, c: int, inp, out) -> None:
+        p, n, _ = _shard_info(inp)
+        q, nq, _ = _shard_info(out)
+        if nq < n:
+            raise ErrInvalidShardSize("out must be at least as long as in")
+        _check(lib.rsx_gal_mul_slice(self.device, c, p or None, q or None, n, 1))
+
+    def GF16Mul(self, a: int, b: int) -> int:
+        return lib.rsx_gf16_mul(a, b)
+
+    def _gf16(self, scalars, inp, outs):
+        p, n, _ = _shard_info(inp)
+        mm = _Marshal(outs)
+        sc = (ctypes.c_uint16 * max(len(scalars), 1))(*scalars)
+        _check(lib.rsx_gf16_mul_slice_xor(self.device, sc, len(scalars), p or None, mm.ptrs, mm.lens, n))
+
+    def GF16MulSliceXor(self, scalar: int, inp, out) -> None:
+        self._gf16([scalar], inp, [out])
+
+    def GF16MulSliceXor8(self, scalars, inp, outs) -> None:
+        assert len(scalars) == 8 and len(outs) == 8
+        self._gf16(list(scalars), inp, list(outs))
+
+
 __all__ = [n for n in dir() if n.startswith(("With", "Err", "New", "Encoder", "RSError"))] + [
-    "device_count", "lib", "LIB_PATH", "Batch", "Server", "OP_ENCODE", "OP_RECONSTRUCT",
+    "device_count", "lib", "LIB_PATH", "Batch", "Server", "LowLevel", "Inv", "OP_ENCODE", "OP_RECONSTRUCT",
     "OP_RECONSTRUCT_DATA", "OP_VERIFY"]

@@ -135,6 +135,20 @@ __global__ __launch_bounds__(kBlock) void k_derivative(const LeoWork in, const L
     }
 }
 
+// One 16-element unit of `in` times each scalar, XORed into each output.
+__global__ __launch_bounds__(kBlock) void k_gf16_mul_xor(const LeoDev f, const Gf16MulXor a) {
+    const size_t units = a.len / F16::kUnitBytes;
+    for (size_t u = static_cast<size_t>(blockIdx.x) * kBlock + threadIdx.x; u < units;
+         u += static_cast<size_t>(gridDim.x) * kBlock) {
+        const F16::E y = F16::load(a.in, static_cast<int>(u));
+        for (int j = 0; j < a.nout; ++j) {
+            F16::E x = F16::load(a.out[j], static_cast<int>(u));
+            F16::mul_add(x, y, table<F16>(f, a.logs[j]));
+            F16::store(a.out[j], static_cast<int>(u), x);
+        }
+    }
+}
+
 // Lanes over column units in x; rows (or butterfly units) in y, grid-strided
 // past 32768 so every launch stays within the y-dimension limit.
 dim3 grid2(size_t units, int rows) {
@@ -143,6 +157,14 @@ dim3 grid2(size_t units, int rows) {
 }
 
 }  // namespace
+
+hipError_t leo_gf16_mul_xor(const LeoDev &f, const Gf16MulXor &a, hipStream_t s) {
+    const size_t units = a.len / F16::kUnitBytes;
+    if (units == 0 || a.nout <= 0) return hipSuccess;
+    const size_t blocks = std::min<size_t>((units + kBlock - 1) / kBlock, 4096);
+    hipLaunchKernelGGL(k_gf16_mul_xor, dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, s, f, a);
+    return hipGetLastError();
+}
 
 hipError_t leo_load_rows(const LeoDev &f, const uint8_t *const *ptrs, const uint16_t *logs,
                          size_t col_off, int nrows, const LeoWork &w, hipStream_t s) {

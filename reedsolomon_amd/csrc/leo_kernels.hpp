@@ -56,6 +56,17 @@ hipError_t leo_derivative(const LeoWork &in, const LeoWork &out, int n, hipStrea
 hipError_t leo_store_rows(const LeoDev &f, const LeoWork &w, const int *rows, const uint16_t *logs,
                           uint8_t *const *dst, size_t col_off, int count, hipStream_t s);
 
+// out[j] ^= exp(logs[j]) * in over the GF(2^16) split layout (LowLevel
+// GF16MulSliceXor[8]); len a multiple of 64, pointers 16-byte aligned.
+struct Gf16MulXor {
+    const uint8_t *in;
+    uint8_t *out[8];
+    uint16_t logs[8];
+    int nout;
+    size_t len;
+};
+hipError_t leo_gf16_mul_xor(const LeoDev &f, const Gf16MulXor &a, hipStream_t s);
+
 // ---- LDS-resident transforms (leo_fused.hip) ----
 // Encode keeps the accumulator and one group's rows of a column slice in LDS
 // (2 regions of m rows, 1 when k <= m); reconstruct keeps all n rows.  A

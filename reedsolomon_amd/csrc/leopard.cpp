@@ -80,6 +80,10 @@ void LeopardDeleter::operator()(LeopardState *s) const { delete s; }
 
 bool leopard8_shape_ok(int k, int m) { return leopard_shape_ok(8, k, m); }
 
+int leo_device_tables(int device, int bits, hipStream_t stream, const LeoDev **out) {
+    return device_tables(device, bits, stream, out);
+}
+
 // Largest fftSkew index the encoder's grouped IFFT reads (leopard.go:764-873
 // on skewLUT = fftSkew[m-1+g*m:]).
 static long max_encode_skew_index(int k, int p) {

@@ -16,6 +16,8 @@ bool leopard8_shape_ok(int k, int m);
 // Shapes inside the tables of the reference (see leopard.cpp).
 bool leopard_shape_ok(int bits, int k, int m);
 int leopard_init(rsx_codec *c);
+// Field tables of `bits` (8 or 16) resident on `device` (uploaded on first use).
+int leo_device_tables(int device, int bits, hipStream_t stream, const LeoDev **out);
 int leopard_encode(rsx_codec *c, uint8_t *const *shards, const size_t *lens, int nshards);
 int leopard_verify(rsx_codec *c, uint8_t *const *shards, const size_t *lens, int nshards, int *ok);
 int leopard_reconstruct(rsx_codec *c, uint8_t *const *shards, const size_t *lens, int nshards,
