@@ -236,6 +236,19 @@ class _Marshal:
 _EMPTY = (ctypes.c_uint8 * 1)()  # address for non-nil empty shards
 
 
+def _zeros_like(x, n):
+    if _is_torch(x):
+        import torch
+        return torch.zeros(n, dtype=torch.uint8, device=x.device)
+    return np.zeros(n, np.uint8)
+
+
+def _host_bytes(x) -> bytes:
+    if _is_torch(x):
+        return x.cpu().numpy().tobytes()
+    return np.asarray(x, np.uint8).tobytes()
+
+
 class _NilMarshal:
     """Pointer + length arrays where None is Go's nil (a NULL pointer) and an
     empty array is a non-nil empty slice."""
@@ -358,6 +371,58 @@ class Encoder:
         for i in range(n):
             if filled[i]:
                 shards[i] = bufs[i]
+
+    # Split / Join (reedsolomon.go:1803-1910, leopard.go:276-314, leopard8.go:293-350)
+    # are data-format helpers with no device work: the shard layout is the
+    # reference's (per-shard size rounded up to ShardSizeMultiple, data
+    # shards cut from the input, zero padding, zeroed parity).
+    def Split(self, data):
+        """Shards of `data` (numpy array or torch tensor): full shards are
+        views of `data`, the rest new zero-padded buffers."""
+        n = len(data)
+        if n == 0:
+            raise ErrShortData()
+        total, k = self.TotalShards(), self._k
+        mult = self.ShardSizeMultiple()
+        if total == 1 and n % mult == 0:
+            return [data]
+        per = (n + k - 1) // k
+        per = (per + mult - 1) // mult * mult
+        full = n // per
+        out = [data[i * per:(i + 1) * per] for i in range(min(full, total))]
+        rest = data[full * per:]
+        for _ in range(total - len(out)):
+            pad = _zeros_like(data, per)
+            m = min(len(rest), per)
+            if m:
+                pad[:m] = rest[:m]
+                rest = rest[m:]
+            out.append(pad)
+        return out
+
+    def Join(self, dst, shards, outSize: int) -> None:
+        """Writes the first outSize bytes of the data shards to dst (a file-like
+        object with write())."""
+        if len(shards) < self._k:
+            raise ErrTooFewShards()
+        shards = shards[:self._k]
+        size = 0
+        for sh in shards:
+            if sh is None:
+                raise ErrReconstructRequired()
+            size += len(sh)
+            if size >= outSize:
+                break
+        if size < outSize:
+            raise ErrShortData()
+        write = outSize
+        for sh in shards:
+            b = _host_bytes(sh)
+            if write < len(b):
+                dst.write(b[:write])
+                return
+            dst.write(b)
+            write -= len(b)
 
     def Update(self, shards, newDatashards) -> None:
         """reedsolomon.go:762.  None = nil.  Parity is updated in place and each
