@@ -43,6 +43,9 @@ extern "C" {
 #define RSX_ERR_RECONSTRUCT_REQUIRED -10 /* ErrReconstructRequired reedsolomon.go:1862 */
 #define RSX_ERR_SINGULAR -11            /* errSingular          matrix.go:184 */
 #define RSX_ERR_MATRIX -12              /* custom-matrix shape errors of New, reedsolomon.go:466-474 */
+#define RSX_ERR_RECONSTRUCT_MISMATCH -14 /* ErrReconstructMismatch streaming.go:461 */
+#define RSX_ERR_STREAM_READ -15         /* StreamReadError  streaming.go:98 (stream index reported) */
+#define RSX_ERR_STREAM_WRITE -16        /* StreamWriteError streaming.go:116 (stream index reported) */
 #define RSX_ERR_DEVICE -20              /* HIP runtime error (no reference counterpart) */
 #define RSX_ERR_NOMEM -21               /* device allocation failed (no reference counterpart) */
 #define RSX_ERR_MIXED_MEMORY -22        /* host and device shards in one call (no counterpart) */
@@ -243,6 +246,34 @@ int rsx_server_run_trace(rsx_server *s, const rsx_request *reqs, const uint64_t 
 /* Counters: batched launches, requests in them, highest shard bytes in flight. */
 int rsx_server_stats(const rsx_server *s, uint64_t *batches, uint64_t *requests,
                      uint64_t *max_inflight_bytes);
+
+/* ---- Streaming (streaming.go) ---------------------------------------------
+ * StreamEncoder of NewStream (streaming.go:147): shards are read and written
+ * through callbacks, block by block (block_size, 0 = 4 MiB), with reads of
+ * the next block overlapping the GPU work on the current one.
+ * A read callback fills buf with up to n bytes of shard `shard` and returns
+ * the count: fewer than n only at the end of the stream (io.ReadFull), < 0
+ * on error.  A write callback returns the bytes written (< n or < 0 is an
+ * error).  On RSX_ERR_STREAM_READ / _WRITE, *err_stream is the stream index
+ * (StreamReadError / StreamWriteError.Stream).  Classic engine only. */
+typedef long long (*rsx_stream_read_fn)(void *ctx, int shard, uint8_t *buf, size_t n);
+typedef long long (*rsx_stream_write_fn)(void *ctx, int shard, const uint8_t *buf, size_t n);
+typedef struct rsx_stream rsx_stream;
+
+int rsx_stream_create(int data_shards, int parity_shards, const rsx_options *opt, size_t block_size,
+                      rsx_stream **out);
+void rsx_stream_free(rsx_stream *s);
+rsx_codec *rsx_stream_codec(rsx_stream *s);
+/* rsStream.Encode, streaming.go:199: reads shards 0..k-1, writes parity 0..m-1. */
+int rsx_stream_encode(rsx_stream *s, rsx_stream_read_fn rd, void *rctx, rsx_stream_write_fn wr, void *wctx,
+                      int *err_stream);
+/* rsStream.Verify, streaming.go:430: reads all TotalShards streams. */
+int rsx_stream_verify(rsx_stream *s, rsx_stream_read_fn rd, void *rctx, int *ok, int *err_stream);
+/* rsStream.Reconstruct, streaming.go:477: valid[i] = stream i is read, fill[i] =
+ * stream i is written (TotalShards flags each; both set is
+ * ErrReconstructMismatch). */
+int rsx_stream_reconstruct(rsx_stream *s, const uint8_t *valid, const uint8_t *fill, rsx_stream_read_fn rd,
+                           void *rctx, rsx_stream_write_fn wr, void *wctx, int *err_stream);
 
 /* ---- LowLevel (mulslice.go, gf16.go) -------------------------------------
  * Synchronous, like the Go functions.  Buffers may be device or host memory

@@ -79,6 +79,14 @@ lib.rsx_gf16_mul.argtypes = [ctypes.c_uint16, ctypes.c_uint16]
 lib.rsx_gf16_mul.restype = ctypes.c_uint16
 lib.rsx_gf16_mul_slice_xor.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_uint16), ctypes.c_int, _P, _PP, _SZ,
                                        ctypes.c_size_t]
+_READ_FN = ctypes.CFUNCTYPE(ctypes.c_longlong, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t)
+_WRITE_FN = ctypes.CFUNCTYPE(ctypes.c_longlong, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t)
+lib.rsx_stream_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t, ctypes.POINTER(_P)]
+lib.rsx_stream_free.argtypes = [_P]
+lib.rsx_stream_free.restype = None
+lib.rsx_stream_encode.argtypes = [_P, _READ_FN, _P, _WRITE_FN, _P, ctypes.POINTER(ctypes.c_int)]
+lib.rsx_stream_verify.argtypes = [_P, _READ_FN, _P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+lib.rsx_stream_reconstruct.argtypes = [_P, _U8, _U8, _READ_FN, _P, _WRITE_FN, _P, ctypes.POINTER(ctypes.c_int)]
 lib.rsx_set_stream.argtypes = [_P, _P]
 lib.rsx_get_stream.argtypes = [_P]
 lib.rsx_get_stream.restype = _P
@@ -108,6 +116,7 @@ _ERRORS = {
     -8: _err("ErrInvalidShardSize", -8, "invalid shard size"),
     -9: _err("ErrShortData", -9, "not enough data to fill the number of requested shards"),
     -10: _err("ErrReconstructRequired", -10, "reconstruction required"),
+    -14: _err("ErrReconstructMismatch", -14, "valid shards and fill shards are mutually exclusive"),
     -11: _err("ErrSingular", -11, "matrix is singular"),
     -12: _err("ErrMatrix", -12, "invalid custom coding matrix"),
     -20: _err("ErrDevice", -20, "HIP device error"),
@@ -740,6 +749,169 @@ class LowLevel:
         self._gf16(list(scalars), inp, list(outs))
 
 
+
+# --- StreamEncoder (streaming.go) ---
+class StreamReadError(RSError):
+    """streaming.go:98: a read from stream `stream` failed (`err`)."""
+    code = -15
+
+    def __init__(self, stream: int, err=None):
+        super().__init__(f"error reading stream {stream}: {err}")
+        self.stream, self.err = stream, err
+
+
+class StreamWriteError(RSError):
+    """streaming.go:116: a write to stream `stream` failed (`err`)."""
+    code = -16
+
+    def __init__(self, stream: int, err=None):
+        super().__init__(f"error writing stream {stream}: {err}")
+        self.stream, self.err = stream, err
+
+
+class _StreamIO:
+    """ctypes callbacks over Python file objects (read(n) / write(bytes))."""
+
+    def __init__(self, readers=None, writers=None):
+        self.readers, self.writers = readers or [], writers or []
+        self.error = None
+        self.rd = _READ_FN(self._read)
+        self.wr = _WRITE_FN(self._write)
+
+    def _read(self, ctx, i, buf, n):
+        try:
+            out, got = bytearray(), 0
+            while got < n:  # io.ReadFull
+                chunk = self.readers[i].read(n - got)
+                if not chunk:
+                    break
+                out += chunk
+                got += len(chunk)
+            ctypes.memmove(buf, bytes(out), got)
+            return got
+        except Exception as e:  # reported as StreamReadError
+            self.error = e
+            return -1
+
+    def _write(self, ctx, i, buf, n):
+        try:
+            w = self.writers[i].write(ctypes.string_at(buf, n))
+            return n if w is None else int(w)
+        except Exception as e:
+            self.error = e
+            return -1
+
+
+def _stream_check(rc, io_, err_stream):
+    if rc == -15:
+        raise StreamReadError(err_stream.value, io_.error)
+    if rc == -16:
+        raise StreamWriteError(err_stream.value, io_.error or "short write")
+    _check(rc)
+
+
+class StreamEncoder:
+    """streaming.go StreamEncoder: shards are file-like objects (read(n) /
+    write(b)); data moves block by block through pinned host buffers and the
+    GPU, reads of the next block overlapping the GPU work."""
+
+    def __init__(self, handle, k, m):
+        self._h, self._k, self._m = handle, k, m
+
+    def __del__(self, _free=lib.rsx_stream_free):
+        h = getattr(self, "_h", None)
+        if h:
+            _free(h)
+            self._h = None
+
+    def Encode(self, data, parity) -> None:
+        if len(data) != self._k or len(parity) != self._m:
+            raise ErrTooFewShards()
+        io_ = _StreamIO(data, parity)
+        es = ctypes.c_int(-1)
+        _stream_check(lib.rsx_stream_encode(self._h, io_.rd, None, io_.wr, None, ctypes.byref(es)), io_, es)
+
+    def Verify(self, shards) -> bool:
+        if len(shards) != self._k + self._m:
+            raise ErrTooFewShards()
+        io_ = _StreamIO(shards)
+        ok, es = ctypes.c_int(0), ctypes.c_int(-1)
+        _stream_check(lib.rsx_stream_verify(self._h, io_.rd, None, ctypes.byref(ok), ctypes.byref(es)), io_, es)
+        return bool(ok.value)
+
+    def Reconstruct(self, valid, fill) -> None:
+        n = self._k + self._m
+        if len(valid) != n or len(fill) != n:
+            raise ErrTooFewShards()
+        io_ = _StreamIO(valid, fill)
+        v = (ctypes.c_uint8 * n)(*[0 if x is None else 1 for x in valid])
+        f = (ctypes.c_uint8 * n)(*[0 if x is None else 1 for x in fill])
+        es = ctypes.c_int(-1)
+        _stream_check(lib.rsx_stream_reconstruct(self._h, v, f, io_.rd, None, io_.wr, None, ctypes.byref(es)),
+                      io_, es)
+
+    def Split(self, data, dst, size: int) -> None:
+        """streaming.go:556-594 (host-only data movement): the data stream,
+        then zero padding to TotalShards * perShard, copied perShard bytes per
+        data shard writer."""
+        if size == 0:
+            raise ErrShortData()
+        if len(dst) != self._k:
+            raise ErrInvShardNum()
+        for i, d in enumerate(dst):
+            if d is None:
+                raise StreamWriteError(i, ErrShardNoData())
+        per = (size + self._k - 1) // self._k
+        pad = (self._k + self._m) * per - size
+        eof = False
+        for d in dst:
+            need = per
+            while need:
+                chunk = b"" if eof else data.read(need)
+                if chunk:
+                    d.write(chunk)
+                    need -= len(chunk)
+                    continue
+                eof = True
+                if pad == 0:
+                    raise ErrShortData()
+                take = min(need, pad)
+                d.write(bytes(take))
+                pad -= take
+                need -= take
+
+    def Join(self, dst, shards, outSize: int) -> None:
+        """streaming.go:529-554 (host-only data movement)."""
+        if len(shards) < self._k:
+            raise ErrTooFewShards()
+        shards = shards[:self._k]
+        for i, sh in enumerate(shards):
+            if sh is None:
+                raise StreamReadError(i, ErrShardNoData())
+        left = outSize
+        for sh in shards:
+            while left > 0:
+                chunk = sh.read(min(left, 1 << 20))
+                if not chunk:
+                    break
+                dst.write(chunk)
+                left -= len(chunk)
+        if left:
+            raise ErrShortData()
+
+
+def NewStream(data_shards: int, parity_shards: int, *opts, block_size: int = 0) -> StreamEncoder:
+    """streaming.go:147 NewStream (classic engine; block_size 0 = 4 MiB)."""
+    o = _Options()
+    lib.rsx_options_default(ctypes.byref(o))
+    for f in opts:
+        f(o)
+    h = ctypes.c_void_p()
+    _check(lib.rsx_stream_create(data_shards, parity_shards, ctypes.byref(o), block_size, ctypes.byref(h)))
+    return StreamEncoder(h.value, data_shards, parity_shards)
+
+
 __all__ = [n for n in dir() if n.startswith(("With", "Err", "New", "Encoder", "RSError"))] + [
-    "device_count", "lib", "LIB_PATH", "Batch", "Server", "LowLevel", "Inv", "OP_ENCODE", "OP_RECONSTRUCT",
+    "device_count", "lib", "LIB_PATH", "Batch", "Server", "LowLevel", "Inv", "NewStream", "StreamEncoder",
+    "StreamReadError", "StreamWriteError", "OP_ENCODE", "OP_RECONSTRUCT",
     "OP_RECONSTRUCT_DATA", "OP_VERIFY"]
