@@ -102,6 +102,7 @@ class Stripes:
             self.buf[o + k * s:o + n * s].zero_()
             self.ptrs.append((ctypes.c_void_p * n)(*[base + o + i * s for i in range(n)]))
             self.full.append((ctypes.c_size_t * n)(*([s] * n)))
+        torch.cuda.synchronize()  # the library's streams read these bytes next
 
     def shard(self, i, j):
         (k, m), s, o = self.shapes[i], self.sizes[i], self.offs[i]
@@ -421,6 +422,7 @@ def leopard(args, torch, rs):
     buf = torch.empty((n, S), dtype=torch.uint8, device="cuda")
     buf[:k].random_(0, 256, generator=gen)
     buf[k:].zero_()
+    torch.cuda.synchronize()
     enc = rs.New(k, m, rs.WithLeopardGF16(True))
     h = enc.handle
     base = buf.data_ptr()

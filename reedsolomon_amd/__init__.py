@@ -245,6 +245,25 @@ class _Marshal:
 _EMPTY = (ctypes.c_uint8 * 1)()  # address for non-nil empty shards
 
 
+def _after_torch(stream_ptr) -> None:
+    """Orders work on the library stream `stream_ptr` after everything queued
+    so far on this thread's current torch stream (the producer of the device
+    tensors passed in), without blocking the host."""
+    import torch
+    cur = torch.cuda.current_stream()
+    if cur.cuda_stream != stream_ptr:
+        torch.cuda.ExternalStream(stream_ptr).wait_stream(cur)
+
+
+def _torch_sync() -> None:
+    """Waits for this thread's current torch stream when torch has a GPU
+    (device tensors handed to the library's own streams must be complete)."""
+    import sys
+    torch = sys.modules.get("torch")
+    if torch is not None and torch.cuda.is_available():
+        torch.cuda.current_stream().synchronize()
+
+
 def _zeros_like(x, n):
     if _is_torch(x):
         import torch
@@ -318,6 +337,10 @@ class Encoder:
     def Synchronize(self) -> None:
         _check(lib.rsx_synchronize(self._h))
 
+    def _before(self, device) -> None:
+        if device:
+            _after_torch(lib.rsx_get_stream(self._h))
+
     def DecodeCacheSize(self) -> int:
         return lib.rsx_decode_cache_size(self._h)
 
@@ -328,6 +351,7 @@ class Encoder:
     # Encoder (reedsolomon.go:25-133)
     def Encode(self, shards) -> None:
         mm = _Marshal(shards)
+        self._before(mm.device)
         _check(lib.rsx_encode(self._h, mm.ptrs, mm.lens, mm.n))
         if mm.device:
             self.Synchronize()
@@ -335,12 +359,14 @@ class Encoder:
     def EncodeIdx(self, data_shard, idx: int, parity) -> None:
         mm = _Marshal(parity)
         p, ln, dev = _shard_info(data_shard)
+        self._before(mm.device or dev)
         _check(lib.rsx_encode_idx(self._h, p or None, ln, idx, mm.ptrs, mm.lens, mm.n))
         if mm.device or dev:
             self.Synchronize()
 
     def Verify(self, shards) -> bool:
         mm = _Marshal(shards)
+        self._before(mm.device)
         ok = ctypes.c_int(0)
         _check(lib.rsx_verify(self._h, mm.ptrs, mm.lens, mm.n, ctypes.byref(ok)))
         return bool(ok.value)
@@ -374,6 +400,7 @@ class Encoder:
             if _shard_info(s)[1] == 0:
                 mm.lens[i] = 0
         filled = (ctypes.c_uint8 * max(n, 1))()
+        self._before(dev)
         _check(fn(self._h, mm.ptrs, mm.lens, mm.n, *extra, filled))
         if dev:
             self.Synchronize()
@@ -437,6 +464,7 @@ class Encoder:
         """reedsolomon.go:762.  None = nil.  Parity is updated in place and each
         changed old data shard becomes old ^ new, as in the reference."""
         a, b = _NilMarshal(shards), _NilMarshal(newDatashards)
+        self._before(a.device or b.device)
         _check(lib.rsx_update(self._h, a.ptrs, a.lens, a.n, b.ptrs, b.lens, b.n))
         if a.device or b.device:
             self.Synchronize()
@@ -444,6 +472,7 @@ class Encoder:
     def DecodeIdx(self, dst, expectInput, input) -> None:
         """reedsolomon.go:1464.  None = nil; expectInput None = merge mode."""
         d, i = _NilMarshal(dst), _NilMarshal(input)
+        self._before(d.device or i.device)
         if expectInput is None:
             _check(lib.rsx_decode_idx(self._h, d.ptrs, d.lens, d.n, None, 0, i.ptrs, i.lens, i.n))
         else:
@@ -591,7 +620,10 @@ class Batch:
         return bool(ok.value)
 
     def launch(self, stream=None) -> None:
+        """Replays the plan on the batch's stream (ordered after this thread's
+        current torch stream) or on `stream` (the caller orders it)."""
         if stream is None:
+            _after_torch(self.stream)
             _check(lib.rsx_batch_launch(self._h))
         else:
             _check(lib.rsx_batch_launch_on(self._h, stream))
@@ -654,6 +686,7 @@ class Server:
         self.close()
 
     def submit(self, enc, op, shards) -> int:
+        _torch_sync()  # the request loop's streams read the tensors later
         req = _Request()
         mm, bufs = _marshal_item(enc, op, shards, req)
         t = ctypes.c_uint64()
@@ -675,6 +708,7 @@ class Server:
         the start and waits for all.  Returns (status, verify_ok, latency_ns,
         makespan_ns); latency runs from the scheduled arrival."""
         n = len(requests)
+        _torch_sync()
         arr = (_Request * max(n, 1))()
         keep = []
         for i, (enc, op, shards) in enumerate(requests):
@@ -719,6 +753,7 @@ class LowLevel:
         pass
 
     def GalMulSlice(self, c: int, inp, out) -> None:
+        _torch_sync()
         p, n, _ = _shard_info(inp)
         q, nq, _ = _shard_info(out)
         if nq < n:
@@ -726,6 +761,7 @@ class LowLevel:
         _check(lib.rsx_gal_mul_slice(self.device, c, p or None, q or None, n, 0))
 
     def GalMulSliceXor(self, c: int, inp, out) -> None:
+        _torch_sync()
         p, n, _ = _shard_info(inp)
         q, nq, _ = _shard_info(out)
         if nq < n:
@@ -736,6 +772,7 @@ class LowLevel:
         return lib.rsx_gf16_mul(a, b)
 
     def _gf16(self, scalars, inp, outs):
+        _torch_sync()
         p, n, _ = _shard_info(inp)
         mm = _Marshal(outs)
         sc = (ctypes.c_uint16 * max(len(scalars), 1))(*scalars)
@@ -780,14 +817,22 @@ class _StreamIO:
 
     def _read(self, ctx, i, buf, n):
         try:
-            out, got = bytearray(), 0
-            while got < n:  # io.ReadFull
-                chunk = self.readers[i].read(n - got)
-                if not chunk:
-                    break
-                out += chunk
-                got += len(chunk)
-            ctypes.memmove(buf, bytes(out), got)
+            view = memoryview((ctypes.c_char * n).from_address(buf)).cast("B")
+            r = self.readers[i]
+            got = 0
+            readinto = getattr(r, "readinto", None)
+            while got < n:  # io.ReadFull, straight into the pinned block
+                if readinto is not None:
+                    k = readinto(view[got:])
+                    if not k:
+                        break
+                else:
+                    chunk = r.read(n - got)
+                    if not chunk:
+                        break
+                    k = len(chunk)
+                    view[got:got + k] = chunk
+                got += k
             return got
         except Exception as e:  # reported as StreamReadError
             self.error = e
@@ -795,7 +840,7 @@ class _StreamIO:
 
     def _write(self, ctx, i, buf, n):
         try:
-            w = self.writers[i].write(ctypes.string_at(buf, n))
+            w = self.writers[i].write(memoryview((ctypes.c_char * n).from_address(buf)).cast("B"))
             return n if w is None else int(w)
         except Exception as e:
             self.error = e

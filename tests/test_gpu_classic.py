@@ -305,3 +305,44 @@ def test_long_shards_stream_path(k, m):
     enc.Reconstruct(work)
     for i in lost:
         assert torch.equal(work[i], orig[i]), i
+
+
+def test_one_codec_shared_by_threads():
+    # An Encoder is safe for concurrent use (reedsolomon_test.go:2158-2196):
+    # several threads encode and reconstruct different stripes with one codec.
+    import threading
+    k, m, size = 10, 4, 1 << 16
+    enc = rs.New(k, m)
+    o = rsref.Classic(k, m)
+    rng = np.random.default_rng(77)
+    stripes = []
+    for _ in range(8):
+        data = [rng.integers(0, 256, size, dtype=np.uint8) for _ in range(k)]
+        ref = [d.copy() for d in data] + [np.zeros(size, np.uint8) for _ in range(m)]
+        o.encode(ref)
+        stripes.append(ref)
+    errors = []
+
+    def work(t):
+        try:
+            for rep in range(5):
+                ref = stripes[(t + rep) % len(stripes)]
+                shards = [torch.from_numpy(x.copy()).to("cuda") for x in ref[:k]] + [
+                    torch.zeros(size, dtype=torch.uint8, device="cuda") for _ in range(m)]
+                enc.Encode(shards)
+                lost = [(t + rep) % (k + m), (t + rep + 3) % (k + m)]
+                for i in lost:
+                    shards[i] = None
+                enc.Reconstruct(shards)
+                for i in range(k + m):
+                    if not np.array_equal(shards[i].cpu().numpy(), ref[i]):
+                        errors.append((t, rep, i))
+        except Exception as e:  # surfaced below
+            errors.append(repr(e))
+
+    threads = [threading.Thread(target=work, args=(t,)) for t in range(6)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    assert not errors, errors[:5]
