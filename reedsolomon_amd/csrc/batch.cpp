@@ -71,6 +71,7 @@ struct rsx_batch {
     std::vector<rsx::LeoBatchTask> leo_tasks[2];
     DeviceBuffer d_leo_tasks[2], d_leo_aux;
     size_t leo_lds[2] = {0, 0};
+    size_t leo_deriv[2] = {0, 0};  // largest n * slice of the reconstruct tasks
     uint32_t leo_slices[2] = {0, 0};
 };
 
@@ -150,7 +151,7 @@ int enqueue(rsx_batch *b, hipStream_t s) {
         const unsigned g = std::min<unsigned>(static_cast<unsigned>(std::max(1, b->cus * 8)), b->leo_slices[fi]);
         RSX_HIP(leo_batch_launch(*b->leo[fi][0].dev, b->d_leo_tasks[fi].as<LeoBatchTask>(),
                                  static_cast<int>(b->leo_tasks[fi].size()), b->leo_slices[fi], b->leo_lds[fi], g,
-                                 ls));
+                                 b->leo_deriv[fi], ls));
     }
     if (fork) RSX_HIP(hipEventRecord(b->join, b->side));
     for (size_t i = 0; i < b->launches.size(); ++i) {
@@ -297,6 +298,7 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
         b->leo[fi].clear();
         b->leo_tasks[fi].clear();
         b->leo_lds[fi] = 0;
+        b->leo_deriv[fi] = 0;
         b->leo_slices[fi] = 0;
     }
     std::vector<uint8_t> laux;  // Leopard error locators / output rows / logs
@@ -570,6 +572,8 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
                 return fail(RSX_ERR_INVALID_INPUT, "batch too large for one plan");
             slice0 += lp.slices;
             b->leo_lds[fi] = std::max(b->leo_lds[fi], lp.lds);
+            if (t.op == 1)
+                b->leo_deriv[fi] = std::max(b->leo_deriv[fi], static_cast<size_t>(t.dec.n) * t.dec.slice);
             b->leo_tasks[fi].push_back(t);
         }
         b->leo_slices[fi] = slice0;

@@ -36,15 +36,16 @@ __device__ __forceinline__ void encode_slice(const LeoDev &f, const LeoEncodeArg
     __syncthreads();  // the next slice reuses the rows
 }
 
-// One column slice of a reconstruct.
-template <class F>
+// One column slice of a reconstruct; MAXD bounds the slots per thread the
+// formal derivative holds in registers (n * slice / 16 / threads).
+template <class F, int MAXD>
 __device__ __forceinline__ void decode_slice(const LeoDev &f, const LeoDecodeArgs &a, uint8_t *lds, size_t col) {
     const int ushift = log2i(a.slice / F::kUnitBytes);
     const int cols = static_cast<int>(a.size - col < static_cast<size_t>(a.slice) ? a.size - col : a.slice);
     const Tile w{lds, a.n + 1, ushift, cols / F::kUnitBytes};
     load_rows<F>(w, a.rows, a.errlocs, a.n, col, f);
     ifft<F>(w, a.n, a.ifft_trunc, f.skew, -1, f);
-    derivative(w, a.n, cols / 16);
+    derivative<MAXD>(w, a.n, cols / 16);
     fft<F>(w, a.n, a.fft_trunc, f);
     store_rows<F>(w, a.out, a.out_rows, a.out_logs, a.nout, col, f);
     __syncthreads();
@@ -58,18 +59,18 @@ __global__ __launch_bounds__(kFBMax) void k_encode_fused(const LeoDev f, const L
         encode_slice<F>(f, a, reinterpret_cast<uint8_t *>(lds_raw), col);
 }
 
-template <class F>
+template <class F, int MAXD>
 __global__ __launch_bounds__(kFBMax) void k_decode_fused(const LeoDev f, const LeoDecodeArgs a) {
     extern __shared__ uint4 lds_raw[];
     for (size_t col = static_cast<size_t>(blockIdx.x) * a.slice; col < a.size;
          col += static_cast<size_t>(gridDim.x) * a.slice)
-        decode_slice<F>(f, a, reinterpret_cast<uint8_t *>(lds_raw), col);
+        decode_slice<F, MAXD>(f, a, reinterpret_cast<uint8_t *>(lds_raw), col);
 }
 
 // Batched: blocks stride over the global slice index of all tasks, find
 // their task by binary search over the first-slice offsets, and run that
 // task's slice.  The task is uniform per block.
-template <class F>
+template <class F, int MAXD>
 __global__ __launch_bounds__(kFB) void k_leo_batch(const LeoDev f, const LeoBatchTask *tasks, int ntasks,
                                                    uint32_t nslices) {
     extern __shared__ uint4 lds_raw[];
@@ -86,7 +87,7 @@ __global__ __launch_bounds__(kFB) void k_leo_batch(const LeoDev f, const LeoBatc
             encode_slice<F>(f, a, lds, static_cast<size_t>(g - tasks[lo].slice0) * a.slice);
         } else {
             const LeoDecodeArgs a = tasks[lo].dec;
-            decode_slice<F>(f, a, lds, static_cast<size_t>(g - tasks[lo].slice0) * a.slice);
+            decode_slice<F, MAXD>(f, a, lds, static_cast<size_t>(g - tasks[lo].slice0) * a.slice);
         }
     }
 }
@@ -141,32 +142,48 @@ hipError_t leo_encode_fused(const LeoDev &f, const LeoEncodeArgs &a, hipStream_t
 }
 
 hipError_t leo_batch_launch(const LeoDev &f, const LeoBatchTask *tasks, int ntasks, uint32_t nslices, size_t lds,
-                            unsigned grid, hipStream_t s) {
+                            unsigned grid, size_t deriv_bytes, hipStream_t s) {
     if (ntasks <= 0 || nslices == 0) return hipSuccess;
+    // The smaller register budget when every reconstruct task fits it.
+    const bool small = deriv_bytes <= kLeoDerivSmall;
     hipError_t e;
+#define RSX_LAUNCH(FT, MD)                                                                          \
+    do {                                                                                             \
+        if ((e = allow_lds(k_leo_batch<FT, MD>, lds)) != hipSuccess) return e;                       \
+        hipLaunchKernelGGL((k_leo_batch<FT, MD>), dim3(grid), dim3(kFB), lds, s, f, tasks, ntasks, nslices); \
+    } while (0)
     if (f.bits == 16) {
-        if ((e = allow_lds(k_leo_batch<F16>, lds)) != hipSuccess) return e;
-        hipLaunchKernelGGL(k_leo_batch<F16>, dim3(grid), dim3(kFB), lds, s, f, tasks, ntasks, nslices);
+        if (small) RSX_LAUNCH(F16, kDerivMax / 2);
+        else RSX_LAUNCH(F16, kDerivMax);
     } else {
-        if ((e = allow_lds(k_leo_batch<F8>, lds)) != hipSuccess) return e;
-        hipLaunchKernelGGL(k_leo_batch<F8>, dim3(grid), dim3(kFB), lds, s, f, tasks, ntasks, nslices);
+        if (small) RSX_LAUNCH(F8, kDerivMax / 2);
+        else RSX_LAUNCH(F8, kDerivMax);
     }
+#undef RSX_LAUNCH
     return hipGetLastError();
 }
 
 hipError_t leo_decode_fused(const LeoDev &f, const LeoDecodeArgs &a, hipStream_t s) {
     // The derivative holds n * slice / 16 slots in registers.
-    if (static_cast<size_t>(a.n) * a.slice > kLeoFusedDecodeLds) return hipErrorInvalidValue;
+    const size_t deriv = static_cast<size_t>(a.n) * a.slice;
+    if (deriv > kLeoFusedDecodeLds) return hipErrorInvalidValue;
     const size_t lds = static_cast<size_t>(a.slice) * (a.n + 1);
     const unsigned threads = threads_for(f.bits, a.n, a.slice, lds);
+    const bool small = deriv <= kLeoDerivSmall;
     hipError_t e;
+#define RSX_LAUNCH(FT, MD)                                                                              \
+    do {                                                                                                 \
+        if ((e = allow_lds(k_decode_fused<FT, MD>, lds)) != hipSuccess) return e;                        \
+        hipLaunchKernelGGL((k_decode_fused<FT, MD>), dim3(grid_for(a.size, a.slice)), dim3(threads), lds, s, f, a); \
+    } while (0)
     if (f.bits == 16) {
-        if ((e = allow_lds(k_decode_fused<F16>, lds)) != hipSuccess) return e;
-        hipLaunchKernelGGL(k_decode_fused<F16>, dim3(grid_for(a.size, a.slice)), dim3(threads), lds, s, f, a);
+        if (small) RSX_LAUNCH(F16, kDerivMax / 2);
+        else RSX_LAUNCH(F16, kDerivMax);
     } else {
-        if ((e = allow_lds(k_decode_fused<F8>, lds)) != hipSuccess) return e;
-        hipLaunchKernelGGL(k_decode_fused<F8>, dim3(grid_for(a.size, a.slice)), dim3(threads), lds, s, f, a);
+        if (small) RSX_LAUNCH(F8, kDerivMax / 2);
+        else RSX_LAUNCH(F8, kDerivMax);
     }
+#undef RSX_LAUNCH
     return hipGetLastError();
 }
 

@@ -74,6 +74,9 @@ hipError_t leo_gf16_mul_xor(const LeoDev &f, const Gf16MulXor &a, hipStream_t s)
 // n * slice bytes in registers for the formal derivative.
 constexpr size_t kLeoFusedEncodeLds = 160u << 10;
 constexpr size_t kLeoFusedDecodeLds = 64u << 10;
+// Reconstructs with n * slice within this many bytes run a kernel variant
+// that keeps half the derivative registers (more workgroups per CU).
+constexpr size_t kLeoDerivSmall = 32u << 10;
 
 struct LeoEncodeArgs {
     const uint8_t *const *data;  // groups*m row pointers, group-major, null past row k
@@ -105,8 +108,9 @@ struct LeoBatchTask {
 
 // Runs every task's slices (tasks sorted by slice0, all of one field) with
 // `grid` persistent blocks of 256 threads and `lds` bytes (the largest task).
+// deriv_bytes: the largest n * slice of the reconstruct tasks (0 if none).
 hipError_t leo_batch_launch(const LeoDev &f, const LeoBatchTask *tasks, int ntasks, uint32_t nslices, size_t lds,
-                            unsigned grid, hipStream_t s);
+                            unsigned grid, size_t deriv_bytes, hipStream_t s);
 
 // Slice width for `regions` LDS regions of `rows` rows, radix-4 steps over
 // `step_rows` rows and shards of `size` bytes, or 0 when even 64-byte slices

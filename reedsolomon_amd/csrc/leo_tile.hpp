@@ -231,12 +231,13 @@ __device__ __forceinline__ void xor_tiles(const Tile &acc, const Tile &tmp, int 
 // Formal derivative in place (leopard.go:619-623): out[t] = in[t] ^ XOR over
 // zero bits b of t (t | 2^b < n) of in[t | 2^b].  Every thread computes its
 // slots into registers, then all store after a barrier.
+template <int MAXD = kDerivMax>
 __device__ __forceinline__ void derivative(const Tile &w, int n, int slots) {
     const int ns = log2i(n);
     const int items = n * slots;
-    uint4 res[kDerivMax];
+    uint4 res[MAXD];
 #pragma unroll
-    for (int j = 0; j < kDerivMax; ++j) {
+    for (int j = 0; j < MAXD; ++j) {
         const int it = threadIdx.x + j * blockDim.x;
         if (it < items) {
             const int t = it & (n - 1), s = it >> ns;
@@ -248,7 +249,7 @@ __device__ __forceinline__ void derivative(const Tile &w, int n, int slots) {
     }
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < kDerivMax; ++j) {
+    for (int j = 0; j < MAXD; ++j) {
         const int it = threadIdx.x + j * blockDim.x;
         if (it < items) *slot(w, it & (n - 1), it >> ns) = res[j];
     }
