@@ -158,7 +158,7 @@ int rsx_decode_cache_size(const rsx_codec *c);
  * size and erasure pattern): one launch per output-count class.  Leopard
  * codecs (leopard.go:139 / 393 / 421, leopard8.go likewise): one LDS-resident
  * launch per field, for 16-byte aligned shards whose rows fit one LDS tile
- * (every GF(2^8) shape; GF(2^16) up to m = 1024 for encode and n = 1024 work
+ * (every GF(2^8) shape; GF(2^16) up to m = 1024 for encode and n = 2048 work
  * rows for reconstruct); other Leopard items get RSX_ERR_NOT_SUPPORTED and
  * take the per-stripe call.  There is no single reference counterpart: each
  * item stands for one call of reedsolomon.go:686 / 857 / 1446 / 1634 and is

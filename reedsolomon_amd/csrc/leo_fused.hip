@@ -16,14 +16,26 @@ namespace {
 using namespace leo;
 using namespace leo_tile;
 
-// One column slice (bytes [col, col + slice) of every shard) of an encode.
+// Tile of the slice that starts `sc` bytes into every shard: a 32-byte
+// GF(2^16) slice is one unit of the 64-byte chunk at sc & ~63.
 template <class F>
-__device__ __forceinline__ void encode_slice(const LeoDev &f, const LeoEncodeArgs &a, uint8_t *lds, size_t col) {
-    const int ushift = log2i(a.slice / F::kUnitBytes);
+__device__ __forceinline__ Tile slice_tile(uint8_t *lds, int rows, int slice, size_t size, size_t sc,
+                                           size_t &col) {
+    const int cols = static_cast<int>(size - sc < static_cast<size_t>(slice) ? size - sc : slice);
+    const bool unit = slice < 64;
+    col = unit ? (sc & ~size_t{63}) : sc;
+    return Tile{lds, rows + 1, log2i(slice / F::kUnitBytes), cols / F::kUnitBytes, unit ? static_cast<int>((sc >> 5) & 1) : 0,
+                unit ? 1 : 2};
+}
+
+// One column slice (bytes [sc, sc + slice) of every shard) of an encode.
+template <class F>
+__device__ __forceinline__ void encode_slice(const LeoDev &f, const LeoEncodeArgs &a, uint8_t *lds, size_t sc) {
     const int region = a.slice * (a.m + 1);
-    const int cols = static_cast<int>(a.size - col < static_cast<size_t>(a.slice) ? a.size - col : a.slice);
-    const Tile acc{lds, a.m + 1, ushift, cols / F::kUnitBytes};
-    const Tile tmp{lds + region, a.m + 1, ushift, cols / F::kUnitBytes};
+    size_t col;
+    const Tile acc = slice_tile<F>(lds, a.m, a.slice, a.size, sc, col);
+    Tile tmp = acc;
+    tmp.base = lds + region;
     for (int g = 0; g < a.groups; ++g) {
         const Tile &w = g == 0 ? acc : tmp;
         const int mtrunc = g == 0 ? (a.k < a.m ? a.k : a.m) : (a.k - g * a.m < a.m ? a.k - g * a.m : a.m);
@@ -39,10 +51,10 @@ __device__ __forceinline__ void encode_slice(const LeoDev &f, const LeoEncodeArg
 // One column slice of a reconstruct; MAXD bounds the slots per thread the
 // formal derivative holds in registers (n * slice / 16 / threads).
 template <class F, int MAXD>
-__device__ __forceinline__ void decode_slice(const LeoDev &f, const LeoDecodeArgs &a, uint8_t *lds, size_t col) {
-    const int ushift = log2i(a.slice / F::kUnitBytes);
-    const int cols = static_cast<int>(a.size - col < static_cast<size_t>(a.slice) ? a.size - col : a.slice);
-    const Tile w{lds, a.n + 1, ushift, cols / F::kUnitBytes};
+__device__ __forceinline__ void decode_slice(const LeoDev &f, const LeoDecodeArgs &a, uint8_t *lds, size_t sc) {
+    size_t col;
+    const Tile w = slice_tile<F>(lds, a.n, a.slice, a.size, sc, col);
+    const int cols = w.ucount * F::kUnitBytes;
     load_rows<F>(w, a.rows, a.errlocs, a.n, col, f);
     ifft<F>(w, a.n, a.ifft_trunc, f.skew, -1, f);
     derivative<MAXD>(w, a.n, cols / 16);
@@ -51,20 +63,26 @@ __device__ __forceinline__ void decode_slice(const LeoDev &f, const LeoDecodeArg
     __syncthreads();
 }
 
+// Slice index of workgroup-loop index g: the two unit slices of a chunk go
+// to one XCD.
+__device__ __forceinline__ size_t slice_start(uint32_t g, uint32_t nslices, int slice) {
+    return static_cast<size_t>(slice < 64 ? xcd_unit_slice(g, nslices) : g) * slice;
+}
+
 template <class F>
 __global__ __launch_bounds__(kFBMax) void k_encode_fused(const LeoDev f, const LeoEncodeArgs a) {
     extern __shared__ uint4 lds_raw[];
-    for (size_t col = static_cast<size_t>(blockIdx.x) * a.slice; col < a.size;
-         col += static_cast<size_t>(gridDim.x) * a.slice)
-        encode_slice<F>(f, a, reinterpret_cast<uint8_t *>(lds_raw), col);
+    const uint32_t nslices = static_cast<uint32_t>((a.size + a.slice - 1) / a.slice);
+    for (uint32_t g = blockIdx.x; g < nslices; g += gridDim.x)
+        encode_slice<F>(f, a, reinterpret_cast<uint8_t *>(lds_raw), slice_start(g, nslices, a.slice));
 }
 
 template <class F, int MAXD>
 __global__ __launch_bounds__(kFBMax) void k_decode_fused(const LeoDev f, const LeoDecodeArgs a) {
     extern __shared__ uint4 lds_raw[];
-    for (size_t col = static_cast<size_t>(blockIdx.x) * a.slice; col < a.size;
-         col += static_cast<size_t>(gridDim.x) * a.slice)
-        decode_slice<F, MAXD>(f, a, reinterpret_cast<uint8_t *>(lds_raw), col);
+    const uint32_t nslices = static_cast<uint32_t>((a.size + a.slice - 1) / a.slice);
+    for (uint32_t g = blockIdx.x; g < nslices; g += gridDim.x)
+        decode_slice<F, MAXD>(f, a, reinterpret_cast<uint8_t *>(lds_raw), slice_start(g, nslices, a.slice));
 }
 
 // Batched: blocks stride over the global slice index of all tasks, find
@@ -82,12 +100,14 @@ __global__ __launch_bounds__(kFB) void k_leo_batch(const LeoDev f, const LeoBatc
             if (tasks[mid].slice0 <= g) lo = mid;
             else hi = mid - 1;
         }
+        const uint32_t end = lo + 1 < ntasks ? tasks[lo + 1].slice0 : nslices;
+        const uint32_t l = g - tasks[lo].slice0, count = end - tasks[lo].slice0;
         if (tasks[lo].op == 0) {
             const LeoEncodeArgs a = tasks[lo].enc;
-            encode_slice<F>(f, a, lds, static_cast<size_t>(g - tasks[lo].slice0) * a.slice);
+            encode_slice<F>(f, a, lds, slice_start(l, count, a.slice));
         } else {
             const LeoDecodeArgs a = tasks[lo].dec;
-            decode_slice<F, MAXD>(f, a, lds, static_cast<size_t>(g - tasks[lo].slice0) * a.slice);
+            decode_slice<F, MAXD>(f, a, lds, slice_start(l, count, a.slice));
         }
     }
 }
@@ -109,6 +129,9 @@ int leo_fused_slice(int bits, int rows, int regions, int step_rows, size_t size,
     while ((step_rows / 4) * (slice / unit) < kFB && slice < 4096 && size / (2 * slice) >= 512) slice *= 2;
     // ... within the LDS budget.
     while (slice > 64 && bytes(slice) > lds_budget) slice /= 2;
+    // GF(2^16) tiles of whole chunks that leave room for fewer than four
+    // workgroups per CU take one 32-byte unit per workgroup instead.
+    if (bits == 16 && slice == 64 && bytes(64) > (40u << 10) && bytes(32) <= lds_budget) slice = 32;
     return bytes(slice) <= lds_budget ? slice : 0;
 }
 

@@ -83,7 +83,8 @@ struct LeoEncodeArgs {
     uint8_t *const *parity;      // p row pointers
     size_t size;                 // bytes per shard (multiple of 64)
     int k, p, m, groups;         // m = ceilPow2(p)
-    int slice;                   // bytes per row per workgroup (power of two >= 64)
+    int slice;                   // bytes per row per workgroup (power of two >= 64; 32 = one
+                                 // GF(2^16) unit of a chunk)
 };
 
 struct LeoDecodeArgs {
@@ -120,7 +121,7 @@ hipError_t leo_encode_fused(const LeoDev &f, const LeoEncodeArgs &a, hipStream_t
 hipError_t leo_decode_fused(const LeoDev &f, const LeoDecodeArgs &a, hipStream_t s);
 
 // ---- GF(2^16) reconstruct in three LDS-tiled passes (leo_multipass.hip) ----
-// For n >= 2048 rows: work arrays hold `slices` 64-byte column slices of all
+// For n >= 4096 rows (one tile holds 2048): work arrays hold `slices` 64-byte column slices of all
 // n rows, slice-major.
 struct LeoMultipassArgs {
     const uint8_t *const *rows;  // n row pointers (null = zero / erased)

@@ -1,5 +1,5 @@
 // leo_multipass.hip — Leopard GF(2^16) reconstruct for transforms taller than
-// one LDS tile (n >= 2048 rows).
+// one LDS tile (n >= 4096 rows; the passes also hold for n = 2048).
 //
 // The n-row transform (leopard.go:421-647: scale by error locators, IFFT,
 // formal derivative, FFT, reveal) runs as three passes over 64-byte column
@@ -181,7 +181,7 @@ __global__ __launch_bounds__(kMB) void k_dec_pass2(const LeoDev f, const LeoMult
     store_tile<F>(y, R, [&](int l) { return wrow(a.work, a.n, s, grow(l)); });    // u
 }
 
-template <class F>
+template <class F, int MAXD>
 __global__ __launch_bounds__(kMB) void k_dec_pass3(const LeoDev f, const LeoMultipassArgs a) {
     extern __shared__ uint4 lds_raw[];
     const int s = blockIdx.x;
@@ -194,9 +194,9 @@ __global__ __launch_bounds__(kMB) void k_dec_pass3(const LeoDev f, const LeoMult
         const int slots = kSlice / 16;
         const int ps = log2i(a.P);
         const int items = a.P * slots;
-        uint4 res[kDerivMax];
+        uint4 res[MAXD];
 #pragma unroll
-        for (int jj = 0; jj < kDerivMax; ++jj) {
+        for (int jj = 0; jj < MAXD; ++jj) {
             const int it = threadIdx.x + jj * kMB;
             if (it < items) {
                 const int t = it & (a.P - 1), sl = it >> ps;
@@ -208,7 +208,7 @@ __global__ __launch_bounds__(kMB) void k_dec_pass3(const LeoDev f, const LeoMult
         }
         __syncthreads();
 #pragma unroll
-        for (int jj = 0; jj < kDerivMax; ++jj) {
+        for (int jj = 0; jj < MAXD; ++jj) {
             const int it = threadIdx.x + jj * kMB;
             if (it < items) *slot(w, it & (a.P - 1), it >> ps) = res[jj];
         }
@@ -275,12 +275,18 @@ hipError_t leo_decode_multipass(const LeoDev &f, const LeoMultipassArgs &a, hipS
     };
     if ((e = allow(reinterpret_cast<const void *>(k_dec_pass1<F16>), lds1)) != hipSuccess) return e;
     if ((e = allow(reinterpret_cast<const void *>(k_dec_pass2<F16>), lds2)) != hipSuccess) return e;
-    if ((e = allow(reinterpret_cast<const void *>(k_dec_pass3<F16>), lds3)) != hipSuccess) return e;
+    // z = u + L(v) holds P * 4 slots in registers: 8 per thread up to P = 512.
+    const bool small = a.P * (kSlice / 16) <= (kDerivMax / 2) * kMB;
+    if ((e = allow(small ? reinterpret_cast<const void *>(k_dec_pass3<F16, kDerivMax / 2>)
+                         : reinterpret_cast<const void *>(k_dec_pass3<F16, kDerivMax>),
+                   lds3)) != hipSuccess)
+        return e;
     hipLaunchKernelGGL(k_dec_pass1<F16>, dim3(slices, a.n / a.C), dim3(kMB), lds1, s, f, a);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(k_dec_pass2<F16>, dim3(slices, a.P / a.A), dim3(kMB), lds2, s, f, a);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_dec_pass3<F16>, dim3(slices, a.n / a.P), dim3(kMB), lds3, s, f, a);
+    if (small) hipLaunchKernelGGL((k_dec_pass3<F16, kDerivMax / 2>), dim3(slices, a.n / a.P), dim3(kMB), lds3, s, f, a);
+    else hipLaunchKernelGGL((k_dec_pass3<F16, kDerivMax>), dim3(slices, a.n / a.P), dim3(kMB), lds3, s, f, a);
     return hipGetLastError();
 }
 

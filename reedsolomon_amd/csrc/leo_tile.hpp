@@ -20,8 +20,10 @@ constexpr int kDerivMax = kLeoFusedDecodeLds / 16 / kFB;  // uint4 per thread in
 
 // Work rows of one slice in LDS, slot-major: 16-byte slot s of row r sits at
 // base + (s * r1 + pos(r)) * 16 with r1 = rows + 1.  A GF(2^8) column unit
-// is one slot; a GF(2^16) unit is the lo slot and the hi slot two further
-// (the 64-byte chunk layout of the shard, slot by slot).
+// is one slot.  A GF(2^16) unit is a lo slot and a hi slot `hio` further:
+// slices of whole 64-byte chunks keep the chunk layout (hio = 2); a 32-byte
+// slice holds one unit of a chunk (unit `ubase` of it, hio = 1), which halves
+// the LDS of a tile.
 //
 // Butterfly lanes take consecutive rows of one slot; pos() permutes the low
 // four row bits by bits 4 and 5 so that the 16 rows a quarter-wave touches in
@@ -30,9 +32,11 @@ constexpr int kDerivMax = kLeoFusedDecodeLds / 16 / kFB;  // uint4 per thread in
 // consecutive slots of one row; the +1 row of padding keeps those distinct.
 struct Tile {
     uint8_t *base;
-    int r1;      // rows + 1
-    int ushift;  // log2(column units in a full slice)
-    int ucount;  // column units of this slice (the last slice may be narrower)
+    int r1;         // rows + 1
+    int ushift;     // log2(column units in a full slice)
+    int ucount;     // column units of this slice (the last slice may be narrower)
+    int ubase = 0;  // unit of the shard chunk that local unit 0 is
+    int hio = 2;    // GF(2^16): slots from a unit's lo slot to its hi slot
 };
 
 __device__ __forceinline__ int log2i(int x) {
@@ -42,6 +46,17 @@ __device__ __forceinline__ int log2i(int x) {
 }
 
 __device__ __forceinline__ int pos(int r) { return r ^ (((r >> 4) & 1) * 5) ^ (((r >> 5) & 1) * 10); }
+
+// Workgroup index g -> unit slice index when a 64-byte GF(2^16) chunk is
+// split into two 32-byte unit slices (slice 2c + u is unit u of chunk c).
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2, so g and g + 8 share an XCD: they take the two units of one chunk, and
+// the chunk's cache lines are fetched from HBM once.  The tail past the last
+// multiple of 16 keeps g itself.
+__device__ __forceinline__ uint32_t xcd_unit_slice(uint32_t g, uint32_t count) {
+    if (g >= (count & ~15u)) return g;
+    return (((g >> 4) << 4) | ((g & 7) << 1)) | ((g >> 3) & 1);
+}
 
 __device__ __forceinline__ uint4 *slot(const Tile &t, int r, int s) {
     return reinterpret_cast<uint4 *>(t.base + ((s * t.r1 + pos(r)) << 4));
@@ -56,11 +71,15 @@ struct Lds<F8> {
 };
 template <>
 struct Lds<F16> {
-    __device__ static int lo(int u) { return ((u >> 1) << 2) | (u & 1); }
-    __device__ static F16::E load(const Tile &t, int r, int u) { return {*slot(t, r, lo(u)), *slot(t, r, lo(u) + 2)}; }
+    __device__ static int lo(const Tile &t, int u) { return t.hio == 2 ? (((u >> 1) << 2) | (u & 1)) : (u << 1); }
+    __device__ static F16::E load(const Tile &t, int r, int u) {
+        const int l = lo(t, u);
+        return {*slot(t, r, l), *slot(t, r, l + t.hio)};
+    }
     __device__ static void store(const Tile &t, int r, int u, const F16::E &e) {
-        *slot(t, r, lo(u)) = e.lo;
-        *slot(t, r, lo(u) + 2) = e.hi;
+        const int l = lo(t, u);
+        *slot(t, r, l) = e.lo;
+        *slot(t, r, l + t.hio) = e.hi;
     }
 };
 
@@ -186,7 +205,7 @@ __device__ __forceinline__ void load_rows(const Tile &w, const uint8_t *const *s
             v[j] = F::zero();
             if (it < items && (it & umask) < w.ucount) {
                 const uint8_t *s = src[it >> w.ushift];
-                if (s) v[j] = F::gload(s + col, it & umask);
+                if (s) v[j] = F::gload(s + col, (it & umask) + w.ubase);
             }
         }
 #pragma unroll
@@ -215,7 +234,7 @@ __device__ __forceinline__ void store_rows(const Tile &w, uint8_t *const *dst, c
         typename F::E v = F::zero();
         if (logs) F::mul_add(v, x, table<F>(f, logs[q]));
         else v = x;
-        F::gstore(dst[q] + col, u, v);
+        F::gstore(dst[q] + col, u + w.ubase, v);
     });
 }
 
