@@ -71,7 +71,7 @@ struct rsx_batch {
     std::vector<rsx::LeoBatchTask> leo_tasks[2];
     DeviceBuffer d_leo_tasks[2], d_leo_aux;
     size_t leo_lds[2] = {0, 0};
-    size_t leo_deriv[2] = {0, 0};  // largest n * slice of the reconstruct tasks
+    size_t leo_deriv[2] = {0, 0};  // largest register bytes (LeoItemPlan::regs) of the tasks
     uint32_t leo_slices[2] = {0, 0};
 };
 
@@ -572,8 +572,7 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
                 return fail(RSX_ERR_INVALID_INPUT, "batch too large for one plan");
             slice0 += lp.slices;
             b->leo_lds[fi] = std::max(b->leo_lds[fi], lp.lds);
-            if (t.op == 1)
-                b->leo_deriv[fi] = std::max(b->leo_deriv[fi], static_cast<size_t>(t.dec.n) * t.dec.slice);
+            b->leo_deriv[fi] = std::max(b->leo_deriv[fi], lp.regs);
             b->leo_tasks[fi].push_back(t);
         }
         b->leo_slices[fi] = slice0;

@@ -29,22 +29,37 @@ __device__ __forceinline__ Tile slice_tile(uint8_t *lds, int rows, int slice, si
 }
 
 // One column slice (bytes [sc, sc + slice) of every shard) of an encode.
-template <class F>
+// With several m-row groups, the XOR of their IFFTs accumulates in
+// registers (MAXD slots per thread, row-fastest like the derivative) while
+// the one LDS tile takes the next group; the last group's IFFT is folded
+// into the tile in place before the FFT.
+template <class F, int MAXD>
 __device__ __forceinline__ void encode_slice(const LeoDev &f, const LeoEncodeArgs &a, uint8_t *lds, size_t sc) {
-    const int region = a.slice * (a.m + 1);
     size_t col;
-    const Tile acc = slice_tile<F>(lds, a.m, a.slice, a.size, sc, col);
-    Tile tmp = acc;
-    tmp.base = lds + region;
+    const Tile w = slice_tile<F>(lds, a.m, a.slice, a.size, sc, col);
+    const int ms = log2i(a.m);
+    const int items = a.m * (w.ucount * F::kUnitBytes / 16);
+    uint4 acc[MAXD];
     for (int g = 0; g < a.groups; ++g) {
-        const Tile &w = g == 0 ? acc : tmp;
         const int mtrunc = g == 0 ? (a.k < a.m ? a.k : a.m) : (a.k - g * a.m < a.m ? a.k - g * a.m : a.m);
         load_rows<F>(w, a.data + static_cast<size_t>(g) * a.m, nullptr, a.m, col, f);
         ifft<F>(w, a.m, mtrunc, f.skew + a.m - 1 + g * a.m, 0, f);
-        if (g > 0) xor_tiles(acc, tmp, region / 16);
+        if (a.groups == 1) break;
+        const bool first = g == 0, last = g == a.groups - 1;
+#pragma unroll
+        for (int j = 0; j < MAXD; ++j) {
+            const int it = threadIdx.x + j * blockDim.x;
+            if (it < items) {
+                uint4 *p = slot(w, it & (a.m - 1), it >> ms);
+                if (first) acc[j] = *p;
+                else if (last) *p = xor4(acc[j], *p);
+                else acc[j] = xor4(acc[j], *p);
+            }
+        }
+        __syncthreads();  // before the next group's rows (or the FFT) reuse the tile
     }
-    fft<F>(acc, a.m, a.p, f);
-    store_rows<F>(acc, a.parity, nullptr, nullptr, a.p, col, f);
+    fft<F>(w, a.m, a.p, f);
+    store_rows<F>(w, a.parity, nullptr, nullptr, a.p, col, f);
     __syncthreads();  // the next slice reuses the rows
 }
 
@@ -69,12 +84,12 @@ __device__ __forceinline__ size_t slice_start(uint32_t g, uint32_t nslices, int 
     return static_cast<size_t>(slice < 64 ? xcd_unit_slice(g, nslices) : g) * slice;
 }
 
-template <class F>
+template <class F, int MAXD>
 __global__ __launch_bounds__(kFBMax) void k_encode_fused(const LeoDev f, const LeoEncodeArgs a) {
     extern __shared__ uint4 lds_raw[];
     const uint32_t nslices = static_cast<uint32_t>((a.size + a.slice - 1) / a.slice);
     for (uint32_t g = blockIdx.x; g < nslices; g += gridDim.x)
-        encode_slice<F>(f, a, reinterpret_cast<uint8_t *>(lds_raw), slice_start(g, nslices, a.slice));
+        encode_slice<F, MAXD>(f, a, reinterpret_cast<uint8_t *>(lds_raw), slice_start(g, nslices, a.slice));
 }
 
 template <class F, int MAXD>
@@ -104,7 +119,7 @@ __global__ __launch_bounds__(kFB) void k_leo_batch(const LeoDev f, const LeoBatc
         const uint32_t l = g - tasks[lo].slice0, count = end - tasks[lo].slice0;
         if (tasks[lo].op == 0) {
             const LeoEncodeArgs a = tasks[lo].enc;
-            encode_slice<F>(f, a, lds, slice_start(l, count, a.slice));
+            encode_slice<F, MAXD>(f, a, lds, slice_start(l, count, a.slice));
         } else {
             const LeoDecodeArgs a = tasks[lo].dec;
             decode_slice<F, MAXD>(f, a, lds, slice_start(l, count, a.slice));
@@ -151,23 +166,34 @@ static unsigned threads_for(int bits, int rows, int slice, size_t lds) {
 }
 
 hipError_t leo_encode_fused(const LeoDev &f, const LeoEncodeArgs &a, hipStream_t s) {
-    const size_t lds = static_cast<size_t>(a.groups > 1 ? 2 : 1) * a.slice * (a.m + 1);
+    // Several groups accumulate m * slice bytes in registers.
+    const size_t acc = a.groups > 1 ? static_cast<size_t>(a.m) * a.slice : 0;
+    if (acc > kLeoFusedDecodeLds) return hipErrorInvalidValue;
+    const size_t lds = static_cast<size_t>(a.slice) * (a.m + 1);
     const unsigned threads = threads_for(f.bits, a.m, a.slice, lds);
+    const bool small = acc <= kLeoDerivSmall;
     hipError_t e;
+#define RSX_LAUNCH(FT, MD)                                                                              \
+    do {                                                                                                 \
+        if ((e = allow_lds(k_encode_fused<FT, MD>, lds)) != hipSuccess) return e;                        \
+        hipLaunchKernelGGL((k_encode_fused<FT, MD>), dim3(grid_for(a.size, a.slice)), dim3(threads), lds, s, f, a); \
+    } while (0)
     if (f.bits == 16) {
-        if ((e = allow_lds(k_encode_fused<F16>, lds)) != hipSuccess) return e;
-        hipLaunchKernelGGL(k_encode_fused<F16>, dim3(grid_for(a.size, a.slice)), dim3(threads), lds, s, f, a);
+        if (small) RSX_LAUNCH(F16, kDerivMax / 2);
+        else RSX_LAUNCH(F16, kDerivMax);
     } else {
-        if ((e = allow_lds(k_encode_fused<F8>, lds)) != hipSuccess) return e;
-        hipLaunchKernelGGL(k_encode_fused<F8>, dim3(grid_for(a.size, a.slice)), dim3(threads), lds, s, f, a);
+        if (small) RSX_LAUNCH(F8, kDerivMax / 2);
+        else RSX_LAUNCH(F8, kDerivMax);
     }
+#undef RSX_LAUNCH
     return hipGetLastError();
 }
 
 hipError_t leo_batch_launch(const LeoDev &f, const LeoBatchTask *tasks, int ntasks, uint32_t nslices, size_t lds,
                             unsigned grid, size_t deriv_bytes, hipStream_t s) {
     if (ntasks <= 0 || nslices == 0) return hipSuccess;
-    // The smaller register budget when every reconstruct task fits it.
+    // The smaller register budget when every task's derivative (reconstruct)
+    // or group accumulator (encode) fits it.
     const bool small = deriv_bytes <= kLeoDerivSmall;
     hipError_t e;
 #define RSX_LAUNCH(FT, MD)                                                                          \

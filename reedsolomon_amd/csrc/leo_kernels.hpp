@@ -68,10 +68,10 @@ struct Gf16MulXor {
 hipError_t leo_gf16_mul_xor(const LeoDev &f, const Gf16MulXor &a, hipStream_t s);
 
 // ---- LDS-resident transforms (leo_fused.hip) ----
-// Encode keeps the accumulator and one group's rows of a column slice in LDS
-// (2 regions of m rows, 1 when k <= m); reconstruct keeps all n rows.  A
-// region of R rows takes slice * (R + 1) bytes.  Reconstruct also holds
-// n * slice bytes in registers for the formal derivative.
+// Encode keeps one group's m rows of a column slice in LDS and, with several
+// groups, their XOR accumulator (m * slice bytes) in registers; reconstruct
+// keeps all n rows in LDS and n * slice bytes in registers for the formal
+// derivative.  R rows take slice * (R + 1) bytes of LDS.
 constexpr size_t kLeoFusedEncodeLds = 160u << 10;
 constexpr size_t kLeoFusedDecodeLds = 64u << 10;
 // Reconstructs with n * slice within this many bytes run a kernel variant

@@ -238,15 +238,6 @@ __device__ __forceinline__ void store_rows(const Tile &w, uint8_t *const *dst, c
     });
 }
 
-// acc ^= tmp, every 16-byte position of two equally shaped tiles.
-__device__ __forceinline__ void xor_tiles(const Tile &acc, const Tile &tmp, int positions) {
-    for (int it = threadIdx.x; it < positions; it += blockDim.x) {
-        uint4 *a = reinterpret_cast<uint4 *>(acc.base) + it;
-        *a = xor4(*a, reinterpret_cast<const uint4 *>(tmp.base)[it]);
-    }
-    __syncthreads();
-}
-
 // Formal derivative in place (leopard.go:619-623): out[t] = in[t] ^ XOR over
 // zero bits b of t (t | 2^b < n) of in[t | 2^b].  Every thread computes its
 // slots into registers, then all store after a barrier.

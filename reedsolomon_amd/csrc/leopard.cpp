@@ -154,6 +154,13 @@ size_t chunk_for(size_t size, int rows, size_t budget) {
 constexpr size_t kEncodeBudget = 64u << 20;
 constexpr size_t kDecodeBudget = 64u << 20;
 
+// LDS budget of a one-tile encode: the m rows of a slice, and with several
+// groups their XOR accumulator in registers (at most the derivative's
+// m * slice <= 64 KiB).
+size_t encode_lds_budget(int m, int groups) {
+    return groups > 1 ? std::min(kLeoFusedEncodeLds, kLeoFusedDecodeLds / m * (m + 1)) : kLeoFusedEncodeLds;
+}
+
 bool all_aligned(const std::vector<uint8_t *> &p) {
     uintptr_t bits = 0;
     for (uint8_t *q : p) bits |= reinterpret_cast<uintptr_t>(q);
@@ -221,7 +228,7 @@ int encode_device(rsx_codec *c, const std::vector<uint8_t *> &data, const std::v
 
     const uint8_t *const *dptr = reinterpret_cast<const uint8_t *const *>(s.ptrs.ptr);
     uint8_t *const *pptr = reinterpret_cast<uint8_t *const *>(s.ptrs.as<uint8_t *>() + static_cast<size_t>(groups) * m);
-    const int slice = s.fused ? leo_fused_slice(s.bits, m, groups > 1 ? 2 : 1, m, size, kLeoFusedEncodeLds) : 0;
+    const int slice = s.fused ? leo_fused_slice(s.bits, m, 1, m, size, encode_lds_budget(m, groups)) : 0;
     if (slice) {
         const LeoEncodeArgs a{dptr, pptr, size, k, p, m, groups, slice};
         RSX_HIP(leo_encode_fused(*s.dev, a, c->stream));
@@ -585,7 +592,7 @@ int leopard_plan_item(rsx_codec *c, int op, uint8_t *const *shards, const size_t
     if (!rec) {
         const int groups = (k + m - 1) / m;
         // Other items fill the device: size the slice for full workgroups.
-        const int slice = leo_fused_slice(s.bits, m, groups > 1 ? 2 : 1, m, kBatchedSize, kLeoFusedEncodeLds);
+        const int slice = leo_fused_slice(s.bits, m, 1, m, kBatchedSize, encode_lds_budget(m, groups));
         if (!slice) return fail(RSX_ERR_NOT_SUPPORTED, "Leopard shape too tall for the batched kernel");
         out.ptr_off = pool.size();
         for (int g = 0; g < groups; ++g)
@@ -597,7 +604,8 @@ int leopard_plan_item(rsx_codec *c, int op, uint8_t *const *shards, const size_t
         for (int i = 0; i < p; ++i) pool.push_back(shards[k + i]);
         out.task.op = 0;
         out.task.enc = LeoEncodeArgs{nullptr, nullptr, size, k, p, m, groups, slice};
-        out.lds = static_cast<size_t>(groups > 1 ? 2 : 1) * slice * (m + 1);
+        out.lds = static_cast<size_t>(slice) * (m + 1);
+        out.regs = groups > 1 ? static_cast<size_t>(m) * slice : 0;
         out.slices = static_cast<uint32_t>((size + slice - 1) / slice);
         (void)f;
         return RSX_OK;
@@ -633,6 +641,7 @@ int leopard_plan_item(rsx_codec *c, int op, uint8_t *const *shards, const size_t
     out.task.dec = LeoDecodeArgs{nullptr, nullptr, nullptr, nullptr, nullptr, size, n, nout,
                                  s.bits == 16 ? pat->input_count : m + k, m + k, slice};
     out.lds = static_cast<size_t>(slice) * (n + 1);
+    out.regs = static_cast<size_t>(n) * slice;
     out.slices = static_cast<uint32_t>((size + slice - 1) / slice);
     out.empty = nout == 0;
     return RSX_OK;

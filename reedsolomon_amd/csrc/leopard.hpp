@@ -33,6 +33,7 @@ struct LeoItemPlan {
     const LeoDev *dev = nullptr;
     LeoBatchTask task{};
     size_t lds = 0;
+    size_t regs = 0;  // bytes a workgroup holds in registers (derivative / group accumulator)
     uint32_t slices = 0;
     size_t ptr_off = 0, errl_off = 0, rows_off = 0, logs_off = 0;
     size_t parity_pool = 0;  // encode: pool index of the first parity pointer
