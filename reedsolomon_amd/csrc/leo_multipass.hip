@@ -34,7 +34,11 @@ namespace {
 using namespace leo;
 using namespace leo_tile;
 
-constexpr int kMB = 256;     // threads per workgroup
+constexpr int kMB = 256;     // threads per workgroup of pass 3 (z = u + L(v) in registers)
+// Passes 1 and 2 hold 64 KiB tiles (two workgroups per CU): 512 threads
+// keep 16 waves per CU busy (5% faster than 256 at n = 8192).
+constexpr int kMB1 = 512;    // threads per workgroup of pass 1
+constexpr int kMB2 = 512;    // threads per workgroup of pass 2
 constexpr int kSlice = 64;   // bytes per row per workgroup: one 64-byte chunk
 
 __device__ __forceinline__ uint8_t *wrow(uint8_t *work, int n, int s, int r) {
@@ -48,16 +52,16 @@ __device__ __forceinline__ void load_tile(const Tile &w, int nrows, RowPtr rp) {
     constexpr int kBatch = 4;
     const int items = nrows << w.ushift;
     const int umask = (1 << w.ushift) - 1;
-    for (int it0 = threadIdx.x; it0 < items; it0 += kBatch * kMB) {
+    for (int it0 = threadIdx.x; it0 < items; it0 += kBatch * blockDim.x) {
         typename F::E v[kBatch];
 #pragma unroll
         for (int j = 0; j < kBatch; ++j) {
-            const int it = it0 + j * kMB;
+            const int it = it0 + j * blockDim.x;
             if (it < items) v[j] = F::gload(rp(it >> w.ushift), it & umask);
         }
 #pragma unroll
         for (int j = 0; j < kBatch; ++j) {
-            const int it = it0 + j * kMB;
+            const int it = it0 + j * blockDim.x;
             if (it < items) Lds<F>::store(w, it >> w.ushift, it & umask, v[j]);
         }
     }
@@ -82,7 +86,7 @@ __device__ __forceinline__ void strided_step4(const Tile &w, const Tile *w2, int
     const int qs = log2i(nq);
     const int A = 1 << ashift;
     const int items = (A << qs) << w.ushift;
-    for (int it = threadIdx.x; it < items; it += kMB) {
+    for (int it = threadIdx.x; it < items; it += blockDim.x) {
         const int a = it & (A - 1);
         const int q = (it >> ashift) & ((1 << qs) - 1);
         const int u = it >> (ashift + qs);
@@ -117,7 +121,7 @@ __device__ __forceinline__ void strided_step4(const Tile &w, const Tile *w2, int
 }
 
 template <class F>
-__global__ __launch_bounds__(kMB) void k_dec_pass1(const LeoDev f, const LeoMultipassArgs a) {
+__global__ __launch_bounds__(kMB1) void k_dec_pass1(const LeoDev f, const LeoMultipassArgs a) {
     extern __shared__ uint4 lds_raw[];
     const int s = blockIdx.x;
     const int base = blockIdx.y * a.C;
@@ -129,7 +133,7 @@ __global__ __launch_bounds__(kMB) void k_dec_pass1(const LeoDev f, const LeoMult
 }
 
 template <class F>
-__global__ __launch_bounds__(kMB) void k_dec_pass2(const LeoDev f, const LeoMultipassArgs a) {
+__global__ __launch_bounds__(kMB2) void k_dec_pass2(const LeoDev f, const LeoMultipassArgs a) {
     extern __shared__ uint4 lds_raw[];
     const int s = blockIdx.x;
     const int lo0 = blockIdx.y * a.A;
@@ -150,7 +154,7 @@ __global__ __launch_bounds__(kMB) void k_dec_pass2(const LeoDev f, const LeoMult
         const int dj = dist / a.P;
         const unsigned lm = f.skew[dist - 1];
         const int items = (R / 2) << in.ushift;
-        for (int it = threadIdx.x; it < items; it += kMB) {
+        for (int it = threadIdx.x; it < items; it += blockDim.x) {
             const int u = it & 1, p = it >> 1;  // p < A * dj
             const int l0 = p, l1 = p + (dj << ashift);
             typename F::E x0 = Lds<F>::load(in, l0, u), x1 = Lds<F>::load(in, l1, u);
@@ -165,7 +169,7 @@ __global__ __launch_bounds__(kMB) void k_dec_pass2(const LeoDev f, const LeoMult
         const int slots = kSlice / 16;
         const int items = R * slots;
         const int rshift = ashift + jshift;
-        for (int it = threadIdx.x; it < items; it += kMB) {
+        for (int it = threadIdx.x; it < items; it += blockDim.x) {
             const int l = it & (R - 1), sl = it >> rshift;
             const int j = l >> ashift, lo = l & (a.A - 1);
             uint4 acc = *slot(in, l, sl);
@@ -197,7 +201,7 @@ __global__ __launch_bounds__(kMB) void k_dec_pass3(const LeoDev f, const LeoMult
         uint4 res[MAXD];
 #pragma unroll
         for (int jj = 0; jj < MAXD; ++jj) {
-            const int it = threadIdx.x + jj * kMB;
+            const int it = threadIdx.x + jj * blockDim.x;
             if (it < items) {
                 const int t = it & (a.P - 1), sl = it >> ps;
                 uint4 acc = gload4(wrow(a.work, a.n, s, base + t) + sl * 16);
@@ -209,7 +213,7 @@ __global__ __launch_bounds__(kMB) void k_dec_pass3(const LeoDev f, const LeoMult
         __syncthreads();
 #pragma unroll
         for (int jj = 0; jj < MAXD; ++jj) {
-            const int it = threadIdx.x + jj * kMB;
+            const int it = threadIdx.x + jj * blockDim.x;
             if (it < items) *slot(w, it & (a.P - 1), it >> ps) = res[jj];
         }
         __syncthreads();
@@ -281,9 +285,9 @@ hipError_t leo_decode_multipass(const LeoDev &f, const LeoMultipassArgs &a, hipS
                          : reinterpret_cast<const void *>(k_dec_pass3<F16, kDerivMax>),
                    lds3)) != hipSuccess)
         return e;
-    hipLaunchKernelGGL(k_dec_pass1<F16>, dim3(slices, a.n / a.C), dim3(kMB), lds1, s, f, a);
+    hipLaunchKernelGGL(k_dec_pass1<F16>, dim3(slices, a.n / a.C), dim3(kMB1), lds1, s, f, a);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_dec_pass2<F16>, dim3(slices, a.P / a.A), dim3(kMB), lds2, s, f, a);
+    hipLaunchKernelGGL(k_dec_pass2<F16>, dim3(slices, a.P / a.A), dim3(kMB2), lds2, s, f, a);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (small) hipLaunchKernelGGL((k_dec_pass3<F16, kDerivMax / 2>), dim3(slices, a.n / a.P), dim3(kMB), lds3, s, f, a);
     else hipLaunchKernelGGL((k_dec_pass3<F16, kDerivMax>), dim3(slices, a.n / a.P), dim3(kMB), lds3, s, f, a);
