@@ -10,7 +10,8 @@ planned, graph-replayed launch for all encodes, one for all reconstructs).
 `value` is the batched path; the plain path is reported beside it.
 
 Also measured (unless --quick): RS(10,4) over 1 MiB shards (1024 stripes,
-batched encode + reconstruct), the mixed workload (8192 stripes, (k, m) in
+batched encode + reconstruct), RS(10,3) over one stripe of 10 x 1 MiB (plain
+Encode, Verify, Reconstruct of data shards 3 and 7), the mixed workload (8192 stripes, (k, m) in
 {(4,2),(6,3),(10,4),(17,3),(32,8)}, shards log-uniform 4 KiB-1 MiB rounded to
 64 B, 30% reconstructs with 1..m erasures) batched vs plain, and Leopard
 GF(2^16) k=4096, m=1024, 64 KiB shards (encode + reconstruct of 1024 erasures).
@@ -30,8 +31,10 @@ arrivals at 0.7x that rate; p50/p99 latency from arrival to completion.
 
 Prints one JSON line.  `roofline` is for the dominant kernel (the batched
 RS(16,4) encode pass over all 256 stripes), timed per launch with HIP events
-on the stream it runs on; `cpu_baseline` times the CPU oracle's SIMD
-restatement of the reference on the host cores for a bounded sample.
+on the stream it runs on.  The cpu_baseline leg checks samples of the GPU's
+output bytes against the CPU oracle (`oracle_checks`) and times the oracle's
+restatement of the reference on the host cores for bounded samples
+(`cpu_baseline`; --no-cpu skips the timing, not the checks).
 """
 from __future__ import annotations
 
@@ -66,7 +69,7 @@ def _parse():
                     help="served stream: Poisson rate as a fraction of the measured saturation rate")
     ap.add_argument("--hbm-budget-gib", type=float, default=16.0,
                     help="served stream: shard bytes admitted in flight")
-    ap.add_argument("--only", choices=["rs10_4", "mixed", "leopard16", "served", "stream"],
+    ap.add_argument("--only", choices=["rs10_4", "rs10_3", "mixed", "leopard16", "served", "stream"],
                     help="run one secondary section alone and print its JSON (for profiling)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="with --encode-only: also copy 1 GiB HBM->HBM 5 times (counter calibration)")
@@ -122,7 +125,8 @@ def batch_items(rs, st, codecs, ops, lens):
 
 def plan(rs, b, items, n):
     status = (ctypes.c_int * n)()
-    filled = (ctypes.c_uint8 * (n * 64))()
+    # one reconstruct flag per shard of every item
+    filled = (ctypes.c_uint8 * max(1, sum(items[i].nshards for i in range(n))))()
     _check(rs.lib, rs.lib.rsx_batch_plan(b._h, items, n, status, filled))
     bad = [status[i] for i in range(n) if status[i] != 0]
     if bad:
@@ -235,17 +239,10 @@ def headline(args, torch, rs):
     _check(lib, lib.rsx_synchronize(h))
     benc.synchronize()
 
-    # parity of one column slice of stripe 0 against the CPU oracle
-    parity_ok = None
-    try:
-        from oracle import rsref
-        cols = 1 << 16
-        ref = [st.shard(0, i)[:cols].cpu().numpy().copy() for i in range(n)]
-        want = [r.copy() for r in ref[:k]] + [np.zeros(cols, np.uint8) for _ in range(m)]
-        rsref.Classic(k, m).encode(want)
-        parity_ok = all(np.array_equal(want[k + i], ref[k + i]) for i in range(m))
-    except Exception as ex:  # the oracle is a checker only
-        _log(f"bench: oracle check skipped: {ex}")
+    # one column slice of stripe 0, checked against the CPU oracle in the
+    # cpu_baseline leg
+    cols = 1 << 16
+    sample = [st.shard(0, i)[:cols].cpu().numpy().copy() for i in range(n)]
 
     def timed(step, sync):
         torch.cuda.synchronize()
@@ -274,7 +271,7 @@ def headline(args, torch, rs):
         bytes_per_step=bytes_per_step,
         plain_gbps=round(plain, 1), batched_gbps=round(batched, 1),
         ms_plain=round(t_plain / args.steps * 1e3, 3), ms_batched=round(t_batch / args.steps * 1e3, 3),
-        parity_ok=parity_ok,
+        sample=sample,
         roofline={"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                   "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": measured_traffic(launch_bytes),
                   "kernel": f"batch_pass<4> (RS(16,4) encode of {nst} stripes, {launch_bytes} B per launch)",
@@ -330,6 +327,61 @@ def rs10_4(args, torch, rs):
                encode_pct_hbm_peak=round(100 * nst * n * S / t_enc / 1e9 / HBM_PEAK_GBPS, 2),
                tuning=dict(zip(("tile_bytes", "blocks_per_cu"), benc.tuning())))
     del st, benc, brec
+    torch.cuda.empty_cache()
+    return out
+
+
+def config0(args, torch, rs):
+    """RS(10,3), 10 x 1 MiB data shards (the reference's own CPU-runnable
+    case): one plain C-ABI call each for Encode, Verify and Reconstruct with
+    data shards 3 and 7 erased, timed with HIP events on the codec's stream."""
+    lib = rs.lib
+    k, m, S = 10, 3, 1 << 20
+    n = k + m
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(0xABADC0CAC01A)
+    buf = torch.empty((n, S), dtype=torch.uint8, device="cuda")
+    buf[:k].random_(0, 256, generator=gen)
+    buf[k:].zero_()
+    torch.cuda.synchronize()
+    enc = rs.New(k, m)
+    h = enc.handle
+    base = buf.data_ptr()
+    ptrs = (ctypes.c_void_p * n)(*[base + i * S for i in range(n)])
+    full = (ctypes.c_size_t * n)(*([S] * n))
+    lens = (ctypes.c_size_t * n)(*[0 if i in (3, 7) else S for i in range(n)])
+    filled = (ctypes.c_uint8 * n)()
+    ok = ctypes.c_int()
+    _check(lib, lib.rsx_encode(h, ptrs, full, n))
+    _check(lib, lib.rsx_verify(h, ptrs, full, n, ctypes.byref(ok)))
+    verify_ok = ok.value == 1
+    _check(lib, lib.rsx_synchronize(h))
+    sample = buf.cpu().numpy().copy()  # checked against the oracle in the cpu_baseline leg
+    buf[3].zero_()
+    buf[7].zero_()
+    torch.cuda.synchronize()
+    _check(lib, lib.rsx_reconstruct(h, ptrs, lens, n, filled))
+    _check(lib, lib.rsx_synchronize(h))
+    rebuilt_ok = bool(np.array_equal(buf.cpu().numpy(), sample)) and list(filled) == [
+        1 if i in (3, 7) else 0 for i in range(n)]
+    stream = lib.rsx_get_stream(h)
+    reps = 50
+    times = {}
+    for name, fn in (("encode", lambda: _check(lib, lib.rsx_encode(h, ptrs, full, n))),
+                     ("verify", lambda: _check(lib, lib.rsx_verify(h, ptrs, full, n, ctypes.byref(ok)))),
+                     ("reconstruct", lambda: _check(lib, lib.rsx_reconstruct(h, ptrs, lens, n, filled)))):
+        for _ in range(5):
+            fn()
+        times[name] = event_time(torch, stream, fn, reps)
+    b = n * S
+    out = dict(k=k, m=m, shard_bytes=S, erased=[3, 7], verify_ok=verify_ok, reconstruct_equals_original=rebuilt_ok)
+    for name, t in times.items():
+        out[name] = {"us": round(t * 1e6, 1), "gbps": round(b / t / 1e9, 1),
+                     "pct_hbm_peak": round(100.0 * b / t / 1e9 / HBM_PEAK_GBPS, 2)}
+    _log(f"bench: RS(10,3) 1 MiB plain: encode {out['encode']['gbps']} GB/s, verify {out['verify']['gbps']} GB/s, "
+         f"reconstruct(3,7) {out['reconstruct']['gbps']} GB/s")
+    out["sample"] = sample
+    del buf
     torch.cuda.empty_cache()
     return out
 
@@ -433,7 +485,19 @@ def leopard(args, torch, rs):
     lens = (ctypes.c_size_t * n)(*[0 if i in lost else S for i in range(n)])
     filled = (ctypes.c_uint8 * n)()
     _check(lib, lib.rsx_encode(h, ptrs, full, n))
+    _check(lib, lib.rsx_synchronize(h))
+    # the first 16 KiB of every shard (whole 64-byte chunks encode on their
+    # own), checked against the CPU oracle in the cpu_baseline leg
+    sample_cols = 16 << 10
+    sample = buf[:, :sample_cols].cpu().numpy().copy()
+    want = buf.clone()
+    for i in lost:
+        buf[i].zero_()
+    torch.cuda.synchronize()
     _check(lib, lib.rsx_reconstruct(h, ptrs, lens, n, filled))
+    _check(lib, lib.rsx_synchronize(h))
+    rebuilt_ok = bool(torch.equal(buf, want))
+    del want
     reps = 3
     t0 = time.perf_counter()
     for _ in range(reps):
@@ -448,7 +512,9 @@ def leopard(args, torch, rs):
     out = dict(k=k, m=m, shard_bytes=S, erasures=m, encode_ms=round(t_enc * 1e3, 2),
                reconstruct_ms=round(t_rec * 1e3, 2), encode_gbps=round(b / t_enc / 1e9, 1),
                reconstruct_gbps=round(b / t_rec / 1e9, 1),
-               encode_plus_reconstruct_gbps=round(2 * b / (t_enc + t_rec) / 1e9, 1))
+               encode_plus_reconstruct_gbps=round(2 * b / (t_enc + t_rec) / 1e9, 1),
+               reconstruct_equals_original=rebuilt_ok,
+               sample=dict(data=sample, lost=sorted(lost), k=k, m=m))
     del buf
     torch.cuda.empty_cache()
     return out
@@ -582,24 +648,25 @@ def stream(args, torch, rs):
                 encode_gbps=round(b / t_enc / 1e9, 2), note="host memory streams via Python file objects")
 
 
-def cpu_baseline(seconds: float, shard_bytes: int):
-    """The oracle's SIMD restatement of the reference CPU path on the host
-    cores: RS(16,4) encode (16 -> 4) and an equal-cost reconstruct product
-    (16 survivors -> 4 rebuilt) on one stripe, alternated for ~`seconds`."""
+def _cpu_model():
     try:
-        from oracle import rsref
-    except Exception as ex:
-        _log(f"bench: cpu baseline skipped: {ex}")
-        return None
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
-    k, m = 16, 4
-    S = shard_bytes
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return ""
+
+
+def _cpu_rate(rsref, k, m, S, threads, isa, seconds):
+    """GB/s ((k+m)*S per product) of the oracle's SIMD restatement alternating
+    an encode product (k -> m) and an equal-shaped reconstruct product."""
     rng = np.random.default_rng(1)
     data = [rng.integers(0, 256, S, dtype=np.uint8) for _ in range(k)]
     outs = [np.zeros(S, np.uint8) for _ in range(m)]
     rows_enc = rsref.codec_matrix(k, m)[k:]
     rows_rec = rng.integers(1, 256, (m, k), dtype=np.uint8)
-    isa = rsref.fast_isa()
     rsref.code_some_shards_fast(rows_enc, data, outs, threads, isa)
     t0 = time.perf_counter()
     reps = 0
@@ -607,20 +674,72 @@ def cpu_baseline(seconds: float, shard_bytes: int):
         rsref.code_some_shards_fast(rows_enc if reps % 2 == 0 else rows_rec, data, outs, threads, isa)
         reps += 1
     el = time.perf_counter() - t0
-    cpu_model = ""
+    return reps * (k + m) * S / el / 1e9, reps, el
+
+
+def cpu_baseline(args, head_sample, c0, leo):
+    """The CPU leg.  The oracle checks the GPU's bytes (64 KiB of every shard of
+    the headline's stripe 0, the RS(10,3) stripe, 16 KiB of every Leopard
+    shard) and, unless --no-cpu, times the oracle's restatement of the
+    reference CPU path on the host cores.  Returns (baseline, checks)."""
     try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    cpu_model = line.split(":", 1)[1].strip()
-                    break
-    except OSError:
-        pass
-    return {"value": round(reps * (k + m) * S / el / 1e9, 2), "unit": "GB/s", "cores": threads,
-            "kind": "port",
+        from oracle import rsref
+    except Exception as ex:
+        _log(f"bench: cpu leg skipped: {ex}")
+        return None, {}
+    checks = {}
+    if head_sample is not None:
+        k, m = 16, 4
+        want = [r.copy() for r in head_sample[:k]] + [np.zeros(len(head_sample[0]), np.uint8) for _ in range(m)]
+        rsref.Classic(k, m).encode(want)
+        checks["rs16_4"] = all(np.array_equal(want[k + i], head_sample[k + i]) for i in range(m))
+    if c0:
+        smp, k, m = c0["sample"], c0["k"], c0["m"]
+        want = [smp[i].copy() for i in range(k)] + [np.zeros(smp.shape[1], np.uint8) for _ in range(m)]
+        rsref.Classic(k, m).encode(want)
+        checks["rs10_3"] = (all(np.array_equal(want[k + i], smp[k + i]) for i in range(m))
+                            and c0["verify_ok"] and c0["reconstruct_equals_original"])
+    leo_cpu = None
+    if leo:
+        smp = leo["sample"]
+        d, k, m = smp["data"], smp["k"], smp["m"]
+        n, cols = k + m, d.shape[1]
+        o = rsref.Leopard(k, m, 16)
+        want = [d[i].copy() for i in range(k)] + [np.zeros(cols, np.uint8) for _ in range(m)]
+        t0 = time.perf_counter()
+        o.encode(want)
+        t_enc = time.perf_counter() - t0
+        enc_ok = all(np.array_equal(want[k + i], d[k + i]) for i in range(m))
+        lost = set(smp["lost"])
+        shards = [None if i in lost else want[i] for i in range(n)]
+        t0 = time.perf_counter()
+        rc = o.reconstruct(shards)
+        t_rec = time.perf_counter() - t0
+        rec_ok = rc == 0 and all(np.array_equal(shards[i], d[i]) for i in range(n))
+        checks["leopard16"] = enc_ok and rec_ok and leo["reconstruct_equals_original"]
+        leo_cpu = {"encode_gbps": round(n * cols / t_enc / 1e9, 3),
+                   "reconstruct_gbps": round(n * cols / t_rec / 1e9, 3), "cores": 1, "kind": "port",
+                   "sample": f"k={k} m={m}, the first {cols} B of every shard, {len(lost)} erasures; "
+                             "scalar C restatement of leopard.go, 1 thread (the reference runs Leopard "
+                             "single-threaded, with SIMD)"}
+    if args.no_cpu:
+        return None, checks
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
+    isa = rsref.fast_isa()
+    isa_name = ["scalar", "AVX2 PSHUFB", "AVX-512 GFNI"][isa]
+    S = int(args.shard_mib * (1 << 20)) // 64 * 64
+    rate, reps, el = _cpu_rate(rsref, 16, 4, S, threads, isa, args.cpu_seconds)
+    rate1, _, _ = _cpu_rate(rsref, 16, 4, S, 1, isa, 2.0)
+    c0_rate, _, _ = _cpu_rate(rsref, 10, 3, 1 << 20, threads, isa, 2.0)
+    c0_rate1, _, _ = _cpu_rate(rsref, 10, 3, 1 << 20, 1, isa, 2.0)
+    return {"value": round(rate, 2), "unit": "GB/s", "cores": threads, "kind": "port",
             "sample": f"{reps} alternating RS(16,4) encode / 4-erasure reconstruct products of one "
-                      f"stripe of {S} B shards in {el:.1f} s; {['scalar', 'AVX2 PSHUFB', 'AVX-512 GFNI'][isa]} "
-                      f"restatement of the reference kernels, {threads} threads, {cpu_model}"}
+                      f"stripe of {S} B shards in {el:.1f} s; {isa_name} restatement of the reference "
+                      f"kernels, {threads} threads, {_cpu_model()}",
+            "one_thread_gbps": round(rate1, 2),
+            "rs10_3_1mib": {"gbps": round(c0_rate, 2), "cores": threads, "one_thread_gbps": round(c0_rate1, 2),
+                            "sample": "alternating RS(10,3) encode / 3-row products of one 10 x 1 MiB stripe, 2 s"},
+            "leopard16": leo_cpu}, checks
 
 
 def main() -> int:
@@ -634,21 +753,29 @@ def main() -> int:
 
     torch.cuda.set_device(0)
     if args.only:
-        fn = {"rs10_4": rs10_4, "mixed": mixed, "leopard16": leopard, "served": served, "stream": stream}[args.only]
-        print(json.dumps({"section": args.only, **fn(args, torch, rs)}), flush=True)
+        fn = {"rs10_4": rs10_4, "rs10_3": config0, "mixed": mixed, "leopard16": leopard, "served": served,
+              "stream": stream}[args.only]
+        sec = fn(args, torch, rs)
+        sec.pop("sample", None)
+        print(json.dumps({"section": args.only, **sec}), flush=True)
         return 0
     head = headline(args, torch, rs)
     if args.encode_only:
+        head.pop("sample", None)
         print(json.dumps({"kernel": "batch_pass<4>", **head}), flush=True)
         return 0
-    mix = leo = srv = r104 = strm = None
+    mix = leo = srv = r104 = c0 = strm = None
     if not args.quick:
         r104 = rs10_4(args, torch, rs)
+        c0 = config0(args, torch, rs)
         mix = mixed(args, torch, rs)
         leo = leopard(args, torch, rs)
         srv = served(args, torch, rs)
         strm = stream(args, torch, rs)
-    cpu = None if args.no_cpu else cpu_baseline(args.cpu_seconds, 4 << 20)
+    cpu, checks = cpu_baseline(args, head.pop("sample", None), c0, leo)
+    for sec in (c0, leo):
+        if sec:
+            sec.pop("sample", None)
     out = {
         "metric": METRIC,
         "value": head["batched_gbps"],
@@ -674,11 +801,13 @@ def main() -> int:
                   "pct_hbm_peak": round(100.0 * head["plain_gbps"] / HBM_PEAK_GBPS, 2),
                   "path": "one C-ABI call per stripe"},
         "batched_over_plain": round(head["batched_gbps"] / head["plain_gbps"], 2),
-        "parity_check_vs_oracle": head["parity_ok"],
+        "parity_check_vs_oracle": checks.get("rs16_4"),
+        "oracle_checks": checks,
         "roofline": head["roofline"],
         "plain_encode_launch": head["plain_encode_launch"],
         "tuning": head["tuning"],
         "rs10_4_1mib": r104,
+        "rs10_3_1mib": c0,
         "mixed_workload": mix,
         "leopard16": leo,
         "served": srv,

@@ -45,6 +45,8 @@ struct rsx_batch {
         uint32_t in_off, out_off;
         uint16_t k, nout, mtot, o0;
         uint8_t aligned;
+        uint8_t mode;   // Gf8Mode
+        uint32_t item;  // compare mode: the Verify item
     };
     std::vector<Task> tasks[9];          // by output-count class 1..8
     std::vector<const uint8_t *> pool;   // pointer pool
@@ -56,8 +58,10 @@ struct rsx_batch {
     bool planned = false;
     bool use_graph = true;
 
-    // Verify items: recomputed parity goes to `scratch`, compared per item
-    // into `d_flags`, which is copied to the pinned `h_flags` at the end.
+    // Verify items set d_flags[item] on a mismatch, copied to the pinned
+    // h_flags at the end: classic items in their compare-mode pass, Leopard
+    // items by recomputing parity into `scratch` and comparing (`cmp`).
+    bool has_verify = false;
     std::vector<CompareEntry> cmp;
     DeviceBuffer d_cmp, d_flags, scratch;
     uint32_t *h_flags = nullptr;
@@ -108,8 +112,9 @@ int build_launches(rsx_batch *b, uint32_t tile) {
             d.nout = t.nout;
             d.mtot = t.mtot;
             d.o0 = t.o0;
-            d.accumulate = 0;
+            d.mode = t.mode;
             d.aligned = t.aligned;
+            d.item = t.item;
             all.push_back(d);
             t0 += (t.len + tile - 1) / tile;
             maxk = std::max<int>(maxk, t.k);
@@ -129,13 +134,14 @@ int build_launches(rsx_batch *b, uint32_t tile) {
     for (auto &L : b->launches) {
         L.tasks = b->d_tasks.as<BatchTask>() + reinterpret_cast<size_t>(L.tasks);
         L.ptrs = b->d_ptrs.as<const uint8_t *>();
+        L.flags = b->d_flags.as<uint32_t>();
     }
     return RSX_OK;
 }
 
 int enqueue(rsx_batch *b, hipStream_t s) {
     const unsigned grid = static_cast<unsigned>(std::max(1, b->cus * b->blocks_per_cu));
-    const bool verify = !b->cmp.empty();
+    const bool verify = b->has_verify;
     if (verify)
         RSX_HIP(launch_fill_u32(b->d_flags.as<uint32_t>(), static_cast<size_t>(b->nitems), 0u, s));
     const bool leo = !b->leo_tasks[0].empty() || !b->leo_tasks[1].empty();
@@ -161,8 +167,9 @@ int enqueue(rsx_batch *b, hipStream_t s) {
     if (fork) RSX_HIP(hipStreamWaitEvent(s, b->join, 0));
 
     if (verify) {
-        RSX_HIP(launch_batch_compare(b->d_cmp.as<CompareEntry>(), static_cast<int>(b->cmp.size()),
-                                     b->d_flags.as<uint32_t>(), s));
+        if (!b->cmp.empty())
+            RSX_HIP(launch_batch_compare(b->d_cmp.as<CompareEntry>(), static_cast<int>(b->cmp.size()),
+                                         b->d_flags.as<uint32_t>(), s));
         RSX_HIP(hipMemcpyAsync(b->h_flags, b->d_flags.ptr, sizeof(uint32_t) * b->nitems,
                                hipMemcpyDeviceToHost, s));
     }
@@ -291,6 +298,7 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
     for (auto &v : b->tasks) v.clear();
     b->pool.clear();
     b->planned = false;
+    b->has_verify = false;
     b->cmp.clear();
     b->nitems = n;
     b->is_verify.assign(static_cast<size_t>(std::max(n, 0)), 0);
@@ -390,23 +398,10 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
                 continue;
             }
             for (int j = 0; j < k; ++j) in.push_back(item.shards[j]);
-            if (item.op == RSX_OP_VERIFY) {
-                // Parity is recomputed into plan scratch (pointers patched once
-                // the scratch buffer is sized) and compared after the passes.
-                const size_t len = lens[0];
-                const size_t pitch = (len + 255) / 256 * 256;
-                VerifyOut vo{static_cast<size_t>(it), {}, {}, len};
-                for (int i = k; i < total; ++i) {
-                    vo.slots.push_back(scratch_bytes);
-                    vo.parity.push_back(item.shards[i]);
-                    out.push_back(nullptr);  // patched to scratch once it is sized
-                    scratch_bytes += pitch;
-                }
-                vouts.push_back(std::move(vo));
-                b->is_verify[it] = 1;
-            } else {
-                for (int i = k; i < total; ++i) out.push_back(item.shards[i]);
-            }
+            // Verify: the pass recomputes the parity in registers and
+            // compares it with these shards (compare mode) instead of writing.
+            for (int i = k; i < total; ++i) out.push_back(item.shards[i]);
+            if (item.op == RSX_OP_VERIFY) b->is_verify[it] = 1;
             tables = c->parity_tables_host.data();
             mtot = c->m;
         } else if (item.op == RSX_OP_RECONSTRUCT || item.op == RSX_OP_RECONSTRUCT_DATA) {
@@ -458,26 +453,16 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
             bool missing = false;
             for (const uint8_t *p : in) missing |= p == nullptr;
             for (size_t i = 0; i < out.size(); ++i) {
-                if (b->is_verify[it]) continue;
                 missing |= out[i] == nullptr;
                 ptrs.push_back(out[i]);
             }
-            if (item.op == RSX_OP_VERIFY)
-                for (int i = k; i < total; ++i) {
-                    missing |= item.shards[i] == nullptr;
-                    ptrs.push_back(item.shards[i]);
-                }
             bool host = false;
             if (!missing)
                 for (const void *p : ptrs) host |= !ranges.is_device(p);
             if (missing) err_item = fail(RSX_ERR_INVALID_INPUT, "no buffer for a shard of a batched item");
             else if (host) err_item = fail(RSX_ERR_MIXED_MEMORY, "batched items need shards in device memory");
             if (err_item) {
-                if (b->is_verify[it] == 1) {
-                    scratch_bytes -= vouts.back().slots.size() * ((lens[0] + 255) / 256 * 256);
-                    vouts.pop_back();
-                    b->is_verify[it] = 0;
-                }
+                b->is_verify[it] = 0;
                 if (fl) std::memset(fl, 0, c->total);
                 status[it] = err_item;
                 continue;
@@ -499,10 +484,7 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
         for (int o0 = 0; o0 < mtot; o0 += 8) {
             const int nout = std::min(8, mtot - o0);
             const uint32_t out_off = static_cast<uint32_t>(b->pool.size());
-            for (int i = 0; i < nout; ++i) {
-                if (b->is_verify[it]) scratch_patch.emplace_back(b->pool.size(), vouts.back().slots[o0 + i]);
-                b->pool.push_back(out[o0 + i]);
-            }
+            for (int i = 0; i < nout; ++i) b->pool.push_back(out[o0 + i]);
             rsx_batch::Task t{};
             t.tables = nullptr;
             t.len = len;
@@ -513,11 +495,13 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
             t.mtot = static_cast<uint16_t>(mtot);
             t.o0 = static_cast<uint16_t>(o0);
             t.aligned = (bits & 15) == 0;
+            t.mode = item.op == RSX_OP_VERIFY ? kGf8Compare : kGf8Clear;
+            t.item = static_cast<uint32_t>(it);
             b->tasks[nout].push_back(t);
             pend.push_back({nout, b->tasks[nout].size() - 1, toff});
         }
     }
-    // Scratch for verify items; patch the placeholder output pointers.
+    // Scratch for Leopard verify items; patch the placeholder output pointers.
     if (!vouts.empty()) {
         int e2 = b->scratch.reserve(std::max<size_t>(scratch_bytes, 256));
         if (e2) return e2;
@@ -530,7 +514,11 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
         if (e3) return e3;
         RSX_HIP(hipMemcpyAsync(b->d_cmp.ptr, b->cmp.data(), b->cmp.size() * sizeof(CompareEntry),
                                hipMemcpyHostToDevice, b->stream));
-        if ((e3 = b->d_flags.reserve(sizeof(uint32_t) * std::max(n, 1)))) return e3;
+    }
+    for (int it = 0; it < n; ++it) b->has_verify |= b->is_verify[it] == 1;
+    if (b->has_verify) {
+        int e3 = b->d_flags.reserve(sizeof(uint32_t) * std::max(n, 1));
+        if (e3) return e3;
         if (b->h_flags_cap < static_cast<size_t>(n)) {
             if (b->h_flags) (void)hipHostFree(b->h_flags);
             b->h_flags = nullptr;

@@ -18,21 +18,23 @@ struct BatchTask {
     uint32_t in_off;         // k input pointers at ptrs[in_off]
     uint32_t out_off;        // nout output pointers at ptrs[out_off]
     uint16_t k, nout, mtot, o0;
-    uint8_t accumulate;      // 0: out = M*in, 1: out ^= M*in
+    uint8_t mode;            // Gf8Mode: clear, accumulate, or compare (Verify)
     uint8_t aligned;         // every pointer 16-byte aligned
-    uint8_t pad[6];
+    uint8_t pad[2];
+    uint32_t item;           // compare mode: flags[item] = 1 where out != M*in
 };
 
 struct BatchLaunch {
     const BatchTask *tasks;
     const uint8_t *const *ptrs;
+    uint32_t *flags;  // per item: Verify mismatch
     int ntasks;
     int maxk;       // largest k among the tasks (sizes the LDS tables)
     uint32_t ntiles;
     uint32_t tile;  // bytes per tile (multiple of 16)
 };
 
-// Verify items: recomputed parity `a` vs the item's parity `b`.
+// Leopard Verify items: recomputed parity `a` vs the item's parity `b`.
 struct CompareEntry {
     const uint8_t *a;
     const uint8_t *b;

@@ -55,7 +55,8 @@ __global__ __launch_bounds__(kBlock) void batch_pass(const BatchLaunch L) {
         }
         const uint8_t *const *in = L.ptrs + task.in_off;
         uint8_t *const *out = (uint8_t *const *)(L.ptrs + task.out_off);
-        const bool acc = task.accumulate != 0;
+        const int mode = task.mode;
+        uint32_t *flag = mode == kGf8Compare ? L.flags + task.item : nullptr;
         const uint64_t b0 = static_cast<uint64_t>(t - task.tile0) * L.tile;
         const uint64_t b1 = b0 + L.tile < task.len ? b0 + L.tile : task.len;
         uint64_t vend = b0 >> 4;  // first vector not done by the vector paths
@@ -63,17 +64,18 @@ __global__ __launch_bounds__(kBlock) void batch_pass(const BatchLaunch L) {
             const uint64_t v0 = b0 >> 4, v1 = b1 >> 4;
             const uint64_t nchunk = (v1 - v0) / per_chunk;
             for (uint64_t c = wave; c < nchunk; c += kBlock / 64)
-                chunk<M, R, Shape<M>::G>(in, out, k, task.nout, acc, tab, v0 + c * per_chunk, lane);
+                chunk<M, R, Shape<M>::G>(in, out, k, task.nout, mode, tab, v0 + c * per_chunk, lane, flag, 1u);
             for (uint64_t v = v0 + nchunk * per_chunk + threadIdx.x; v < v1; v += kBlock)
-                vector1<M>(in, out, k, task.nout, acc, tab, v);
+                vector1<M>(in, out, k, task.nout, mode, tab, v, flag, 1u);
             vend = v1;
         }
         // Bytes past the last whole vector (every byte of an unaligned task).
         for (uint64_t b = (vend << 4) + threadIdx.x; b < b1; b += kBlock) {
             for (int i = 0; i < task.nout; ++i) {
-                uint32_t a = acc ? out[i][b] : 0u;
+                uint32_t a = mode == kGf8Accumulate ? out[i][b] : 0u;
                 for (int j = 0; j < k; ++j) a ^= mul_byte(in[j][b], lo[j * M + i], hi[j * M + i]);
-                out[i][b] = static_cast<uint8_t>(a);
+                if (mode != kGf8Compare) out[i][b] = static_cast<uint8_t>(a);
+                else if (out[i][b] != static_cast<uint8_t>(a)) *flag = 1u;
             }
         }
     }

@@ -128,7 +128,7 @@ int classify_all(const std::vector<const void *> &ptrs, MemKind &kind) {
 // Runs out = rows * in (or out ^= rows * in) in passes of at most kMaxOut
 // outputs.  `tables` is [nin][nrows][5].
 int run_matmul(rsx_codec *c, const uint32_t *tables, const std::vector<const uint8_t *> &in,
-               const std::vector<uint8_t *> &out, size_t len, bool accumulate) {
+               const std::vector<uint8_t *> &out, size_t len, int mode, uint32_t *flag, uint32_t flag_value) {
     const int nin = static_cast<int>(in.size());
     const int nrows = static_cast<int>(out.size());
     if (nin > kMaxIn) return fail(RSX_ERR_INVALID_INPUT, "too many inputs for one pass");
@@ -142,7 +142,9 @@ int run_matmul(rsx_codec *c, const uint32_t *tables, const std::vector<const uin
         p.k = nin;
         p.mtot = nrows;
         p.o0 = o0;
-        p.accumulate = accumulate ? 1 : 0;
+        p.mode = mode;
+        p.flag = flag;
+        p.flag_value = flag_value;
         RSX_HIP(launch_gf8_pass(p, c->stream));
     }
     return RSX_OK;
@@ -276,6 +278,7 @@ void rsx_free(rsx_codec *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     c->decode_cache.clear();
     c->leo.reset();
+    if (c->host_flag) (void)hipHostFree(c->host_flag);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -335,7 +338,7 @@ int rsx_encode(rsx_codec *c, uint8_t *const *shards, const size_t *lens, int nsh
     if (kind == MemKind::Device) {
         std::vector<const uint8_t *> in(shards, shards + c->k);
         std::vector<uint8_t *> out(shards + c->k, shards + c->total);
-        return run_matmul(c, tab, in, out, len, false);
+        return run_matmul(c, tab, in, out, len, kGf8Clear);
     }
     std::lock_guard<std::mutex> lock(c->mu);
     std::vector<int> load(c->total, -1), slots(c->total);
@@ -347,7 +350,7 @@ int rsx_encode(rsx_codec *c, uint8_t *const *shards, const size_t *lens, int nsh
     if ((err = stage_in(c, len, load, slots, shards, dev, c->total))) return err;
     std::vector<const uint8_t *> in(dev.begin(), dev.begin() + c->k);
     std::vector<uint8_t *> out(dev.begin() + c->k, dev.end());
-    if ((err = run_matmul(c, tab, in, out, len, false))) return err;
+    if ((err = run_matmul(c, tab, in, out, len, kGf8Clear))) return err;
     for (int i = 0; i < c->m; ++i)
         RSX_HIP(hipMemcpyAsync(shards[c->k + i], out[i], len, hipMemcpyDeviceToHost, c->stream));
     RSX_HIP(hipStreamSynchronize(c->stream));
@@ -374,7 +377,7 @@ int rsx_encode_idx(rsx_codec *c, const uint8_t *data, size_t len, int idx, uint8
     if (kind == MemKind::Device) {
         std::vector<const uint8_t *> in{data};
         std::vector<uint8_t *> out(parity, parity + nparity);
-        return run_matmul(c, tab, in, out, len, true);
+        return run_matmul(c, tab, in, out, len, kGf8Accumulate);
     }
     std::lock_guard<std::mutex> lock(c->mu);
     std::vector<uint8_t *> all(parity, parity + nparity);
@@ -385,7 +388,7 @@ int rsx_encode_idx(rsx_codec *c, const uint8_t *data, size_t len, int idx, uint8
     if ((err = stage_in(c, len, load, slots, all.data(), dev, nparity + 1))) return err;
     std::vector<const uint8_t *> in{dev[nparity]};
     std::vector<uint8_t *> out(dev.begin(), dev.begin() + nparity);
-    if ((err = run_matmul(c, tab, in, out, len, true))) return err;
+    if ((err = run_matmul(c, tab, in, out, len, kGf8Accumulate))) return err;
     for (int i = 0; i < nparity; ++i)
         RSX_HIP(hipMemcpyAsync(parity[i], out[i], len, hipMemcpyDeviceToHost, c->stream));
     RSX_HIP(hipStreamSynchronize(c->stream));
@@ -415,24 +418,34 @@ int rsx_verify(rsx_codec *c, uint8_t *const *shards, const size_t *lens, int nsh
         for (int i = 0; i < c->total; ++i) load[i] = slots[i] = i;
         if ((err = stage_in(c, len, load, slots, shards, dev, c->total))) return err;
     }
-    const size_t pitch = align_up(len, kPitchAlign);
-    if ((err = c->scratch.reserve(pitch * c->m))) return err;
-    if ((err = c->flag.reserve(sizeof(int)))) return err;
+    // The parity is recomputed in registers and compared with the stored
+    // parity in the same pass (reedsolomon.go:1373-1387 re-encodes into
+    // temporaries and compares).  A mismatch stores this call's epoch straight
+    // into a pinned host word mapped into the device's address space, so the
+    // flag needs neither a reset nor a copy back; calls on one codec are
+    // serialised by c->mu.
+    if (!c->host_flag) {
+        if (hipHostMalloc(reinterpret_cast<void **>(&c->host_flag), sizeof(uint32_t)) != hipSuccess ||
+            hipHostGetDevicePointer(reinterpret_cast<void **>(&c->dev_flag), c->host_flag, 0) != hipSuccess) {
+            if (c->host_flag) (void)hipHostFree(c->host_flag);
+            c->host_flag = c->dev_flag = nullptr;
+            (void)hipGetLastError();
+            return fail(RSX_ERR_NOMEM, "pinned Verify flag");
+        }
+        *static_cast<volatile uint32_t *>(c->host_flag) = 0;
+        c->verify_epoch = 0;
+    }
+    if (c->verify_epoch == 0xFFFFFFFFu) {  // wrapped: nothing of a running call can still store
+        *static_cast<volatile uint32_t *>(c->host_flag) = 0;
+        c->verify_epoch = 0;
+    }
+    const uint32_t epoch = ++c->verify_epoch;
     std::vector<const uint8_t *> in(dev.begin(), dev.begin() + c->k);
-    std::vector<uint8_t *> out(c->m);
-    for (int i = 0; i < c->m; ++i) out[i] = c->scratch.as<uint8_t>() + pitch * i;
-    if ((err = run_matmul(c, c->parity_tables.as<uint32_t>(), in, out, len, false))) return err;
-    std::vector<CompareEntry> cmp(c->m);
-    for (int i = 0; i < c->m; ++i) cmp[i] = {out[i], dev[c->k + i], len, 0, 0};
-    if ((err = c->cmp.reserve(cmp.size() * sizeof(CompareEntry)))) return err;
-    RSX_HIP(hipMemcpyAsync(c->cmp.ptr, cmp.data(), cmp.size() * sizeof(CompareEntry), hipMemcpyHostToDevice,
-                           c->stream));
-    RSX_HIP(launch_fill_u32(c->flag.as<uint32_t>(), 1, 0u, c->stream));
-    RSX_HIP(launch_batch_compare(c->cmp.as<CompareEntry>(), c->m, c->flag.as<uint32_t>(), c->stream));
-    uint32_t mismatch = 0;
-    RSX_HIP(hipMemcpyAsync(&mismatch, c->flag.ptr, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    std::vector<uint8_t *> out(dev.begin() + c->k, dev.end());
+    if ((err = run_matmul(c, c->parity_tables.as<uint32_t>(), in, out, len, kGf8Compare, c->dev_flag, epoch)))
+        return err;
     RSX_HIP(hipStreamSynchronize(c->stream));
-    *ok = mismatch == 0;
+    *ok = *static_cast<volatile uint32_t *>(c->host_flag) != epoch;
     return RSX_OK;
 }
 
@@ -541,7 +554,7 @@ int reconstruct_classic(rsx_codec *c, uint8_t *const *shards, const size_t *lens
         for (int i : plan->inputs) in.push_back(shards[i]);
         std::vector<uint8_t *> out;
         for (int i : plan->outputs) out.push_back(shards[i]);
-        if ((err = run_matmul(c, tab, in, out, len, false))) return err;
+        if ((err = run_matmul(c, tab, in, out, len, kGf8Clear))) return err;
     } else {
         std::vector<int> load(k + nout, -1), slots(k + nout);
         for (int s = 0; s < k + nout; ++s) slots[s] = s;
@@ -550,7 +563,7 @@ int reconstruct_classic(rsx_codec *c, uint8_t *const *shards, const size_t *lens
         if ((err = stage_in(c, len, load, slots, shards, dev, k + nout))) return err;
         std::vector<const uint8_t *> in(dev.begin(), dev.begin() + k);
         std::vector<uint8_t *> out(dev.begin() + k, dev.end());
-        if ((err = run_matmul(c, tab, in, out, len, false))) return err;
+        if ((err = run_matmul(c, tab, in, out, len, kGf8Clear))) return err;
         for (int s = 0; s < nout; ++s)
             RSX_HIP(hipMemcpyAsync(shards[plan->outputs[s]], out[s], len, hipMemcpyDeviceToHost,
                                    c->stream));

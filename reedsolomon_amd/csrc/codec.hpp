@@ -68,9 +68,11 @@ class DeviceRanges {
 // Shared helpers of the classic engine (codec.cpp).
 size_t shard_size(const size_t *lens, int n);
 int check_shards(const size_t *lens, int n, bool nilok);
-// out = rows * in, or out ^= rows * in when accumulate; `tables` is [nin][nrows][5].
+// out = rows * in (mode 0), out ^= rows * in (mode 1), or *flag = flag_value
+// where out != rows * in (mode 2, out is only read); `tables` is [nin][nrows][5].
 int run_matmul(rsx_codec *c, const uint32_t *tables, const std::vector<const uint8_t *> &in,
-               const std::vector<uint8_t *> &out, size_t len, bool accumulate);
+               const std::vector<uint8_t *> &out, size_t len, int mode, uint32_t *flag = nullptr,
+               uint32_t flag_value = 0);
 // Coefficient tables of `rows` (nrows rows of nin bytes) into dst ([nin][nrows][5]).
 int upload_tables(const std::vector<const uint8_t *> &rows, int nin, DeviceBuffer &dst,
                   std::vector<uint32_t> &host, hipStream_t stream);
@@ -101,9 +103,9 @@ struct rsx_codec {
     std::unordered_map<std::string, std::unique_ptr<rsx::DecodePlan>> decode_cache;
     uint64_t decode_epoch = 0;  // incremented whenever decode_cache is flushed
     rsx::DeviceBuffer staging;  // host-memory calls: shards staged in HBM
-    rsx::DeviceBuffer scratch;  // Verify: recomputed parity
-    rsx::DeviceBuffer flag;     // Verify: mismatch flag
-    rsx::DeviceBuffer cmp;      // Verify: compare list
+    uint32_t *host_flag = nullptr;     // Verify: pinned word, the epoch of the last mismatch
+    uint32_t *dev_flag = nullptr;      // Verify: host_flag's device address
+    uint32_t verify_epoch = 0;         // Verify: value a mismatch of the current call stores
     rsx::DeviceBuffer aux_tables;  // Update / DecodeIdx: per-call coefficient tables
     rsx::DeviceBuffer one_table;   // the table of coefficient 1 (XOR passes)
 

@@ -11,12 +11,16 @@
 // l reads the 16 bytes at offset 16*l of each chunk) and walks the k inputs
 // in groups of G, so each input is read as R KiB contiguous with
 // non-temporal loads (no reuse), and the M outputs stay in VGPRs until one
-// non-temporal store per output.
+// non-temporal store per output.  In compare mode (Verify) the outputs are
+// read instead and checked against the accumulators; a lane that finds a
+// difference stores a caller-chosen value to a flag word.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+
+#include "kernels.hpp"
 
 namespace rsx {
 namespace gf8dev {
@@ -110,10 +114,13 @@ __device__ __forceinline__ void stage_tables(uint4 *lo, uint32_t *hi, const uint
 }
 
 // One wave-chunk: R KiB of every shard starting at 16-byte vector `base`
-// (lane-relative vector base + r*64 + lane).  Outputs i < nout are written.
+// (lane-relative vector base + r*64 + lane).  Outputs i < nout are written
+// (compared in kGf8Compare mode).
 template <int M, int R, int G>
-__device__ __forceinline__ void chunk(const uint8_t *const *in, uint8_t *const *out, int k, int nout,
-                                      bool accumulate, const Tables &t, uint64_t base, int lane) {
+__device__ __forceinline__ void chunk(const uint8_t *const *in, uint8_t *const *out, int k, int nout, int mode,
+                                      const Tables &t, uint64_t base, int lane, uint32_t *flag,
+                                      uint32_t flag_value) {
+    const bool accumulate = mode == kGf8Accumulate;
     uint32_t acc[R][M][4];
 #pragma unroll
     for (int r = 0; r < R; ++r)
@@ -158,6 +165,19 @@ __device__ __forceinline__ void chunk(const uint8_t *const *in, uint8_t *const *
 #pragma unroll
         for (int r = 0; r < R; ++r) mac_one<M>(acc[r], x[r], t, j);
     }
+    if (mode == kGf8Compare) {
+        uint32_t diff = 0;
+#pragma unroll
+        for (int i = 0; i < M; ++i)
+            if (i < nout)
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const uint4 o = load_nt(out[i], base + r * 64 + lane);
+                    diff |= (o.x ^ acc[r][i][0]) | (o.y ^ acc[r][i][1]) | (o.z ^ acc[r][i][2]) | (o.w ^ acc[r][i][3]);
+                }
+        if (diff) *flag = flag_value;  // every writer stores the same value
+        return;
+    }
 #pragma unroll
     for (int i = 0; i < M; ++i)
         if (i < nout)
@@ -168,9 +188,9 @@ __device__ __forceinline__ void chunk(const uint8_t *const *in, uint8_t *const *
 // One 16-byte vector per lane: the R = 1 chunk with lane offset folded into
 // v, inputs fetched eight at a time.
 template <int M>
-__device__ __forceinline__ void vector1(const uint8_t *const *in, uint8_t *const *out, int k, int nout,
-                                        bool accumulate, const Tables &t, uint64_t v) {
-    chunk<M, 1, 8>(in, out, k, nout, accumulate, t, v, 0);
+__device__ __forceinline__ void vector1(const uint8_t *const *in, uint8_t *const *out, int k, int nout, int mode,
+                                        const Tables &t, uint64_t v, uint32_t *flag, uint32_t flag_value) {
+    chunk<M, 1, 8>(in, out, k, nout, mode, t, v, 0, flag, flag_value);
 }
 
 // Streaming shape per output count: more outputs -> fewer chunks per wave,

@@ -52,14 +52,13 @@ __global__ __launch_bounds__(kBlock) void gf8_pass_stream(const Gf8Pass p) {
     const uint64_t per_chunk = static_cast<uint64_t>(R > 0 ? R : 1) * 64;
     const uint64_t nchunk = R > 0 ? nvec / per_chunk : 0;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const bool acc = p.accumulate != 0;
     for (uint64_t c = static_cast<uint64_t>(blockIdx.x) * (kBlock / 64) + wave; c < nchunk;
          c += static_cast<uint64_t>(gridDim.x) * (kBlock / 64))
-        gf8dev::chunk<M, (R > 0 ? R : 1), gf8dev::Shape<M>::G>(p.in, p.out, p.k, p.nout, acc, t, c * per_chunk,
-                                                               lane);
+        gf8dev::chunk<M, (R > 0 ? R : 1), gf8dev::Shape<M>::G>(p.in, p.out, p.k, p.nout, p.mode, t, c * per_chunk,
+                                                               lane, p.flag, p.flag_value);
     for (uint64_t v = nchunk * per_chunk + static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; v < nvec;
          v += static_cast<uint64_t>(gridDim.x) * kBlock)
-        gf8dev::vector1<M>(p.in, p.out, p.k, p.nout, acc, t, v);
+        gf8dev::vector1<M>(p.in, p.out, p.k, p.nout, p.mode, t, v, p.flag, p.flag_value);
 }
 
 // Byte-granular form for the tail of a shard (len % 16) and for shards whose
@@ -69,13 +68,14 @@ __global__ __launch_bounds__(kBlock) void gf8_pass_bytes(const Gf8Pass p, size_t
     for (size_t b = start + static_cast<size_t>(blockIdx.x) * kBlock + threadIdx.x; b < p.len;
          b += stride) {
         for (int i = 0; i < p.nout; ++i) {
-            uint32_t acc = p.accumulate ? p.out[i][b] : 0u;
+            uint32_t acc = p.mode == kGf8Accumulate ? p.out[i][b] : 0u;
             const uint32_t *tab = p.tables + static_cast<size_t>(p.o0 + i) * 5;
             for (int j = 0; j < p.k; ++j, tab += static_cast<size_t>(p.mtot) * 5) {
                 const Split3 s = split(p.in[j][b]);
                 acc ^= mul4(s, tab[0], tab[1], tab[2], tab[3], tab[4]);
             }
-            p.out[i][b] = static_cast<uint8_t>(acc);
+            if (p.mode != kGf8Compare) p.out[i][b] = static_cast<uint8_t>(acc);
+            else if (p.out[i][b] != static_cast<uint8_t>(acc)) *p.flag = p.flag_value;
         }
     }
 }

@@ -13,6 +13,13 @@ namespace rsx {
 constexpr int kMaxIn = 256;   // classic engine: k <= 256
 constexpr int kMaxOut = 8;    // outputs per pass; more outputs => more passes
 
+// What a GF(2^8) pass does with its outputs.
+enum Gf8Mode : int {
+    kGf8Clear = 0,       // out = M*in
+    kGf8Accumulate = 1,  // out ^= M*in
+    kGf8Compare = 2,     // *flag = flag_value where out != M*in; out is not written
+};
+
 struct Gf8Pass {
     const uint8_t *in[kMaxIn];
     uint8_t *out[kMaxOut];
@@ -22,7 +29,9 @@ struct Gf8Pass {
     int mtot;                // outputs described by tables
     int o0;                  // first output of this pass in tables
     int nout;                // outputs of this pass (<= kMaxOut)
-    int accumulate;          // 0: out = M*in (clear); 1: out ^= M*in
+    int mode;                // Gf8Mode
+    uint32_t *flag;          // compare mode: set to flag_value where out != M*in
+    uint32_t flag_value;
 };
 
 // Launches the pass on `stream`.  All pointers must be device pointers.

@@ -103,7 +103,7 @@ int rsx_gal_mul_slice(int device, uint8_t c, const uint8_t *in, uint8_t *out, si
     p.mtot = 1;
     p.o0 = 0;
     p.nout = 1;
-    p.accumulate = do_xor ? 1 : 0;
+    p.mode = do_xor ? kGf8Accumulate : kGf8Clear;
     RSX_HIP(launch_gf8_pass(p, d->stream));
     if (kind == MemKind::Host) RSX_HIP(hipMemcpyAsync(out, dout, len, hipMemcpyDeviceToHost, d->stream));
     RSX_HIP(hipStreamSynchronize(d->stream));

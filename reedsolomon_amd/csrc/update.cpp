@@ -11,6 +11,7 @@
 #include <cstring>
 
 #include "codec.hpp"
+#include "kernels.hpp"
 
 using namespace rsx;
 
@@ -36,7 +37,7 @@ int xor_into(rsx_codec *c, const std::vector<const uint8_t *> &src, const std::v
     if (err) return err;
     for (size_t i = 0; i < src.size(); ++i) {
         if (!len[i]) continue;
-        if ((err = run_matmul(c, one, {src[i]}, {dst[i]}, len[i], true))) return err;
+        if ((err = run_matmul(c, one, {src[i]}, {dst[i]}, len[i], kGf8Accumulate))) return err;
     }
     return RSX_OK;
 }
@@ -126,7 +127,7 @@ int rsx_update(rsx_codec *c, uint8_t *const *shards, const size_t *lens, int nsh
         if (e || m == 0) return e;
         std::vector<const uint8_t *> in(old.begin(), old.end());
         std::vector<uint8_t *> out(d.begin() + 2 * nc, d.end());
-        return run_matmul(c, c->aux_tables.as<uint32_t>(), in, out, size, true);
+        return run_matmul(c, c->aux_tables.as<uint32_t>(), in, out, size, kGf8Accumulate);
     });
 }
 
@@ -274,7 +275,7 @@ int rsx_decode_idx(rsx_codec *c, uint8_t *const *dst, const size_t *dst_lens, in
     err = on_device(c, ptrs, back, size, [&](const std::vector<uint8_t *> &d) {
         std::vector<uint8_t *> out(d.begin(), d.begin() + nout);
         std::vector<const uint8_t *> in(d.begin() + nout, d.end());
-        return run_matmul(c, c->aux_tables.as<uint32_t>(), in, out, size, true);
+        return run_matmul(c, c->aux_tables.as<uint32_t>(), in, out, size, kGf8Accumulate);
     });
     return err;
 }

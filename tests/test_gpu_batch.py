@@ -115,6 +115,53 @@ def test_manual_tuning_gives_same_bytes(tile, bpc):
         assert np.array_equal(shards[i].cpu().numpy(), ref[i])
 
 
+def test_batched_classic_verify_compare_mode():
+    # Verify items run as compare-mode passes (parity recomputed in registers
+    # and checked against the item's parity shards): good and corrupt items of
+    # several shapes, byte tails, unaligned shards and more than 8 parity
+    # shards (two passes) in one plan, replayed with and without the graph.
+    rng = np.random.default_rng(919)
+    shapes = [(10, 4, 1 << 16), (6, 3, 4096 + 21), (5, 2, 1000), (10, 12, 8192), (17, 3, 64 * 33)]
+    items, want = [], []
+    for k, m, size in shapes:
+        enc = rs.New(k, m)
+        data = [rng.integers(0, 256, size, dtype=np.uint8) for _ in range(k)]
+        full = data + [np.zeros(size, np.uint8) for _ in range(m)]
+        rsref.Classic(k, m).encode(full)
+        for corrupt in (None, "parity", "data", "tail"):
+            shift = 3 if size == 1000 else 0  # unaligned shards
+            pool = torch.zeros((k + m) * (size + shift) + shift, dtype=torch.uint8, device="cuda")
+            sh = []
+            for i in range(k + m):
+                t = pool[shift + i * (size + shift):shift + i * (size + shift) + size]
+                t.copy_(torch.from_numpy(full[i]))
+                sh.append(t)
+            if corrupt == "parity":
+                sh[k + m - 1][size // 2] ^= 0x11
+            elif corrupt == "data":
+                sh[0][7] ^= 0x80
+            elif corrupt == "tail":
+                sh[k][size - 1] ^= 0x01
+            items.append((enc, rs.OP_VERIFY, sh))
+            want.append(corrupt is None)
+    # an erroneous verify item (a missing parity shard) beside them, reported
+    # as the plain call reports it
+    bad_enc = rs.New(4, 2)
+    bad_item = [torch.zeros(64, dtype=torch.uint8, device="cuda") for _ in range(5)] + [None]
+    with pytest.raises(rs.ErrShardSize):
+        bad_enc.Verify(bad_item)
+    items.append((bad_enc, rs.OP_VERIFY, bad_item))
+    b = rs.Batch(0)
+    for use_graph in (True, False):
+        b.set_graph(use_graph)
+        for _ in range(2):  # a replay starts from cleared flags
+            status = b.run(items)
+            assert status[:-1] == [0] * (len(items) - 1), status
+            assert status[-1] == rs.ErrShardSize.code
+            for idx, ok in enumerate(want):
+                assert b.verify_result(idx) == ok, idx
+
+
 def _leo_codeword(rng, bits, k, m, size):
     data = [rng.integers(0, 256, size, dtype=np.uint8) for _ in range(k)]
     full = data + [np.zeros(size, np.uint8) for _ in range(m)]

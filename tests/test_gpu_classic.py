@@ -260,6 +260,52 @@ def test_verify_detects_corruption_and_errors():
         enc.Encode([dev(s) for s in full[:k]] + [s.copy() for s in full[k:]])
 
 
+# (k, m, size, byte offset of the shards in their buffer, memory): the fused
+# Verify pass recomputes parity in registers and compares it with the stored
+# parity; corrupt one byte in each region a pass covers and alternate good and
+# corrupt calls, so a flag left over from one call cannot decide the next.
+VERIFY_CASES = [
+    (10, 4, 1 << 20, 0, "dev"),       # wave chunks only
+    (10, 3, 4096 + 16 * 37, 0, "dev"),  # chunks + per-vector tail
+    (6, 3, 4096 + 21, 0, "dev"),      # + byte tail (len % 16)
+    (5, 2, 1000, 3, "dev"),           # unaligned shards: byte path only
+    (10, 12, 8192, 0, "dev"),         # two passes of outputs (8 + 4)
+    (4, 2, 5000, 0, "host"),          # staged through HBM
+]
+
+
+@pytest.mark.parametrize("k,m,size,shift,mem", VERIFY_CASES)
+def test_fused_verify_finds_every_corrupt_byte(k, m, size, shift, mem):
+    rng = np.random.default_rng(size + k + m)
+    data = rand_data(rng, k, size)
+    _, full = oracle_encode(k, m, data)
+    enc = rs.New(k, m)
+    if mem == "host":
+        shards = [s.copy() for s in full]
+    else:
+        pool = torch.zeros((k + m) * (size + shift) + shift, dtype=torch.uint8, device="cuda")
+        shards = []
+        for i in range(k + m):
+            t = pool[shift + i * (size + shift):shift + i * (size + shift) + size]
+            t.copy_(torch.from_numpy(full[i]))
+            shards.append(t)
+    assert enc.Verify(shards)
+    spots = sorted({0, size - 1, size // 2, (size // 16) * 16 - 1 if size >= 16 else 0, size - size % 16})
+    for p in range(m):
+        for b in spots:
+            if b >= size:
+                continue
+            shards[k + p][b] ^= 0x40
+            assert not enc.Verify(shards), (p, b)
+            shards[k + p][b] ^= 0x40
+            assert enc.Verify(shards), (p, b)
+    # a corrupt data byte fails too
+    shards[k // 2][size // 3] ^= 1
+    assert not enc.Verify(shards)
+    shards[k // 2][size // 3] ^= 1
+    assert enc.Verify(shards)
+
+
 def test_rs16_4_4mib_stripes_round_trip():
     # The RS(16,4) 4 MiB workload on a few stripes: the first stripe bit-exact
     # against the oracle, every stripe through encode -> erase -> reconstruct.
