@@ -59,11 +59,10 @@ struct rsx_batch {
     bool use_graph = true;
 
     // Verify items set d_flags[item] on a mismatch, copied to the pinned
-    // h_flags at the end: classic items in their compare-mode pass, Leopard
-    // items by recomputing parity into `scratch` and comparing (`cmp`).
+    // h_flags at the end: the passes recompute the parity in registers (classic)
+    // or LDS (Leopard) and compare it with the item's parity shards.
     bool has_verify = false;
-    std::vector<CompareEntry> cmp;
-    DeviceBuffer d_cmp, d_flags, scratch;
+    DeviceBuffer d_flags;
     uint32_t *h_flags = nullptr;
     size_t h_flags_cap = 0;
     int nitems = 0;
@@ -167,9 +166,6 @@ int enqueue(rsx_batch *b, hipStream_t s) {
     if (fork) RSX_HIP(hipStreamWaitEvent(s, b->join, 0));
 
     if (verify) {
-        if (!b->cmp.empty())
-            RSX_HIP(launch_batch_compare(b->d_cmp.as<CompareEntry>(), static_cast<int>(b->cmp.size()),
-                                         b->d_flags.as<uint32_t>(), s));
         RSX_HIP(hipMemcpyAsync(b->h_flags, b->d_flags.ptr, sizeof(uint32_t) * b->nitems,
                                hipMemcpyDeviceToHost, s));
     }
@@ -299,7 +295,6 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
     b->pool.clear();
     b->planned = false;
     b->has_verify = false;
-    b->cmp.clear();
     b->nitems = n;
     b->is_verify.assign(static_cast<size_t>(std::max(n, 0)), 0);
     for (int fi = 0; fi < 2; ++fi) {
@@ -310,15 +305,6 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
         b->leo_slices[fi] = 0;
     }
     std::vector<uint8_t> laux;  // Leopard error locators / output rows / logs
-    struct VerifyOut {
-        size_t item;
-        std::vector<size_t> slots;  // scratch offsets of the recomputed parity rows
-        std::vector<const uint8_t *> parity;
-        size_t len;
-    };
-    std::vector<VerifyOut> vouts;
-    std::vector<std::pair<size_t, size_t>> scratch_patch;  // (pool index, scratch offset)
-    size_t scratch_bytes = 0;
     // Every table the plan reads is copied into `htab` (host) when first
     // seen and uploaded once; keyed by (host table, codec, decode-cache
     // epoch) so a table freed by a cache flush is never matched again.
@@ -347,7 +333,7 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
         if (fl) std::memset(fl, 0, c->total);
         if (c->engine) {
             // Leopard: one task of the LDS-resident batch kernel; Verify is
-            // an encode into plan scratch plus a compare.
+            // an encode whose parity rows are compared instead of written.
             if (item.op != RSX_OP_ENCODE && item.op != RSX_OP_VERIFY && item.op != RSX_OP_RECONSTRUCT &&
                 item.op != RSX_OP_RECONSTRUCT_DATA) {
                 status[it] = fail(RSX_ERR_INVALID_INPUT, "unknown batch op");
@@ -361,16 +347,7 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
                 continue;
             }
             if (item.op == RSX_OP_VERIFY) {
-                const size_t len = lp.task.enc.size;
-                const size_t pitch = (len + 255) / 256 * 256;
-                VerifyOut vo{static_cast<size_t>(it), {}, {}, len};
-                for (int i = 0; i < c->m; ++i) {
-                    vo.slots.push_back(scratch_bytes);
-                    vo.parity.push_back(item.shards[c->k + i]);
-                    scratch_patch.emplace_back(lp.parity_pool + i, scratch_bytes);
-                    scratch_bytes += pitch;
-                }
-                vouts.push_back(std::move(vo));
+                lp.verify_item = it;
                 b->is_verify[it] = 1;
             }
             if (!lp.empty) b->leo[lp.bits == 16 ? 1 : 0].push_back(lp);
@@ -501,20 +478,6 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
             pend.push_back({nout, b->tasks[nout].size() - 1, toff});
         }
     }
-    // Scratch for Leopard verify items; patch the placeholder output pointers.
-    if (!vouts.empty()) {
-        int e2 = b->scratch.reserve(std::max<size_t>(scratch_bytes, 256));
-        if (e2) return e2;
-        uint8_t *base = b->scratch.as<uint8_t>();
-        for (auto &pp : scratch_patch) b->pool[pp.first] = base + pp.second;
-        for (auto &vo : vouts)
-            for (size_t r = 0; r < vo.slots.size(); ++r)
-                b->cmp.push_back({base + vo.slots[r], vo.parity[r], vo.len, static_cast<uint32_t>(vo.item), 0});
-        int e3 = b->d_cmp.reserve(b->cmp.size() * sizeof(CompareEntry));
-        if (e3) return e3;
-        RSX_HIP(hipMemcpyAsync(b->d_cmp.ptr, b->cmp.data(), b->cmp.size() * sizeof(CompareEntry),
-                               hipMemcpyHostToDevice, b->stream));
-    }
     for (int it = 0; it < n; ++it) b->has_verify |= b->is_verify[it] == 1;
     if (b->has_verify) {
         int e3 = b->d_flags.reserve(sizeof(uint32_t) * std::max(n, 1));
@@ -549,6 +512,10 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
             if (t.op == 0) {
                 t.enc.data = pp;
                 t.enc.parity = b->d_ptrs.as<uint8_t *>() + lp.parity_pool;
+                if (lp.verify_item >= 0) {
+                    t.enc.flag = b->d_flags.as<uint32_t>() + lp.verify_item;
+                    t.enc.flag_value = 1;
+                }
             } else {
                 t.dec.rows = pp;
                 t.dec.out = pp + t.dec.n;

@@ -150,6 +150,31 @@ int run_matmul(rsx_codec *c, const uint32_t *tables, const std::vector<const uin
     return RSX_OK;
 }
 
+int verify_flag_begin(rsx_codec *c, uint32_t **dev_flag, uint32_t *epoch) {
+    if (!c->host_flag) {
+        if (hipHostMalloc(reinterpret_cast<void **>(&c->host_flag), sizeof(uint32_t)) != hipSuccess ||
+            hipHostGetDevicePointer(reinterpret_cast<void **>(&c->dev_flag), c->host_flag, 0) != hipSuccess) {
+            if (c->host_flag) (void)hipHostFree(c->host_flag);
+            c->host_flag = c->dev_flag = nullptr;
+            (void)hipGetLastError();
+            return fail(RSX_ERR_NOMEM, "pinned Verify flag");
+        }
+        *static_cast<volatile uint32_t *>(c->host_flag) = 0;
+        c->verify_epoch = 0;
+    }
+    if (c->verify_epoch == 0xFFFFFFFFu) {  // wrapped; the previous call has synchronised
+        *static_cast<volatile uint32_t *>(c->host_flag) = 0;
+        c->verify_epoch = 0;
+    }
+    *epoch = ++c->verify_epoch;
+    *dev_flag = c->dev_flag;
+    return RSX_OK;
+}
+
+bool verify_flag_mismatch(const rsx_codec *c, uint32_t epoch) {
+    return *static_cast<volatile const uint32_t *>(c->host_flag) == epoch;
+}
+
 // Uploads coefficient tables for `rows` (nrows x nin) into a [nin][nrows][5]
 // device buffer and leaves the same tables in `host`.
 int upload_tables(const std::vector<const uint8_t *> &rows, int nin, DeviceBuffer &dst,
@@ -420,32 +445,15 @@ int rsx_verify(rsx_codec *c, uint8_t *const *shards, const size_t *lens, int nsh
     }
     // The parity is recomputed in registers and compared with the stored
     // parity in the same pass (reedsolomon.go:1373-1387 re-encodes into
-    // temporaries and compares).  A mismatch stores this call's epoch straight
-    // into a pinned host word mapped into the device's address space, so the
-    // flag needs neither a reset nor a copy back; calls on one codec are
-    // serialised by c->mu.
-    if (!c->host_flag) {
-        if (hipHostMalloc(reinterpret_cast<void **>(&c->host_flag), sizeof(uint32_t)) != hipSuccess ||
-            hipHostGetDevicePointer(reinterpret_cast<void **>(&c->dev_flag), c->host_flag, 0) != hipSuccess) {
-            if (c->host_flag) (void)hipHostFree(c->host_flag);
-            c->host_flag = c->dev_flag = nullptr;
-            (void)hipGetLastError();
-            return fail(RSX_ERR_NOMEM, "pinned Verify flag");
-        }
-        *static_cast<volatile uint32_t *>(c->host_flag) = 0;
-        c->verify_epoch = 0;
-    }
-    if (c->verify_epoch == 0xFFFFFFFFu) {  // wrapped: nothing of a running call can still store
-        *static_cast<volatile uint32_t *>(c->host_flag) = 0;
-        c->verify_epoch = 0;
-    }
-    const uint32_t epoch = ++c->verify_epoch;
+    // temporaries and compares); a mismatch stores this call's epoch straight
+    // into the mapped host flag, so there is neither a reset nor a copy back.
+    uint32_t *flag = nullptr, epoch = 0;
+    if ((err = verify_flag_begin(c, &flag, &epoch))) return err;
     std::vector<const uint8_t *> in(dev.begin(), dev.begin() + c->k);
     std::vector<uint8_t *> out(dev.begin() + c->k, dev.end());
-    if ((err = run_matmul(c, c->parity_tables.as<uint32_t>(), in, out, len, kGf8Compare, c->dev_flag, epoch)))
-        return err;
+    if ((err = run_matmul(c, c->parity_tables.as<uint32_t>(), in, out, len, kGf8Compare, flag, epoch))) return err;
     RSX_HIP(hipStreamSynchronize(c->stream));
-    *ok = *static_cast<volatile uint32_t *>(c->host_flag) != epoch;
+    *ok = !verify_flag_mismatch(c, epoch);
     return RSX_OK;
 }
 

@@ -76,6 +76,13 @@ int run_matmul(rsx_codec *c, const uint32_t *tables, const std::vector<const uin
 // Coefficient tables of `rows` (nrows rows of nin bytes) into dst ([nin][nrows][5]).
 int upload_tables(const std::vector<const uint8_t *> &rows, int nin, DeviceBuffer &dst,
                   std::vector<uint32_t> &host, hipStream_t stream);
+// Verify's mismatch flag: a pinned host word mapped into the device's address
+// space.  verify_flag_begin returns its device address and the value a
+// mismatch of this call stores (a per-codec epoch, so the word needs no reset
+// between calls); after the stream is synchronised, verify_flag_mismatch
+// tells whether that value was stored.  Caller holds c->mu.
+int verify_flag_begin(rsx_codec *c, uint32_t **dev_flag, uint32_t *epoch);
+bool verify_flag_mismatch(const rsx_codec *c, uint32_t epoch);
 // Staging slot pointers of `len` bytes in the codec's HBM buffer (nslots slots).
 int stage_slots(rsx_codec *c, size_t len, size_t nslots, std::vector<uint8_t *> &dev);
 

@@ -76,6 +76,10 @@ struct F16 {
         x.lo = xor4(x.lo, y.lo);
         x.hi = xor4(x.hi, y.hi);
     }
+    __device__ static uint32_t differ(const E &x, const E &y) {
+        const uint4 a = xor4(x.lo, y.lo), b = xor4(x.hi, y.hi);
+        return a.x | a.y | a.z | a.w | b.x | b.y | b.z | b.w;
+    }
     // (lo, hi) of one dword pair times the table's element, XORed into (xl, xh).
     __device__ static void mac1(uint32_t &xl, uint32_t &xh, uint32_t L, uint32_t H, ConstU32 *t) {
         const uint32_t a0 = L & 0x07070707u, a1 = (L >> 3) & 0x07070707u, a2 = (L >> 6) & 0x03030303u;
@@ -112,6 +116,10 @@ struct F8 {
     __device__ static void gstore(uint8_t *row, int u, const E &e) { gstore4(row + 16 * u, e); }
     __device__ static E zero() { return make_uint4(0, 0, 0, 0); }
     __device__ static void xor_into(E &x, const E &y) { x = xor4(x, y); }
+    __device__ static uint32_t differ(const E &x, const E &y) {
+        const uint4 a = xor4(x, y);
+        return a.x | a.y | a.z | a.w;
+    }
     __device__ static uint32_t mac1(uint32_t x, uint32_t v, ConstU32 *t) {
         const uint32_t g0 = v & 0x07070707u, g1 = (v >> 3) & 0x07070707u, g2 = (v >> 6) & 0x03030303u;
         return xor3(xor3(x, sel(t[1], t[0], g0), sel(t[3], t[2], g1)), sel(t[4], t[4], g2), 0u);

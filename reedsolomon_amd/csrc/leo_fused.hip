@@ -59,7 +59,8 @@ __device__ __forceinline__ void encode_slice(const LeoDev &f, const LeoEncodeArg
         __syncthreads();  // before the next group's rows (or the FFT) reuse the tile
     }
     fft<F>(w, a.m, a.p, f);
-    store_rows<F>(w, a.parity, nullptr, nullptr, a.p, col, f);
+    if (a.flag) compare_rows<F>(w, a.parity, a.p, col, a.flag, a.flag_value);
+    else store_rows<F>(w, a.parity, nullptr, nullptr, a.p, col, f);
     __syncthreads();  // the next slice reuses the rows
 }
 

@@ -85,6 +85,8 @@ struct LeoEncodeArgs {
     int k, p, m, groups;         // m = ceilPow2(p)
     int slice;                   // bytes per row per workgroup (power of two >= 64; 32 = one
                                  // GF(2^16) unit of a chunk)
+    uint32_t *flag;              // Verify: when set, compare the parity rows with `parity`
+    uint32_t flag_value;         // instead of writing them; a mismatch stores flag_value
 };
 
 struct LeoDecodeArgs {

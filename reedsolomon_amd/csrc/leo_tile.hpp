@@ -238,6 +238,20 @@ __device__ __forceinline__ void store_rows(const Tile &w, uint8_t *const *dst, c
     });
 }
 
+// Verify: compares rows [0, count) of w with dst[q] (+col); a thread that
+// finds a difference stores `value` to *flag (every writer stores the same).
+template <class F>
+__device__ __forceinline__ void compare_rows(const Tile &w, uint8_t *const *dst, int count, size_t col,
+                                             uint32_t *flag, uint32_t value) {
+    uint32_t diff = 0;
+    for_units(w, count, [&](int q, int u) {
+        const typename F::E x = Lds<F>::load(w, q, u);
+        const typename F::E y = F::gload(dst[q] + col, u + w.ubase);
+        diff |= F::differ(x, y);
+    });
+    if (diff) *flag = value;
+}
+
 // Formal derivative in place (leopard.go:619-623): out[t] = in[t] ^ XOR over
 // zero bits b of t (t | 2^b < n) of in[t | 2^b].  Every thread computes its
 // slots into registers, then all store after a barrier.

@@ -205,9 +205,21 @@ int fft(rsx_codec *c, const LeopardState &s, const LeoWork &w, int mtrunc, int m
     return RSX_OK;
 }
 
+// Column slice of the one-LDS-tile encode of `size`-byte shards, or 0 when
+// the shape runs one launch per layer.
+int encode_tile_slice(const rsx_codec *c, size_t size) {
+    const LeopardState &s = *c->leo;
+    const int m = ceil_pow2(c->m);
+    const int groups = (c->k + m - 1) / m;
+    return s.fused ? leo_fused_slice(s.bits, m, 1, m, size, encode_lds_budget(m, groups)) : 0;
+}
+
 // Encode with device pointers (16-byte aligned) for data (k) and parity (p).
+// With `flag` (one-tile shapes only: encode_tile_slice != 0) the parity rows
+// are compared with `parity` instead of written, and a mismatch stores
+// flag_value to *flag.
 int encode_device(rsx_codec *c, const std::vector<uint8_t *> &data, const std::vector<uint8_t *> &parity,
-                  size_t size) {
+                  size_t size, uint32_t *flag = nullptr, uint32_t flag_value = 0) {
     LeopardState &s = *c->leo;
     const int k = c->k, p = c->m;
     const int m = ceil_pow2(p);
@@ -228,9 +240,10 @@ int encode_device(rsx_codec *c, const std::vector<uint8_t *> &data, const std::v
 
     const uint8_t *const *dptr = reinterpret_cast<const uint8_t *const *>(s.ptrs.ptr);
     uint8_t *const *pptr = reinterpret_cast<uint8_t *const *>(s.ptrs.as<uint8_t *>() + static_cast<size_t>(groups) * m);
-    const int slice = s.fused ? leo_fused_slice(s.bits, m, 1, m, size, encode_lds_budget(m, groups)) : 0;
+    const int slice = encode_tile_slice(c, size);
+    if (flag && !slice) return fail(RSX_ERR_INVALID_INPUT, "compare-mode encode needs the one-tile schedule");
     if (slice) {
-        const LeoEncodeArgs a{dptr, pptr, size, k, p, m, groups, slice};
+        const LeoEncodeArgs a{dptr, pptr, size, k, p, m, groups, slice, flag, flag_value};
         RSX_HIP(leo_encode_fused(*s.dev, a, c->stream));
         return RSX_OK;
     }
@@ -471,6 +484,17 @@ int leopard_verify(rsx_codec *c, uint8_t *const *shards, const size_t *lens, int
     std::vector<bool> use(total, true), load(total, true), save(total, false);
     Resolved r;
     if ((err = resolve(c, shards, total, size, use, load, r))) return err;
+    if (encode_tile_slice(c, size)) {
+        // The parity rows are recomputed in LDS and compared with the stored
+        // ones by the encode kernel itself: no temporaries, no compare launch.
+        uint32_t *flag = nullptr, epoch = 0;
+        if ((err = verify_flag_begin(c, &flag, &epoch))) return err;
+        std::vector<uint8_t *> data(r.dev.begin(), r.dev.begin() + k), parity(r.dev.begin() + k, r.dev.end());
+        if ((err = encode_device(c, data, parity, size, flag, epoch))) return err;
+        RSX_HIP(hipStreamSynchronize(c->stream));
+        *ok = !verify_flag_mismatch(c, epoch);
+        return RSX_OK;
+    }
     if ((err = s.scratch.reserve(size * p))) return err;
     if ((err = s.flag.reserve(sizeof(int)))) return err;
     std::vector<uint8_t *> data(r.dev.begin(), r.dev.begin() + k), tmp(p);

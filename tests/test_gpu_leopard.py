@@ -196,6 +196,13 @@ def test_lds_and_layered_schedules_match_oracle(bits, k, m, size, monkeypatch):
         for i in range(m):
             assert np.array_equal(host(shards[k + i]), want[k + i]), (layered, i)
         assert enc.Verify(shards)
+        # one corrupt byte at a time (last parity row's last byte, first parity
+        # byte, a data byte), each followed by a clean call
+        for i, b in ((k + m - 1, size - 1), (k, 0), (k // 2, size // 2)):
+            shards[i][b] ^= 0x5A
+            assert not enc.Verify(shards), (layered, i, b)
+            shards[i][b] ^= 0x5A
+            assert enc.Verify(shards), (layered, i, b)
         work = [None if i in lost else shards[i] for i in range(k + m)]
         enc.Reconstruct(work)
         for i in range(k + m):
