@@ -322,6 +322,12 @@ static void free_rows(uint8_t **w, int rows) {
     free(w);
 }
 
+/* newFF16 (leopard.go:58) only bounds k + m by 65536; with k + ceilPow2(m) >
+ * 65536 the codeword positions leave the field and the encoder indexes past
+ * fftSkew (a panic in the reference).  This restatement reports it instead;
+ * within the bound every skew index the encoder reads is < 65535. */
+static int leo16_shape_ok(int k, int p) { return (long)k + ceil_pow2(p) <= ORDER16; }
+
 /* leopard.go:139-150 Encode + 169-214 encode + 216-266 encodeChunk. */
 int rsref_leo16_encode(int k, int p, uint8_t *const *shards, const size_t *lens, int nshards) {
     init16();
@@ -330,6 +336,7 @@ int rsref_leo16_encode(int k, int p, uint8_t *const *shards, const size_t *lens,
     if (err) return err;
     size_t size = shard_size(lens, nshards);
     if (size % 64) return RSREF_ERR_INVALID_SHARD_SIZE;
+    if (!leo16_shape_ok(k, p)) return RSREF_ERR_MAX_SHARD_NUM;
     int m = ceil_pow2(p);
     int mtrunc = k < m ? k : m;
     int last_count = k % m;
@@ -414,6 +421,7 @@ static int errlocs16(int k, int p, const uint8_t *erased, ffe *errlocs, int *inp
 int rsref_leo16_errlocs(int k, int p, const uint8_t *erased, int recover_all, uint16_t *out) {
     (void)recover_all;
     init16();
+    if (!leo16_shape_ok(k, p)) return RSREF_ERR_MAX_SHARD_NUM;
     int ic;
     return errlocs16(k, p, erased, out, &ic);
 }
@@ -438,6 +446,7 @@ int rsref_leo16_reconstruct(int k, int p, uint8_t *const *shards, const size_t *
     if (present < k) return RSREF_ERR_TOO_FEW_SHARDS;
     size_t size = shard_size(lens, nshards);
     if (size % 64) return RSREF_ERR_INVALID_SHARD_SIZE;
+    if (!leo16_shape_ok(k, p)) return RSREF_ERR_MAX_SHARD_NUM;
 
     int m = ceil_pow2(p), n = ceil_pow2(m + k);
     uint8_t *erased = malloc(total);

@@ -62,3 +62,11 @@ def test_leopard8_invalid_shape_rejected():
     # k + ceilPow2(m) > 256 indexes past the reference's GF(2^8) tables.
     with pytest.raises(rs.ErrMaxShardNum):
         rs.New(200, 56, rs.WithLeopardGF(True))
+
+
+@pytest.mark.parametrize("k,m", [(65000, 536), (1, 65535)])
+def test_leopard16_out_of_field_shape_rejected(k, m):
+    # k + m <= 65536 passes the reference's newFF16 check, but k + ceilPow2(m)
+    # > 65536 indexes past its fftSkew table.
+    with pytest.raises(rs.ErrMaxShardNum):
+        rs.New(k, m, rs.WithLeopardGF16(True))

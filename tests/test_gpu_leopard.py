@@ -170,6 +170,32 @@ def test_leopard8_192_56_64kib():
 # reconstructs) and one launch per butterfly layer (RSX_LEOPARD_LAYERED=1).
 # GF(2^16) with n >= 2048 rows reconstructs in three LDS-tiled passes; 64*600
 # bytes at n = 2048 spans two column chunks of the work arrays.
+@pytest.mark.parametrize("k,m", [(32768, 32768), (64512, 1024)])
+def test_largest_in_field_gf16_shapes(k, m):
+    # k + ceilPow2(m) = 65536: every field position is a shard.  (32768, 32768)
+    # encodes one launch per layer and reconstructs n = 65536 rows in three
+    # passes; (64512, 1024) encodes 63 groups in one LDS tile.
+    size = 128
+    rng = np.random.default_rng(k + m)
+    data = [rng.integers(0, 256, size, dtype=np.uint8) for _ in range(k)]
+    o, want = oracle_encode(16, k, m, data)
+    buf = torch.zeros((k + m, size), dtype=torch.uint8, device="cuda")
+    buf[:k] = torch.from_numpy(np.stack(data)).to("cuda")
+    shards = [buf[i] for i in range(k + m)]
+    enc = rs.New(k, m, opt(16))
+    enc.Encode(shards)
+    got = buf.cpu().numpy()
+    for i in range(m):
+        assert np.array_equal(got[k + i], want[k + i]), i
+    assert enc.Verify(shards)
+    lost = sorted(rng.choice(k + m, m, replace=False).tolist())
+    gone = set(lost)
+    work = [None if i in gone else shards[i] for i in range(k + m)]
+    enc.Reconstruct(work)
+    rebuilt = np.stack([host(work[i]) for i in lost])
+    assert np.array_equal(rebuilt, np.stack([want[i] for i in lost]))
+
+
 SCHEDULE_CASES = [(8, 192, 56, 64 * 37), (8, 17, 3, 64 * 129), (8, 128, 128, 64 * 5), (8, 3, 2, 64 * 1000),
                   (16, 100, 20, 64 * 41), (16, 300, 300, 64 * 3), (16, 33, 7, 64 * 300), (16, 5, 3, 64 * 777),
                   (16, 1000, 100, 64 * 600), (16, 1500, 600, 64 * 5), (16, 2000, 50, 64 * 2),

@@ -145,6 +145,34 @@ def test_leopard_errors():
         assert codec.reconstruct(work) == rsref.ERR_TOO_FEW_SHARDS
 
 
+@pytest.mark.parametrize("k,p", [(65000, 536), (1, 65535), (64513, 1024)])
+def test_leo16_out_of_field_shape_reported(k, p):
+    # newFF16 (leopard.go:58) accepts k + p <= 65536, but with k + ceilPow2(p)
+    # > 65536 the encoder reads past fftSkew (a panic in the reference): the
+    # oracle reports ErrMaxShardNum for every entry point instead.
+    codec = rsref.Leopard(k, p, 16)
+    shards = [np.zeros(64, np.uint8) for _ in range(k + p)]
+    assert codec.encode(shards) == rsref.ERR_MAX_SHARD_NUM
+    assert codec.verify(shards)[0] == rsref.ERR_MAX_SHARD_NUM
+    work = [None] + shards[1:]
+    assert codec.reconstruct(work) == rsref.ERR_MAX_SHARD_NUM
+
+
+def test_leo16_largest_in_field_shape_round_trips():
+    # k + ceilPow2(p) = 65536 exactly: every position of the field is a shard.
+    k, p, size = 32768, 32768, 64
+    rng = np.random.default_rng(65536)
+    shards = [rng.integers(0, 256, size, dtype=np.uint8) for _ in range(k)] + [np.zeros(size, np.uint8)
+                                                                              for _ in range(p)]
+    codec = rsref.Leopard(k, p, 16)
+    assert codec.encode(shards) == 0
+    assert codec.verify(shards) == (0, True)
+    lost = set(rng.choice(k + p, p, replace=False).tolist())
+    work = [None if i in lost else shards[i] for i in range(k + p)]
+    assert codec.reconstruct(work) == 0
+    assert all(np.array_equal(work[i], shards[i]) for i in range(k + p))
+
+
 def test_leo8_invalid_shape_reported():
     # k + ceilPow2(m) > 256 indexes past the reference's tables (a panic there).
     codec = rsref.Leopard(200, 56, 8)
