@@ -169,7 +169,8 @@ int rsx_decode_cache_size(const rsx_codec *c);
 #define RSX_OP_ENCODE 1
 #define RSX_OP_RECONSTRUCT 2
 #define RSX_OP_RECONSTRUCT_DATA 3
-#define RSX_OP_VERIFY 4           /* parity recomputed into plan scratch and compared */
+#define RSX_OP_VERIFY 4           /* parity recomputed in the pass and compared with the item's
+                                     parity shards; nothing is written */
 
 typedef struct rsx_batch_item {
     rsx_codec *codec;       /* codec from rsx_new */
@@ -201,7 +202,10 @@ int rsx_batch_verify_result(const rsx_batch *b, int item, int *ok);
  * 0: launches enqueue the kernels directly (no per-plan instantiate cost). */
 int rsx_batch_set_graph(rsx_batch *b, int enabled);
 /* Tile bytes per work unit and resident blocks per CU; 0, 0 = autotune at
- * the next plan (the choice is kept for later plans of this batch). */
+ * the next plan (the choice is kept for later plans of this batch).  An
+ * autotuning rsx_batch_plan times the plan by running it a few times, so
+ * during that call it reads the items' inputs and writes their outputs (the
+ * bytes a launch writes); set the tuning first where that is not wanted. */
 int rsx_batch_set_tuning(rsx_batch *b, int tile_bytes, int blocks_per_cu);
 int rsx_batch_get_tuning(const rsx_batch *b, int *tile_bytes, int *blocks_per_cu);
 
