@@ -237,13 +237,26 @@ int rsx_server_create(int device, size_t hbm_budget, int max_batch, rsx_server *
     s->device = device;
     s->budget = hbm_budget ? hbm_budget : (size_t{4} << 30);
     s->max_batch = max_batch > 0 ? max_batch : 4096;
+    auto release_slots = [&] {
+        for (Slot &sl : s->slots) {
+            if (sl.ev) (void)hipEventDestroy(sl.ev);
+            rsx_batch_free(sl.b);
+        }
+    };
     for (Slot &sl : s->slots) {
         int rc = rsx_batch_create(device, &sl.b);
-        if (rc) return rc;
+        if (rc) {
+            release_slots();
+            return rc;
+        }
         rsx_batch_set_graph(sl.b, 0);
         rsx_batch_set_tuning(sl.b, 16 << 10, 8);
-        if (hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming) != hipSuccess)
+        if (hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming) != hipSuccess) {
+            sl.ev = nullptr;
+            (void)hipGetLastError();
+            release_slots();
             return rsx::fail(RSX_ERR_DEVICE, "event creation failed");
+        }
     }
     rsx_server *p = s.release();
     p->sched = std::thread(scheduler_loop, p);
@@ -297,9 +310,15 @@ int rsx_server_poll(rsx_server *s, uint64_t ticket, int *done, int *status, int 
 
 int rsx_server_wait(rsx_server *s, uint64_t ticket, int *status, int *verify_ok, uint64_t *latency_ns) {
     std::unique_lock<std::mutex> lock(s->mu);
-    auto it = s->reqs.find(ticket);
-    if (it == s->reqs.end()) return rsx::fail(RSX_ERR_INVALID_INPUT, "unknown ticket");
-    s->done_cv.wait(lock, [&] { return it->second.done; });
+    if (s->reqs.find(ticket) == s->reqs.end()) return rsx::fail(RSX_ERR_INVALID_INPUT, "unknown ticket");
+    // Look the ticket up again after every wake-up: submits may rehash the
+    // map while this thread waits, and another thread may report it first.
+    s->done_cv.wait(lock, [&] {
+        const auto f = s->reqs.find(ticket);
+        return f == s->reqs.end() || f->second.done;
+    });
+    const auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) return rsx::fail(RSX_ERR_INVALID_INPUT, "ticket reported by another call");
     *status = it->second.status;
     *verify_ok = it->second.verify_ok;
     *latency_ns = it->second.latency_ns;
