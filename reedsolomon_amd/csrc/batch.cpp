@@ -50,7 +50,7 @@ struct rsx_batch {
     };
     std::vector<Task> tasks[9];          // by output-count class 1..8
     std::vector<const uint8_t *> pool;   // pointer pool
-    DeviceBuffer d_tasks, d_ptrs, d_tables;
+    DeviceBuffer d_tasks, d_ptrs, d_tables, d_tile_task;
     std::vector<BatchLaunch> launches;   // one per non-empty class
     std::vector<int> classes;
     hipGraph_t graph = nullptr;
@@ -87,9 +87,19 @@ void destroy_graph(rsx_batch *b) {
     b->graph = nullptr;
 }
 
+// Tiles above which a plan finds tasks by binary search instead of keeping
+// a tile -> task table (4 bytes per tile) on the device.
+constexpr uint64_t kTileTableMax = uint64_t{1} << 24;
+
 // Lays the tasks out for one tile size and uploads them.
 int build_launches(rsx_batch *b, uint32_t tile) {
     std::vector<BatchTask> all;
+    std::vector<uint32_t> tile_task;  // per class, concatenated: class-local task index of every tile
+    std::vector<size_t> tt_off;
+    uint64_t total_tiles = 0;
+    for (int mc = 1; mc <= 8; ++mc)
+        for (const auto &t : b->tasks[mc]) total_tiles += (t.len + tile - 1) / tile;
+    const bool use_table = total_tiles <= kTileTableMax;
     b->launches.clear();
     b->classes.clear();
     for (int mc = 1; mc <= 8; ++mc) {
@@ -100,7 +110,10 @@ int build_launches(rsx_batch *b, uint32_t tile) {
         uint64_t t0 = 0;
         int maxk = 1;
         const size_t first = all.size();
+        tt_off.push_back(tile_task.size());
         for (const auto &t : b->tasks[mc]) {
+            if (use_table)
+                tile_task.insert(tile_task.end(), (t.len + tile - 1) / tile, static_cast<uint32_t>(all.size() - first));
             BatchTask d{};
             d.tables = t.tables;
             d.len = t.len;
@@ -129,11 +142,18 @@ int build_launches(rsx_batch *b, uint32_t tile) {
     if (err) return err;
     RSX_HIP(hipMemcpyAsync(b->d_tasks.ptr, all.data(), all.size() * sizeof(BatchTask),
                            hipMemcpyHostToDevice, b->stream));
+    if (use_table) {
+        if ((err = b->d_tile_task.reserve(std::max<size_t>(tile_task.size(), 1) * sizeof(uint32_t)))) return err;
+        RSX_HIP(hipMemcpyAsync(b->d_tile_task.ptr, tile_task.data(), tile_task.size() * sizeof(uint32_t),
+                               hipMemcpyHostToDevice, b->stream));
+    }
     RSX_HIP(hipStreamSynchronize(b->stream));
-    for (auto &L : b->launches) {
+    for (size_t i = 0; i < b->launches.size(); ++i) {
+        BatchLaunch &L = b->launches[i];
         L.tasks = b->d_tasks.as<BatchTask>() + reinterpret_cast<size_t>(L.tasks);
         L.ptrs = b->d_ptrs.as<const uint8_t *>();
         L.flags = b->d_flags.as<uint32_t>();
+        L.tile_task = use_table ? b->d_tile_task.as<uint32_t>() + tt_off[i] : nullptr;
     }
     return RSX_OK;
 }

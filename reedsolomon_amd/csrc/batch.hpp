@@ -27,7 +27,8 @@ struct BatchTask {
 struct BatchLaunch {
     const BatchTask *tasks;
     const uint8_t *const *ptrs;
-    uint32_t *flags;  // per item: Verify mismatch
+    uint32_t *flags;            // per item: Verify mismatch
+    const uint32_t *tile_task;  // task of every tile (null: binary search over tile0)
     int ntasks;
     int maxk;       // largest k among the tasks (sizes the LDS tables)
     uint32_t ntiles;

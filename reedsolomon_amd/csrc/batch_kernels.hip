@@ -5,8 +5,9 @@
 //
 // Work list: tasks of at most M outputs; each task is cut into column tiles
 // of `tile` bytes, numbered by a prefix sum.  Blocks stride over the global
-// tile index, find their task by binary search, and stage the task's
-// coefficient tables into LDS only when the task changes.  Every tile is a
+// tile index, find their task (one load from a tile -> task table, or a
+// binary search over the prefix sums for very large plans), and stage the
+// task's coefficient tables into LDS only when the task changes.  Every tile is a
 // complete sub-problem (the product is column-separable), so tiles need no
 // ordering.
 #include "batch.hpp"
@@ -44,7 +45,7 @@ __global__ __launch_bounds__(kBlock) void batch_pass(const BatchLaunch L) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int cached = -1;
     for (uint32_t t = blockIdx.x; t < L.ntiles; t += gridDim.x) {
-        const int ti = find_task(L.tasks, L.ntasks, t);
+        const int ti = L.tile_task ? static_cast<int>(L.tile_task[t]) : find_task(L.tasks, L.ntasks, t);
         const BatchTask task = L.tasks[ti];
         const int k = task.k;
         if (ti != cached) {
