@@ -147,7 +147,7 @@ int leo_fused_slice(int bits, int rows, int regions, int step_rows, size_t size,
     while (slice > 64 && bytes(slice) > lds_budget) slice /= 2;
     // GF(2^16) tiles of whole chunks that leave room for fewer than four
     // workgroups per CU take one 32-byte unit per workgroup instead.
-    if (bits == 16 && slice == 64 && bytes(64) > (40u << 10) && bytes(32) <= lds_budget) slice = 32;
+    if (bits == 16 && slice >= 64 && bytes(64) > (40u << 10) && bytes(32) <= lds_budget) slice = 32;
     return bytes(slice) <= lds_budget ? slice : 0;
 }
 

@@ -200,7 +200,8 @@ SCHEDULE_CASES = [(8, 192, 56, 64 * 37), (8, 17, 3, 64 * 129), (8, 128, 128, 64 
                   (16, 100, 20, 64 * 41), (16, 300, 300, 64 * 3), (16, 33, 7, 64 * 300), (16, 5, 3, 64 * 777),
                   (16, 1000, 100, 64 * 600), (16, 1500, 600, 64 * 5), (16, 2000, 50, 64 * 2),
                   (16, 1000, 100, 64 * 37), (16, 1100, 1000, 64 * 19),
-                  (16, 2100, 1100, 64 * 2), (16, 1100, 1100, 64 * 3)]
+                  (16, 2100, 1100, 64 * 2), (16, 1100, 1100, 64 * 3), (16, 1000, 3000, 64 * 3),
+                  (16, 1024, 1024, 64 * 21)]
 
 
 @pytest.mark.parametrize("bits,k,m,size", SCHEDULE_CASES)
