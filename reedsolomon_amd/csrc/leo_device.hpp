@@ -39,6 +39,15 @@ __device__ __forceinline__ uint4 gload4(const uint8_t *p) {
 __device__ __forceinline__ void gstore4(uint8_t *p, const uint4 &v) {
     *(GlobalV4 *)(reinterpret_cast<uintptr_t>(p)) = gu32x4{v.x, v.y, v.z, v.w};
 }
+typedef uint32_t gu32x2 __attribute__((ext_vector_type(2)));
+using GlobalV2 = __attribute__((address_space(1))) gu32x2;
+__device__ __forceinline__ uint2 gload2(const uint8_t *p) {
+    const gu32x2 v = *(const GlobalV2 *)(reinterpret_cast<uintptr_t>(p));
+    return make_uint2(v[0], v[1]);
+}
+__device__ __forceinline__ void gstore2(uint8_t *p, const uint2 &v) {
+    *(GlobalV2 *)(reinterpret_cast<uintptr_t>(p)) = gu32x2{v.x, v.y};
+}
 __device__ __forceinline__ ConstU32 *as_const(const uint32_t *p) {
     return (ConstU32 *)(reinterpret_cast<uintptr_t>(p));
 }
@@ -96,6 +105,40 @@ struct F16 {
         mac1(x.lo.y, x.hi.y, y.lo.y, y.hi.y, t);
         mac1(x.lo.z, x.hi.z, y.lo.z, y.hi.z, t);
         mac1(x.lo.w, x.hi.w, y.lo.w, y.hi.w, t);
+    }
+};
+
+// ---- GF(2^16), quarter chunks: 8 elements per unit, the 8 low bytes at
+// offset 8q of a 64-byte chunk and their 8 high bytes 32 further ----
+struct F16Q {
+    struct E {
+        uint2 lo, hi;
+    };
+    static constexpr int kUnitBytes = 16;  // row bytes one unit covers
+    static constexpr int kTW = F16::kTW;
+    static constexpr unsigned kMod = F16::kMod;
+
+    __device__ static size_t offset(int u) { return static_cast<size_t>(u >> 2) * 64 + (u & 3) * 8; }
+    __device__ static E gload(const uint8_t *row, int u) {
+        const uint8_t *c = row + offset(u);
+        return {gload2(c), gload2(c + 32)};
+    }
+    __device__ static void gstore(uint8_t *row, int u, const E &e) {
+        uint8_t *c = row + offset(u);
+        gstore2(c, e.lo);
+        gstore2(c + 32, e.hi);
+    }
+    __device__ static E zero() { return {make_uint2(0, 0), make_uint2(0, 0)}; }
+    __device__ static void xor_into(E &x, const E &y) {
+        x.lo = make_uint2(x.lo.x ^ y.lo.x, x.lo.y ^ y.lo.y);
+        x.hi = make_uint2(x.hi.x ^ y.hi.x, x.hi.y ^ y.hi.y);
+    }
+    __device__ static uint32_t differ(const E &x, const E &y) {
+        return (x.lo.x ^ y.lo.x) | (x.lo.y ^ y.lo.y) | (x.hi.x ^ y.hi.x) | (x.hi.y ^ y.hi.y);
+    }
+    __device__ static void mul_add(E &x, const E &y, ConstU32 *t) {
+        F16::mac1(x.lo.x, x.hi.x, y.lo.x, y.hi.x, t);
+        F16::mac1(x.lo.y, x.hi.y, y.lo.y, y.hi.y, t);
     }
 };
 

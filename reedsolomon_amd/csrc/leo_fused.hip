@@ -6,6 +6,7 @@
 // transform in LDS (radix-4 steps, one barrier each), and writes the output
 // rows once, so HBM sees exactly the algorithmic bytes.
 #include <algorithm>
+#include <type_traits>
 
 #include "leo_tile.hpp"
 
@@ -17,10 +18,15 @@ using namespace leo;
 using namespace leo_tile;
 
 // Tile of the slice that starts `sc` bytes into every shard: a 32-byte
-// GF(2^16) slice is one unit of the 64-byte chunk at sc & ~63.
+// GF(2^16) slice is one unit of the 64-byte chunk at sc & ~63, a 16-byte one
+// (F16Q) one quarter of it.
 template <class F>
 __device__ __forceinline__ Tile slice_tile(uint8_t *lds, int rows, int slice, size_t size, size_t sc,
                                            size_t &col) {
+    if (std::is_same<F, F16Q>::value) {
+        col = sc & ~size_t{63};
+        return Tile{lds, rows + 1, 0, 1, static_cast<int>((sc >> 4) & 3), 1};
+    }
     const int cols = static_cast<int>(size - sc < static_cast<size_t>(slice) ? size - sc : slice);
     const bool unit = slice < 64;
     col = unit ? (sc & ~size_t{63}) : sc;
@@ -82,7 +88,8 @@ __device__ __forceinline__ void decode_slice(const LeoDev &f, const LeoDecodeArg
 // Slice index of workgroup-loop index g: the two unit slices of a chunk go
 // to one XCD.
 __device__ __forceinline__ size_t slice_start(uint32_t g, uint32_t nslices, int slice) {
-    return static_cast<size_t>(slice < 64 ? xcd_unit_slice(g, nslices) : g) * slice;
+    const uint32_t s = slice == 16 ? xcd_quarter_slice(g, nslices) : slice < 64 ? xcd_unit_slice(g, nslices) : g;
+    return static_cast<size_t>(s) * slice;
 }
 
 template <class F, int MAXD>
@@ -163,7 +170,18 @@ static hipError_t allow_lds(K kernel, size_t lds) {
 // workgroups per CU.
 static unsigned threads_for(int bits, int rows, int slice, size_t lds) {
     const bool alone = lds > (80u << 10);
-    return alone && (rows / 4) * (slice / (bits == 16 ? 32 : 16)) >= kFBMax ? kFBMax : kFB;
+    const int units = std::max(1, slice / (bits == 16 ? 32 : 16));
+    return alone && (rows / 4) * units >= kFBMax ? kFBMax : kFB;
+}
+
+// Only while two workgroups fit per CU: at n = 8192 (one 512-thread
+// workgroup per CU) the three-pass schedule is faster (1.79 vs 1.87 ms for
+// k=4096 m=1024 64 KiB); at n = 4096 the quarter tile is 3-6% faster.
+int leo_quarter_slice(int n) {
+    const size_t lds = static_cast<size_t>(16) * (n + 1);
+    const unsigned threads = threads_for(16, n, 16, lds);
+    const size_t deriv = static_cast<size_t>(n) * 16;
+    return lds <= (80u << 10) && deriv <= static_cast<size_t>(kDerivMax) * 16 * threads ? 16 : 0;
 }
 
 hipError_t leo_encode_fused(const LeoDev &f, const LeoEncodeArgs &a, hipStream_t s) {
@@ -214,11 +232,11 @@ hipError_t leo_batch_launch(const LeoDev &f, const LeoBatchTask *tasks, int ntas
 }
 
 hipError_t leo_decode_fused(const LeoDev &f, const LeoDecodeArgs &a, hipStream_t s) {
-    // The derivative holds n * slice / 16 slots in registers.
+    // The derivative holds n * slice / 16 slots in registers, kDerivMax per thread.
     const size_t deriv = static_cast<size_t>(a.n) * a.slice;
-    if (deriv > kLeoFusedDecodeLds) return hipErrorInvalidValue;
     const size_t lds = static_cast<size_t>(a.slice) * (a.n + 1);
     const unsigned threads = threads_for(f.bits, a.n, a.slice, lds);
+    if (deriv > static_cast<size_t>(kDerivMax) * 16 * threads) return hipErrorInvalidValue;
     const bool small = deriv <= kLeoDerivSmall;
     hipError_t e;
 #define RSX_LAUNCH(FT, MD)                                                                              \
@@ -226,7 +244,9 @@ hipError_t leo_decode_fused(const LeoDev &f, const LeoDecodeArgs &a, hipStream_t
         if ((e = allow_lds(k_decode_fused<FT, MD>, lds)) != hipSuccess) return e;                        \
         hipLaunchKernelGGL((k_decode_fused<FT, MD>), dim3(grid_for(a.size, a.slice)), dim3(threads), lds, s, f, a); \
     } while (0)
-    if (f.bits == 16) {
+    if (f.bits == 16 && a.slice == 16) {
+        RSX_LAUNCH(F16Q, kDerivMax);
+    } else if (f.bits == 16) {
         if (small) RSX_LAUNCH(F16, kDerivMax / 2);
         else RSX_LAUNCH(F16, kDerivMax);
     } else {

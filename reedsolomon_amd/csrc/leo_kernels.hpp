@@ -116,14 +116,17 @@ hipError_t leo_batch_launch(const LeoDev &f, const LeoBatchTask *tasks, int ntas
                             unsigned grid, size_t deriv_bytes, hipStream_t s);
 
 // Slice width for `regions` LDS regions of `rows` rows, radix-4 steps over
-// `step_rows` rows and shards of `size` bytes, or 0 when even 64-byte slices
-// exceed `lds_budget`.
+// `step_rows` rows and shards of `size` bytes, or 0 when even the narrowest
+// slice (64 bytes; 32 for GF(2^16)) exceeds `lds_budget`.
 int leo_fused_slice(int bits, int rows, int regions, int step_rows, size_t size, size_t lds_budget);
+// 16 when a GF(2^16) reconstruct of n work rows runs as one LDS tile of
+// quarter-chunk slices (8 elements of every row; n = 4096), else 0.
+int leo_quarter_slice(int n);
 hipError_t leo_encode_fused(const LeoDev &f, const LeoEncodeArgs &a, hipStream_t s);
 hipError_t leo_decode_fused(const LeoDev &f, const LeoDecodeArgs &a, hipStream_t s);
 
 // ---- GF(2^16) reconstruct in three LDS-tiled passes (leo_multipass.hip) ----
-// For n >= 4096 rows (one tile holds 2048): work arrays hold `slices` 64-byte column slices of all
+// For n >= 8192 rows (one tile holds 4096): work arrays hold `slices` 64-byte column slices of all
 // n rows, slice-major.
 struct LeoMultipassArgs {
     const uint8_t *const *rows;  // n row pointers (null = zero / erased)

@@ -1,5 +1,5 @@
 // leo_multipass.hip — Leopard GF(2^16) reconstruct for transforms taller than
-// one LDS tile (n >= 4096 rows; the passes also hold for n = 2048).
+// one LDS tile (n >= 8192 rows; the passes also hold for n = 2048 and 4096).
 //
 // The n-row transform (leopard.go:421-647: scale by error locators, IFFT,
 // formal derivative, FFT, reveal) runs as three passes over 64-byte column

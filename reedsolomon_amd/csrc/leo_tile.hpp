@@ -58,6 +58,13 @@ __device__ __forceinline__ uint32_t xcd_unit_slice(uint32_t g, uint32_t count) {
     return (((g >> 4) << 4) | ((g & 7) << 1)) | ((g >> 3) & 1);
 }
 
+// The same for four 16-byte quarter slices per chunk: g, g + 8, g + 16 and
+// g + 24 take the quarters of one chunk.
+__device__ __forceinline__ uint32_t xcd_quarter_slice(uint32_t g, uint32_t count) {
+    if (g >= (count & ~31u)) return g;
+    return (((g >> 5) << 5) | ((g & 7) << 2)) | ((g >> 3) & 3);
+}
+
 __device__ __forceinline__ uint4 *slot(const Tile &t, int r, int s) {
     return reinterpret_cast<uint4 *>(t.base + ((s * t.r1 + pos(r)) << 4));
 }
@@ -80,6 +87,18 @@ struct Lds<F16> {
         const int l = lo(t, u);
         *slot(t, r, l) = e.lo;
         *slot(t, r, l + t.hio) = e.hi;
+    }
+};
+
+// A quarter-chunk unit is one slot: its 8 low bytes, then its 8 high bytes.
+template <>
+struct Lds<F16Q> {
+    __device__ static F16Q::E load(const Tile &t, int r, int u) {
+        const uint4 v = *slot(t, r, u);
+        return {make_uint2(v.x, v.y), make_uint2(v.z, v.w)};
+    }
+    __device__ static void store(const Tile &t, int r, int u, const F16Q::E &e) {
+        *slot(t, r, u) = make_uint4(e.lo.x, e.lo.y, e.hi.x, e.hi.y);
     }
 };
 

@@ -153,6 +153,7 @@ size_t chunk_for(size_t size, int rows, size_t budget) {
 
 constexpr size_t kEncodeBudget = 64u << 20;
 constexpr size_t kDecodeBudget = 64u << 20;
+constexpr bool kQuarterTiles = true;  // GF(2^16) reconstructs with n = 4096: one quarter-chunk tile
 
 // LDS budget of a one-tile encode: the m rows of a slice, and with several
 // groups their XOR accumulator in registers (at most the derivative's
@@ -340,9 +341,11 @@ int reconstruct_device(rsx_codec *c, const std::vector<uint8_t *> &shards, const
     const uint8_t *const *lptr = reinterpret_cast<const uint8_t *const *>(s.ptrs.ptr);
     uint8_t *const *optr = reinterpret_cast<uint8_t *const *>(s.ptrs.as<uint8_t *>() + n);
     // n rows in LDS; the formal derivative also holds n * slice bytes in registers.
-    const int slice = s.fused ? leo_fused_slice(s.bits, n, 1, n, size, std::min(kLeoFusedEncodeLds,
-                                                                            kLeoFusedDecodeLds / n * (n + 1)))
-                              : 0;
+    int slice = s.fused ? leo_fused_slice(s.bits, n, 1, n, size, std::min(kLeoFusedEncodeLds,
+                                                                      kLeoFusedDecodeLds / n * (n + 1)))
+                        : 0;
+    // n = 4096 GF(2^16) transforms: one tile of quarter-chunk slices.
+    if (!slice && s.fused && s.bits == 16 && kQuarterTiles) slice = leo_quarter_slice(n);
     if (slice) {
         const LeoDecodeArgs a{lptr, pat->d_errlocs.as<uint16_t>(), s.rows.as<int>(), s.logs.as<uint16_t>(), optr,
                               size, n, nout, ifft_trunc, m + k, slice};
