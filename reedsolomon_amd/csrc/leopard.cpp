@@ -64,6 +64,11 @@ struct LeoPattern {
     std::vector<uint16_t> errlocs;  // n entries
     int input_count = 0;
     DeviceBuffer d_errlocs;
+    // Output side of a per-stripe reconstruct, uploaded on first use: work
+    // rows of the erased shards, their reveal logs, and row -> output index.
+    bool outs_ready = false;
+    int nout = 0;
+    DeviceBuffer d_rows, d_logs, d_outidx;
 };
 
 struct LeopardState {
@@ -71,8 +76,9 @@ struct LeopardState {
     bool fused = true;  // LDS-resident kernels where the rows fit
     const LeoField *field = nullptr;
     const LeoDev *dev = nullptr;
-    DeviceBuffer work, work2, ptrs, rows, logs, stage, scratch, flag, cmp, outidx;
-    std::vector<uint8_t *> h_ptrs;
+    DeviceBuffer work, work2, ptrs, rows, stage, scratch, flag, cmp;
+    std::vector<uint8_t *> h_ptrs;  // what `ptrs` holds on the device
+    int enc_rows = 0;               // `rows` holds 0 .. enc_rows-1 (encode's output rows)
     std::unordered_map<std::string, std::unique_ptr<LeoPattern>> patterns;
 };
 
@@ -170,10 +176,17 @@ bool all_aligned(const std::vector<uint8_t *> &p) {
 
 // Uploads a pointer table into `buf` (synchronously ordered on the stream;
 // the host copy stays alive in the state until the call has synchronised).
-int upload_ptrs(rsx_codec *c, DeviceBuffer &buf, const std::vector<uint8_t *> &p) {
-    int err = buf.reserve(std::max<size_t>(p.size(), 1) * sizeof(void *));
+// Row pointer table of a call in s.ptrs; skipped when the device copy already
+// holds this table (repeated calls on the same buffers).  The copy is ordered
+// on the codec's stream after every kernel that read the previous table.
+int upload_ptrs(rsx_codec *c, std::vector<uint8_t *> &&p) {
+    LeopardState &s = *c->leo;
+    const size_t bytes = std::max<size_t>(p.size(), 1) * sizeof(void *);
+    if (p == s.h_ptrs && bytes <= s.ptrs.bytes) return RSX_OK;
+    int err = s.ptrs.reserve(bytes);
     if (err) return err;
-    RSX_HIP(hipMemcpyAsync(buf.ptr, p.data(), p.size() * sizeof(void *), hipMemcpyHostToDevice, c->stream));
+    RSX_HIP(hipMemcpyAsync(s.ptrs.ptr, p.data(), p.size() * sizeof(void *), hipMemcpyHostToDevice, c->stream));
+    s.h_ptrs = std::move(p);
     return RSX_OK;
 }
 
@@ -231,13 +244,15 @@ int encode_device(rsx_codec *c, const std::vector<uint8_t *> &data, const std::v
     std::vector<uint8_t *> tab(static_cast<size_t>(groups) * m + p, nullptr);
     for (int j = 0; j < k; ++j) tab[j] = data[j];
     for (int i = 0; i < p; ++i) tab[static_cast<size_t>(groups) * m + i] = parity[i];
-    s.h_ptrs = tab;
-    int err = upload_ptrs(c, s.ptrs, s.h_ptrs);
+    int err = upload_ptrs(c, std::move(tab));
     if (err) return err;
-    std::vector<int> rows(p);
-    for (int i = 0; i < p; ++i) rows[i] = i;
-    if ((err = s.rows.reserve(p * sizeof(int)))) return err;
-    RSX_HIP(hipMemcpyAsync(s.rows.ptr, rows.data(), p * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    if (s.enc_rows != p) {  // the layer-per-launch path writes rows 0 .. p-1
+        std::vector<int> rows(p);
+        for (int i = 0; i < p; ++i) rows[i] = i;
+        if ((err = s.rows.reserve(p * sizeof(int)))) return err;
+        RSX_HIP(hipMemcpyAsync(s.rows.ptr, rows.data(), p * sizeof(int), hipMemcpyHostToDevice, c->stream));
+        s.enc_rows = p;
+    }
 
     const uint8_t *const *dptr = reinterpret_cast<const uint8_t *const *>(s.ptrs.ptr);
     uint8_t *const *pptr = reinterpret_cast<uint8_t *const *>(s.ptrs.as<uint8_t *>() + static_cast<size_t>(groups) * m);
@@ -318,25 +333,39 @@ int reconstruct_device(rsx_codec *c, const std::vector<uint8_t *> &shards, const
     for (int i = 0; i < p; ++i) tab[i] = erased[k + i] ? nullptr : shards[k + i];
     for (int i = 0; i < k; ++i) tab[m + i] = erased[i] ? nullptr : shards[i];
     const int end = recover_all ? total : k;
-    std::vector<int> rows;
-    std::vector<uint16_t> logs;
     for (int i = 0; i < end; ++i) {
         if (!erased[i]) continue;
-        const int pos = i >= k ? i - k : i + m;
-        rows.push_back(pos);
-        logs.push_back(static_cast<uint16_t>(s.field->modulus - pat->errlocs[pos]));
         tab.push_back(shards[i]);
         filled[i] = 1;
     }
-    const int nout = static_cast<int>(rows.size());
-    s.h_ptrs = tab;
-    if ((err = upload_ptrs(c, s.ptrs, s.h_ptrs))) return err;
-    if ((err = s.rows.reserve(std::max(nout, 1) * sizeof(int)))) return err;
-    if ((err = s.logs.reserve(std::max(nout, 1) * sizeof(uint16_t)))) return err;
-    if (nout) {
-        RSX_HIP(hipMemcpyAsync(s.rows.ptr, rows.data(), nout * sizeof(int), hipMemcpyHostToDevice, c->stream));
-        RSX_HIP(hipMemcpyAsync(s.logs.ptr, logs.data(), nout * sizeof(uint16_t), hipMemcpyHostToDevice, c->stream));
+    if (!pat->outs_ready) {
+        // Output rows, reveal logs (leopard.go:640-646) and row -> output
+        // index depend only on the pattern: uploaded once.
+        std::vector<int> rows, out_index(static_cast<size_t>(n), -1);
+        std::vector<uint16_t> logs;
+        for (int i = 0; i < end; ++i) {
+            if (!erased[i]) continue;
+            const int pos = i >= k ? i - k : i + m;
+            out_index[pos] = static_cast<int>(rows.size());
+            rows.push_back(pos);
+            logs.push_back(static_cast<uint16_t>(s.field->modulus - pat->errlocs[pos]));
+        }
+        const int no = static_cast<int>(rows.size());
+        if ((err = pat->d_rows.reserve(std::max(no, 1) * sizeof(int)))) return err;
+        if ((err = pat->d_logs.reserve(std::max(no, 1) * sizeof(uint16_t)))) return err;
+        if ((err = pat->d_outidx.reserve(out_index.size() * sizeof(int)))) return err;
+        if (no) {
+            RSX_HIP(hipMemcpyAsync(pat->d_rows.ptr, rows.data(), no * sizeof(int), hipMemcpyHostToDevice, c->stream));
+            RSX_HIP(hipMemcpyAsync(pat->d_logs.ptr, logs.data(), no * sizeof(uint16_t), hipMemcpyHostToDevice,
+                                   c->stream));
+        }
+        RSX_HIP(hipMemcpyAsync(pat->d_outidx.ptr, out_index.data(), out_index.size() * sizeof(int),
+                               hipMemcpyHostToDevice, c->stream));
+        pat->nout = no;
+        pat->outs_ready = true;
     }
+    const int nout = pat->nout;
+    if ((err = upload_ptrs(c, std::move(tab)))) return err;
     const int ifft_trunc = s.bits == 16 ? pat->input_count : m + k;
     const uint8_t *const *lptr = reinterpret_cast<const uint8_t *const *>(s.ptrs.ptr);
     uint8_t *const *optr = reinterpret_cast<uint8_t *const *>(s.ptrs.as<uint8_t *>() + n);
@@ -347,7 +376,7 @@ int reconstruct_device(rsx_codec *c, const std::vector<uint8_t *> &shards, const
     // n = 4096 GF(2^16) transforms: one tile of quarter-chunk slices.
     if (!slice && s.fused && s.bits == 16 && kQuarterTiles) slice = leo_quarter_slice(n);
     if (slice) {
-        const LeoDecodeArgs a{lptr, pat->d_errlocs.as<uint16_t>(), s.rows.as<int>(), s.logs.as<uint16_t>(), optr,
+        const LeoDecodeArgs a{lptr, pat->d_errlocs.as<uint16_t>(), pat->d_rows.as<int>(), pat->d_logs.as<uint16_t>(), optr,
                               size, n, nout, ifft_trunc, m + k, slice};
         if (nout) RSX_HIP(leo_decode_fused(*s.dev, a, c->stream));
         return RSX_OK;
@@ -355,17 +384,12 @@ int reconstruct_device(rsx_codec *c, const std::vector<uint8_t *> &shards, const
     int mC = 0, mP = 0, mA = 0;
     if (s.fused && s.bits == 16 && leo_multipass_shape(n, &mC, &mP, &mA)) {
         if (!nout) return RSX_OK;
-        std::vector<int> out_index(static_cast<size_t>(n), -1);
-        for (int q = 0; q < nout; ++q) out_index[rows[q]] = q;
-        if ((err = s.outidx.reserve(out_index.size() * sizeof(int)))) return err;
-        RSX_HIP(hipMemcpyAsync(s.outidx.ptr, out_index.data(), out_index.size() * sizeof(int),
-                               hipMemcpyHostToDevice, c->stream));
         const size_t cw = leo_multipass_chunk(n, size);
         if ((err = s.work.reserve(cw * n))) return err;
         if ((err = s.work2.reserve(cw * n))) return err;
         for (size_t off = 0; off < size; off += cw) {
             const size_t cols = std::min(cw, size - off);
-            const LeoMultipassArgs a{lptr, pat->d_errlocs.as<uint16_t>(), s.outidx.as<int>(), s.logs.as<uint16_t>(),
+            const LeoMultipassArgs a{lptr, pat->d_errlocs.as<uint16_t>(), pat->d_outidx.as<int>(), pat->d_logs.as<uint16_t>(),
                                      optr, s.work.as<uint8_t>(), s.work2.as<uint8_t>(), off,
                                      static_cast<int>(cols / 64), n, ifft_trunc, m + k, mC, mP, mA};
             RSX_HIP(leo_decode_multipass(*s.dev, a, c->stream));
@@ -392,7 +416,7 @@ int reconstruct_device(rsx_codec *c, const std::vector<uint8_t *> &shards, const
         RSX_HIP(leo_derivative(w, w2, n, c->stream));
         if ((err = fft(c, s, w2, m + k, n))) return err;
         if (nout)
-            RSX_HIP(leo_store_rows(f, w2, s.rows.as<int>(), s.logs.as<uint16_t>(), optr, off, nout, c->stream));
+            RSX_HIP(leo_store_rows(f, w2, pat->d_rows.as<int>(), pat->d_logs.as<uint16_t>(), optr, off, nout, c->stream));
     }
     return RSX_OK;
 }
@@ -468,7 +492,9 @@ int leopard_encode(rsx_codec *c, uint8_t *const *shards, const size_t *lens, int
     std::vector<uint8_t *> data(r.dev.begin(), r.dev.begin() + k), parity(r.dev.begin() + k, r.dev.end());
     if ((err = encode_device(c, data, parity, size))) return err;
     if ((err = unstage(c, shards, r, save, size))) return err;
-    RSX_HIP(hipStreamSynchronize(c->stream));
+    // Device-resident calls are ordered on the codec's stream (rsx_synchronize
+    // waits); host-memory calls return with the results in place.
+    if (r.staged) RSX_HIP(hipStreamSynchronize(c->stream));
     return RSX_OK;
 }
 
@@ -550,7 +576,7 @@ int leopard_reconstruct(rsx_codec *c, uint8_t *const *shards, const size_t *lens
     if ((err = resolve(c, shards, total, size, use, load, r))) return err;
     if ((err = reconstruct_device(c, r.dev, erased.data(), recover_all, size, filled))) return err;
     if ((err = unstage(c, shards, r, save, size))) return err;
-    RSX_HIP(hipStreamSynchronize(c->stream));
+    if (r.staged) RSX_HIP(hipStreamSynchronize(c->stream));
     return RSX_OK;
 }
 
