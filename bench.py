@@ -11,7 +11,10 @@ planned, graph-replayed launch for all encodes, one for all reconstructs).
 
 Also measured (unless --quick): RS(10,4) over 1 MiB shards (1024 stripes,
 batched encode + reconstruct), RS(10,3) over one stripe of 10 x 1 MiB (plain
-Encode, Verify, Reconstruct of data shards 3 and 7), the mixed workload (8192 stripes, (k, m) in
+Encode, Verify, Reconstruct of data shards 3 and 7), the reference's own
+benchmark shapes (README encode / reconstruct tables, Leopard 1 KB shards)
+through the plain per-call path beside the published CPU figures, the mixed
+workload (8192 stripes, (k, m) in
 {(4,2),(6,3),(10,4),(17,3),(32,8)}, shards log-uniform 4 KiB-1 MiB rounded to
 64 B, 30% reconstructs with 1..m erasures) batched vs plain, and Leopard
 GF(2^16) k=4096, m=1024, 64 KiB shards (encode + reconstruct of 1024 erasures).
@@ -69,7 +72,7 @@ def _parse():
                     help="served stream: Poisson rate as a fraction of the measured saturation rate")
     ap.add_argument("--hbm-budget-gib", type=float, default=16.0,
                     help="served stream: shard bytes admitted in flight")
-    ap.add_argument("--only", choices=["rs10_4", "rs10_3", "mixed", "leopard16", "served", "stream"],
+    ap.add_argument("--only", choices=["rs10_4", "rs10_3", "mixed", "leopard16", "served", "stream", "reference"],
                     help="run one secondary section alone and print its JSON (for profiling)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="with --encode-only: also copy 1 GiB HBM->HBM 5 times (counter calibration)")
@@ -384,6 +387,147 @@ def config0(args, torch, rs):
     del buf
     torch.cuda.empty_cache()
     return out
+
+
+# The reference's published benchmarks (README.md:531-555 and 493-516), in
+# MB/s (10^6 B/s) of (k+m)*shard bytes per operation, single stripe.
+REF_CLASSIC = [  # (op, k, m, shard bytes, published MB/s: GFNI column / "all" column)
+    ("encode", 5, 2, 1 << 20, 241747), ("encode", 8, 8, 1 << 20, 271680),
+    ("encode", 10, 4, 1 << 20, 273628), ("encode", 50, 20, 1 << 20, 47706),
+    ("reconstruct", 10, 2, 10000, 122764.16), ("reconstruct", 50, 5, 50000, 116532.21),
+    ("reconstruct", 10, 2, 1 << 20, 269362.91), ("reconstruct", 5, 2, 1 << 20, 236882.30),
+    ("reconstruct", 10, 4, 1 << 20, 298882.25), ("reconstruct", 50, 20, 1 << 20, 75888.83),
+    ("reconstruct", 10, 4, 16 << 20, 62017.48),
+]
+REF_LEOPARD_1K = [  # (bits, shards per side, encode, recover all, recover one) MB/s
+    (8, 4, 55138.94, 7695.13, 12775.18), (8, 8, 52264.75, 9425.42, 13794.19),
+    (8, 16, 64844.90, 7709.28, 10713.69), (8, 32, 39757.60, 7671.30, 9575.71),
+    (8, 64, 40143.15, 6733.38, 9210.23), (8, 128, 30116.72, 5937.93, 8806.07),
+    (16, 256, 27153.59, 1187.09, 1269.02), (16, 512, 22315.40, 1888.36, 2136.33),
+    (16, 1024, 19596.76, 2466.09, 3068.25), (16, 2048, 15700.86, 3003.09, 4048.59),
+    (16, 4096, 15134.19, 3190.63, 4524.92), (16, 8192, 13246.07, 3067.94, 4062.52),
+    (16, 16384, 8403.47, 1837.53, 2744.81), (16, 32768, 5558.16, 1317.53, 1958.46),
+]
+
+
+def reference_shapes(args, torch, rs):
+    """The reference's own benchmark shapes through the plain per-call path
+    (one stripe, shards resident in HBM, calls queued back to back on the
+    codec's stream and timed with HIP events), beside its published CPU
+    numbers.  Encode: Encode repeatedly (reedsolomon_test.go:1142-1168).
+    Reconstruct: 1..m random shards erased per call, cycling through 8 seeded
+    patterns (corruptRandom, 1444-1480).  Leopard 1 KB: Encode, and
+    Reconstruct with all k data shards (recover all) or shard 0 (recover one)
+    erased (benchmarkDecode, 1170-1199).  `batched` runs the same operation on
+    many stripes of the shape (about 256 MiB) in one planned launch, the
+    GPU-native form of a small operation (None where the batch kernel does
+    not take the shape)."""
+    lib = rs.lib
+    rng = np.random.default_rng(SEED + 7)
+    reps = 10
+
+    def batched(k, m, size, enc, op, lens_fn):
+        n = k + m
+        nst = int(max(1, min(4096, (256 << 20) // (n * size))))
+        gen = torch.Generator(device="cuda")
+        gen.manual_seed(SEED + 8)
+        st = Stripes([(k, m)] * nst, [size] * nst, torch, gen)
+        for i in range(nst):  # valid codewords first (reconstructs need them)
+            _check(lib, lib.rsx_encode(enc.handle, st.ptrs[i], st.full[i], n))
+        _check(lib, lib.rsx_synchronize(enc.handle))
+        b = rs.Batch(0)
+        items = batch_items(rs, st, {(k, m): enc}, [op] * nst, [lens_fn(i) for i in range(nst)])
+        status = (ctypes.c_int * nst)()
+        filled = (ctypes.c_uint8 * (nst * n))()
+        _check(lib, lib.rsx_batch_plan(b._h, items, nst, status, filled))
+        if any(status[i] for i in range(nst)):
+            return None
+        b.launch()
+        b.synchronize()
+        t = event_time(torch, b.stream, b.launch, reps)
+        del st, b
+        return round(nst * n * size / t / 1e9, 1)
+
+    def stripe(k, m, size, enc):
+        n = k + m
+        buf = torch.empty((n, size), dtype=torch.uint8, device="cuda")
+        buf[:k].random_(0, 256)
+        buf[k:].zero_()
+        torch.cuda.synchronize()
+        ptrs = (ctypes.c_void_p * n)(*[buf.data_ptr() + i * size for i in range(n)])
+        full = (ctypes.c_size_t * n)(*([size] * n))
+        _check(lib, lib.rsx_encode(enc.handle, ptrs, full, n))
+        return buf, ptrs, full
+
+    def timed(enc, calls):
+        stream = lib.rsx_get_stream(enc.handle)
+        for c in calls[:2]:
+            c()
+        i = [0]
+
+        def one():
+            calls[i[0] % len(calls)]()
+            i[0] += 1
+        return event_time(torch, stream, one, reps * len(calls)) if len(calls) > 1 else event_time(
+            torch, stream, calls[0], reps)
+
+    classic = []
+    for op, k, m, size, ref in REF_CLASSIC:
+        n = k + m
+        enc = rs.New(k, m)
+        buf, ptrs, full = stripe(k, m, size, enc)
+        filled = (ctypes.c_uint8 * n)()
+        if op == "encode":
+            calls = [lambda: _check(lib, lib.rsx_encode(enc.handle, ptrs, full, n))]
+            bop, blens = rs.OP_ENCODE, (lambda i: full)
+        else:
+            pats = []
+            for _ in range(8):
+                lost = set(rng.integers(0, n, int(rng.integers(1, m + 1))).tolist())
+                pats.append((ctypes.c_size_t * n)(*[0 if i in lost else size for i in range(n)]))
+            calls = [lambda p=p: _check(lib, lib.rsx_reconstruct(enc.handle, ptrs, p, n, filled)) for p in pats]
+            bop, blens = rs.OP_RECONSTRUCT, (lambda i, pats=pats: pats[i % len(pats)])
+        t = timed(enc, calls)
+        gbps = n * size / t / 1e9
+        del buf
+        bg = batched(k, m, size, enc, bop, blens)
+        classic.append({"op": op, "k": k, "m": m, "shard_bytes": size, "gbps": round(gbps, 1),
+                        "batched_gbps": bg, "reference_mbps": ref, "ratio": round(gbps * 1000 / ref, 1),
+                        "batched_ratio": round(bg * 1000 / ref, 1) if bg else None})
+    leo = []
+    for bits, sh, ref_enc, ref_all, ref_one in REF_LEOPARD_1K:
+        k = m = sh
+        n, size = 2 * sh, 1024
+        enc = rs.New(k, m, rs.WithLeopardGF16(True) if bits == 16 else rs.WithLeopardGF(True))
+        buf, ptrs, full = stripe(k, m, size, enc)
+        filled = (ctypes.c_uint8 * n)()
+        all_lost = (ctypes.c_size_t * n)(*[0 if i < k else size for i in range(n)])
+        one_lost = (ctypes.c_size_t * n)(*[0 if i == 0 else size for i in range(n)])
+        row = {"field": f"GF(2^{bits})", "shards": f"{sh}+{sh}", "shard_bytes": size}
+        cases = (("encode", lambda: _check(lib, lib.rsx_encode(enc.handle, ptrs, full, n)), ref_enc,
+                  rs.OP_ENCODE, full),
+                 ("recover_all", lambda: _check(lib, lib.rsx_reconstruct(enc.handle, ptrs, all_lost, n, filled)),
+                  ref_all, rs.OP_RECONSTRUCT, all_lost),
+                 ("recover_one", lambda: _check(lib, lib.rsx_reconstruct(enc.handle, ptrs, one_lost, n, filled)),
+                  ref_one, rs.OP_RECONSTRUCT, one_lost))
+        for name, fn, ref, bop, blens in cases:
+            t = timed(enc, [fn])
+            row[name] = {"gbps": round(n * size / t / 1e9, 2), "reference_mbps": ref,
+                         "ratio": round(n * size / t / 1e6 / ref, 2)}
+        del buf
+        for name, fn, ref, bop, blens in cases:
+            bg = batched(k, m, size, enc, bop, lambda i, blens=blens: blens)
+            row[name]["batched_gbps"] = bg
+            row[name]["batched_ratio"] = round(bg * 1000 / ref, 1) if bg else None
+        leo.append(row)
+    ratios = [r["ratio"] for r in classic] + [r[x]["ratio"] for r in leo for x in ("encode", "recover_all", "recover_one")]
+    best = [max(r["ratio"], r["batched_ratio"] or 0) for r in classic] + [
+        max(r[x]["ratio"], r[x]["batched_ratio"] or 0) for r in leo for x in ("encode", "recover_all", "recover_one")]
+    _log(f"bench: reference shapes vs published CPU: per-call lowest ratio {min(ratios)}x, "
+         f"best of per-call/batched lowest ratio {min(best)}x")
+    torch.cuda.empty_cache()
+    return {"classic": classic, "leopard_1k": leo, "lowest_ratio_per_call": min(ratios),
+            "lowest_ratio_best_path": min(best)}
 
 
 def mixed(args, torch, rs):
@@ -754,7 +898,7 @@ def main() -> int:
     torch.cuda.set_device(0)
     if args.only:
         fn = {"rs10_4": rs10_4, "rs10_3": config0, "mixed": mixed, "leopard16": leopard, "served": served,
-              "stream": stream}[args.only]
+              "stream": stream, "reference": reference_shapes}[args.only]
         sec = fn(args, torch, rs)
         sec.pop("sample", None)
         print(json.dumps({"section": args.only, **sec}), flush=True)
@@ -764,10 +908,11 @@ def main() -> int:
         head.pop("sample", None)
         print(json.dumps({"kernel": "batch_pass<4>", **head}), flush=True)
         return 0
-    mix = leo = srv = r104 = c0 = strm = None
+    mix = leo = srv = r104 = c0 = strm = refs = None
     if not args.quick:
         r104 = rs10_4(args, torch, rs)
         c0 = config0(args, torch, rs)
+        refs = reference_shapes(args, torch, rs)
         mix = mixed(args, torch, rs)
         leo = leopard(args, torch, rs)
         srv = served(args, torch, rs)
@@ -808,6 +953,7 @@ def main() -> int:
         "tuning": head["tuning"],
         "rs10_4_1mib": r104,
         "rs10_3_1mib": c0,
+        "reference_shapes": refs,
         "mixed_workload": mix,
         "leopard16": leo,
         "served": srv,
