@@ -642,14 +642,19 @@ def leopard(args, torch, rs):
     _check(lib, lib.rsx_synchronize(h))
     rebuilt_ok = bool(torch.equal(buf, want))
     del want
+    # Wall clock per call with the codec's stream drained before and after
+    # (device-resident calls return before their kernels finish).
     reps = 3
+    _check(lib, lib.rsx_synchronize(h))
     t0 = time.perf_counter()
     for _ in range(reps):
         _check(lib, lib.rsx_encode(h, ptrs, full, n))
+    _check(lib, lib.rsx_synchronize(h))
     t_enc = (time.perf_counter() - t0) / reps
     t0 = time.perf_counter()
     for _ in range(reps):
         _check(lib, lib.rsx_reconstruct(h, ptrs, lens, n, filled))
+    _check(lib, lib.rsx_synchronize(h))
     t_rec = (time.perf_counter() - t0) / reps
     b = n * S
     _log(f"bench: leopard16 encode {b / t_enc / 1e9:.1f} GB/s, reconstruct {b / t_rec / 1e9:.1f} GB/s")
