@@ -32,6 +32,14 @@ struct rsx_batch {
     // launch stream, so they overlap the classic passes (also inside graphs).
     hipStream_t side = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
+    // Output-count classes too small to keep the device busy for long (fewer
+    // tiles than two grids) run on these streams, forked from and joined back
+    // to the launch stream, so their tails overlap other classes instead of
+    // idling the device; large classes run in order on the launch stream.
+    static constexpr int kClassStreams = 3;
+    static constexpr bool kSpreadClasses = true;
+    hipStream_t cls[kClassStreams] = {};
+    hipEvent_t cls_join[kClassStreams] = {};
     int cus = 0;
     // tuning: 0 = autotune on first launch
     uint32_t tile = 0;
@@ -165,9 +173,13 @@ int enqueue(rsx_batch *b, hipStream_t s) {
         RSX_HIP(launch_fill_u32(b->d_flags.as<uint32_t>(), static_cast<size_t>(b->nitems), 0u, s));
     const bool leo = !b->leo_tasks[0].empty() || !b->leo_tasks[1].empty();
     const bool fork = leo && !b->launches.empty();
+    const int spread = rsx_batch::kSpreadClasses
+                           ? std::min<int>(static_cast<int>(b->launches.size()) - 1, rsx_batch::kClassStreams)
+                           : 0;
+    if (fork || spread > 0) RSX_HIP(hipEventRecord(b->fork, s));
+    for (int j = 0; j < spread; ++j) RSX_HIP(hipStreamWaitEvent(b->cls[j], b->fork, 0));
     hipStream_t ls = s;
     if (fork) {
-        RSX_HIP(hipEventRecord(b->fork, s));
         RSX_HIP(hipStreamWaitEvent(b->side, b->fork, 0));
         ls = b->side;
     }
@@ -179,9 +191,16 @@ int enqueue(rsx_batch *b, hipStream_t s) {
                                  b->leo_deriv[fi], ls));
     }
     if (fork) RSX_HIP(hipEventRecord(b->join, b->side));
+    int next_side = 0;
     for (size_t i = 0; i < b->launches.size(); ++i) {
         const unsigned g = std::min<unsigned>(grid, std::max<uint32_t>(b->launches[i].ntiles, 1));
-        RSX_HIP(launch_batch_pass(b->launches[i], b->classes[i], g, s));
+        const bool small = static_cast<uint64_t>(b->launches[i].ntiles) < 2ull * grid;
+        hipStream_t cs = small && spread > 0 ? b->cls[next_side++ % spread] : s;
+        RSX_HIP(launch_batch_pass(b->launches[i], b->classes[i], g, cs));
+    }
+    for (int j = 0; j < spread; ++j) {
+        RSX_HIP(hipEventRecord(b->cls_join[j], b->cls[j]));
+        RSX_HIP(hipStreamWaitEvent(s, b->cls_join[j], 0));
     }
     if (fork) RSX_HIP(hipStreamWaitEvent(s, b->join, 0));
 
@@ -272,6 +291,12 @@ int rsx_batch_create(int device, rsx_batch **out) {
         rsx_batch_free(b);
         return fail(RSX_ERR_DEVICE, "stream creation failed");
     }
+    for (int j = 0; j < rsx_batch::kClassStreams; ++j)
+        if (hipStreamCreateWithFlags(&b->cls[j], hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&b->cls_join[j], hipEventDisableTiming) != hipSuccess) {
+            rsx_batch_free(b);
+            return fail(RSX_ERR_DEVICE, "stream creation failed");
+        }
     *out = b;
     return RSX_OK;
 }
@@ -281,11 +306,17 @@ void rsx_batch_free(rsx_batch *b) {
     (void)hipSetDevice(b->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     if (b->side) (void)hipStreamSynchronize(b->side);
+    for (int j = 0; j < rsx_batch::kClassStreams; ++j)
+        if (b->cls[j]) (void)hipStreamSynchronize(b->cls[j]);
     destroy_graph(b);
     if (b->h_flags) (void)hipHostFree(b->h_flags);
     if (b->fork) (void)hipEventDestroy(b->fork);
     if (b->join) (void)hipEventDestroy(b->join);
     if (b->side) (void)hipStreamDestroy(b->side);
+    for (int j = 0; j < rsx_batch::kClassStreams; ++j) {
+        if (b->cls_join[j]) (void)hipEventDestroy(b->cls_join[j]);
+        if (b->cls[j]) (void)hipStreamDestroy(b->cls[j]);
+    }
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
