@@ -99,18 +99,23 @@ void destroy_graph(rsx_batch *b) {
 // a tile -> task table (4 bytes per tile) on the device.
 constexpr uint64_t kTileTableMax = uint64_t{1} << 24;
 
+// Output-count classes of the batched GF(2^8) pass (one kernel each).  Plans
+// keep 8-output tasks: a 16-output class measured slower in mixed plans
+// (9-15-output tasks waste accumulators at one wave per SIMD).
+constexpr int kClasses[] = {1, 2, 3, 4, 5, 6, 7, 8};
+
 // Lays the tasks out for one tile size and uploads them.
 int build_launches(rsx_batch *b, uint32_t tile) {
     std::vector<BatchTask> all;
     std::vector<uint32_t> tile_task;  // per class, concatenated: class-local task index of every tile
     std::vector<size_t> tt_off;
     uint64_t total_tiles = 0;
-    for (int mc = 1; mc <= 8; ++mc)
+    for (int mc : kClasses)
         for (const auto &t : b->tasks[mc]) total_tiles += (t.len + tile - 1) / tile;
     const bool use_table = total_tiles <= kTileTableMax;
     b->launches.clear();
     b->classes.clear();
-    for (int mc = 1; mc <= 8; ++mc) {
+    for (int mc : kClasses) {
         if (b->tasks[mc].empty()) continue;
         BatchLaunch L{};
         L.ntasks = static_cast<int>(b->tasks[mc].size());
@@ -511,6 +516,7 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
         b->pool.insert(b->pool.end(), in.begin(), in.end());
         for (int o0 = 0; o0 < mtot; o0 += 8) {
             const int nout = std::min(8, mtot - o0);
+            const int mc = nout;
             const uint32_t out_off = static_cast<uint32_t>(b->pool.size());
             for (int i = 0; i < nout; ++i) b->pool.push_back(out[o0 + i]);
             rsx_batch::Task t{};
@@ -525,8 +531,8 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
             t.aligned = (bits & 15) == 0;
             t.mode = item.op == RSX_OP_VERIFY ? kGf8Compare : kGf8Clear;
             t.item = static_cast<uint32_t>(it);
-            b->tasks[nout].push_back(t);
-            pend.push_back({nout, b->tasks[nout].size() - 1, toff});
+            b->tasks[mc].push_back(t);
+            pend.push_back({mc, b->tasks[mc].size() - 1, toff});
         }
     }
     for (int it = 0; it < n; ++it) b->has_verify |= b->is_verify[it] == 1;

@@ -126,16 +126,16 @@ int classify_all(const std::vector<const void *> &ptrs, MemKind &kind) {
 }
 
 // Runs out = rows * in (or out ^= rows * in) in passes of at most kMaxOut
-// outputs.  `tables` is [nin][nrows][5].
+// outputs (pass_outputs).  `tables` is [nin][nrows][5].
 int run_matmul(rsx_codec *c, const uint32_t *tables, const std::vector<const uint8_t *> &in,
                const std::vector<uint8_t *> &out, size_t len, int mode, uint32_t *flag, uint32_t flag_value) {
     const int nin = static_cast<int>(in.size());
     const int nrows = static_cast<int>(out.size());
     if (nin > kMaxIn) return fail(RSX_ERR_INVALID_INPUT, "too many inputs for one pass");
-    for (int o0 = 0; o0 < nrows; o0 += kMaxOut) {
+    for (int o0 = 0; o0 < nrows; o0 += pass_outputs(nrows - o0, nin)) {
         Gf8Pass p{};
         for (int j = 0; j < nin; ++j) p.in[j] = in[j];
-        p.nout = std::min(kMaxOut, nrows - o0);
+        p.nout = pass_outputs(nrows - o0, nin);
         for (int i = 0; i < p.nout; ++i) p.out[i] = out[o0 + i];
         p.tables = tables;
         p.len = len;

@@ -197,7 +197,7 @@ __device__ __forceinline__ void vector1(const uint8_t *const *in, uint8_t *const
 // so accumulators + one input group stay within the VGPR budget.
 template <int M>
 struct Shape {
-    static constexpr int R = M <= 4 ? 4 : 2;
+    static constexpr int R = M <= 4 ? 4 : M <= 8 ? 2 : 1;  // R = 1 beat 2 for no M <= 8 (mixed plans 3.0 vs 3.6 TB/s)
     static constexpr int G = 4;
 };
 

@@ -111,24 +111,29 @@ hipError_t launch_gf8_pass(const Gf8Pass &p, hipStream_t stream) {
             // Chunks per wave by shard length: one 16-byte vector per lane
             // below 4 MiB (latency-bound: as many waves as possible), two
             // 1 KiB chunks per wave up to 16 MiB, four beyond (with at most
-            // four outputs, so the accumulators fit).
-            const int R = p.len < (4u << 20) ? 1 : (p.nout <= 4 && p.len >= (16u << 20)) ? 4 : 2;
+            // four outputs, so the accumulators fit).  9-16 outputs run the
+            // 16-output kernel, one vector per lane (64 accumulator VGPRs).
+            const int M = p.nout <= 8 ? p.nout : 16;
+            const int R = M == 16 || p.len < (4u << 20) ? 1 : (p.nout <= 4 && p.len >= (16u << 20)) ? 4 : 2;
             const size_t per_block = static_cast<size_t>(R) * 64 * (kBlock / 64);
             const size_t nchunk_blocks = R == 1 ? 0 : nvec / per_block;
             const size_t rest = nvec - nchunk_blocks * per_block;
             size_t g = std::max<size_t>(nchunk_blocks, (rest + kBlock - 1) / kBlock);
             g = std::max<size_t>(1, std::min<size_t>(g, 65535u * 16));
             const dim3 grid(static_cast<unsigned>(g)), block(kBlock);
-            const size_t lds_bytes = static_cast<size_t>(p.k) * p.nout * 20;
+            const size_t lds_bytes = static_cast<size_t>(p.k) * M * 20;
 #define RSX_CASE(M)                                                                                  \
     case M:                                                                                          \
         if (R == 4) hipLaunchKernelGGL((gf8_pass_stream<M, 4>), grid, block, lds_bytes, stream, p);  \
         else if (R == 2) hipLaunchKernelGGL((gf8_pass_stream<M, 2>), grid, block, lds_bytes, stream, p); \
         else hipLaunchKernelGGL((gf8_pass_stream<M, 0>), grid, block, lds_bytes, stream, p);         \
         break;
-            switch (p.nout) {
+            switch (M) {
                 RSX_CASE(1) RSX_CASE(2) RSX_CASE(3) RSX_CASE(4)
                 RSX_CASE(5) RSX_CASE(6) RSX_CASE(7) RSX_CASE(8)
+            case 16:
+                hipLaunchKernelGGL((gf8_pass_stream<16, 0>), grid, block, lds_bytes, stream, p);
+                break;
             default:
                 return hipErrorInvalidValue;
             }
