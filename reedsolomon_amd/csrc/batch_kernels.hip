@@ -34,8 +34,11 @@ __device__ __forceinline__ int find_task(const BatchTask *tasks, int ntasks, uin
     return lo;
 }
 
+// Two workgroups per CU at least: caps the 7- and 8-output classes at 256
+// VGPRs (280 unbounded, one wave per SIMD); the few spills cost less than
+// the lost occupancy (mixed plans 3.61 -> 4.10 TB/s).
 template <int M>
-__global__ __launch_bounds__(kBlock) void batch_pass(const BatchLaunch L) {
+__global__ __launch_bounds__(kBlock, 2) void batch_pass(const BatchLaunch L) {
     constexpr int R = Shape<M>::R;
     constexpr uint64_t per_chunk = static_cast<uint64_t>(R) * 64;  // vectors per wave-chunk
     extern __shared__ uint4 lds[];
