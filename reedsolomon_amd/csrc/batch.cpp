@@ -579,6 +579,7 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
                 t.dec.errlocs = reinterpret_cast<const uint16_t *>(b->d_leo_aux.as<uint8_t>() + lp.errl_off);
                 t.dec.out_rows = reinterpret_cast<const int *>(b->d_leo_aux.as<uint8_t>() + lp.rows_off);
                 t.dec.out_logs = reinterpret_cast<const uint16_t *>(b->d_leo_aux.as<uint8_t>() + lp.logs_off);
+                t.dec.need = reinterpret_cast<const uint32_t *>(b->d_leo_aux.as<uint8_t>() + lp.need_off);
             }
             if (static_cast<uint64_t>(slice0) + lp.slices > 0xFFFFFFFFull)
                 return fail(RSX_ERR_INVALID_INPUT, "batch too large for one plan");

@@ -80,7 +80,7 @@ __device__ __forceinline__ void decode_slice(const LeoDev &f, const LeoDecodeArg
     load_rows<F>(w, a.rows, a.errlocs, a.n, col, f);
     ifft<F>(w, a.n, a.ifft_trunc, f.skew, -1, f);
     derivative<MAXD>(w, a.n, cols / 16);
-    fft<F>(w, a.n, a.fft_trunc, f);
+    fft<F>(w, a.n, a.fft_trunc, f, a.need);
     store_rows<F>(w, a.out, a.out_rows, a.out_logs, a.nout, col, f);
     __syncthreads();
 }

@@ -10,6 +10,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <vector>
 
 namespace rsx {
 
@@ -98,7 +99,16 @@ struct LeoDecodeArgs {
     size_t size;
     int n, nout, ifft_trunc, fft_trunc;
     int slice;
+    const uint32_t *need;  // FFT groups that hold an output row (leo_fft_need), or null: all
 };
+
+// FFT pruning (leopard.go:1306-1500 errorBitfield): for every FFT level of an
+// n-row transform, one bit per butterfly group, set when a row of the group
+// is an output row; groups without one are skipped (their rows are never
+// written out).  Levels in FFT order: radix-4 steps at dist = n/4, n/16, ...
+// (groups of 4*dist rows), then, for odd log2 n, the radix-2 layer (pairs).
+// Each level starts on a 32-bit word.
+std::vector<uint32_t> leo_fft_need(int n, const std::vector<int> &out_rows);
 
 // One Leopard item of a batched launch: op 0 = encode (enc), 1 = reconstruct
 // (dec); its column slices are global slices [slice0, slice0 + slices).
@@ -139,6 +149,7 @@ struct LeoMultipassArgs {
     int slices;                  // 64-byte slices in this chunk
     int n, ifft_trunc, fft_trunc;
     int C, P, A;                 // tile shapes from leo_multipass_shape
+    const uint32_t *need;        // leo_fft_need bits, or null
 };
 
 // Tile shapes for n rows; false when n is out of the multipass range.

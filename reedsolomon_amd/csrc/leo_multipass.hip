@@ -218,13 +218,19 @@ __global__ __launch_bounds__(kMB) void k_dec_pass3(const LeoDev f, const LeoMult
         }
         __syncthreads();
     }
-    // FFT steps with distance < P, then the final radix-2 layer.
+    // FFT steps with distance < P, then the final radix-2 layer; `need` levels
+    // are walked from the first FFT step so their offsets match.
     int dist4 = a.n, dist = a.n >> 2;
-    for (; dist != 0; dist4 = dist, dist >>= 2)
-        if (dist < a.P) step4<F, false>(w, a.P, base, log2i(dist), a.fft_trunc, f.skew, -1, f);
+    const uint32_t *need = a.need;
+    for (; dist != 0; dist4 = dist, dist >>= 2) {
+        if (dist < a.P) step4<F, false>(w, a.P, base, log2i(dist), a.fft_trunc, f.skew, -1, f, need);
+        if (need) need += need_words(a.n / (4 * dist));
+    }
     if (dist4 == 2 && a.fft_trunc > base) {
         const int pairs = min(a.P / 2, (a.fft_trunc - base + 1) / 2);
         for_rows(w, pairs, [&](int q, int u) {
+            const int pg = (base >> 1) + q;
+            if (need && !((need[pg >> 5] >> (pg & 31)) & 1u)) return;
             const int r = 2 * q;
             typename F::E x = Lds<F>::load(w, r, u), yv = Lds<F>::load(w, r + 1, u);
             fft2<F>(x, yv, f.skew[base + r], f);

@@ -129,15 +129,22 @@ __device__ __forceinline__ void for_units(const Tile &t, int nrows, Fn fn) {
 // groups r < mtrunc of a tile of `rows` consecutive rows whose local row 0 is
 // row `base` of the transform (skews and truncation use the global row), as
 // in k_step4.
+// `need` (one bit per group of 4*dist rows of the whole transform, or null)
+// skips the groups no output row depends on.
 template <class F, bool INV>
 __device__ __forceinline__ void step4(const Tile &w, int rows, int base, int dshift, int mtrunc,
-                                      const uint16_t *skew, int bias, const LeoDev &f) {
+                                      const uint16_t *skew, int bias, const LeoDev &f,
+                                      const uint32_t *need = nullptr) {
     const int dist = 1 << dshift;
     const int left = mtrunc - base;
     if (left <= 0) return;  // uniform across the block: no barrier is skipped by a subset
     const int groups = min(rows >> (dshift + 2), (left + 4 * dist - 1) >> (dshift + 2));
     for_rows(w, groups << dshift, [&](int q, int u) {
         const int r = (q >> dshift) << (dshift + 2);
+        if (need) {
+            const int g = (base + r) >> (dshift + 2);
+            if (!((need[g >> 5] >> (g & 31)) & 1u)) return;
+        }
         const int i = r + (q & (dist - 1));
         const int iend = base + r + dist;
         const unsigned lm01 = skew[iend + bias], lm02 = skew[iend + dist + bias],
@@ -186,17 +193,24 @@ __device__ __forceinline__ void ifft(const Tile &w, int n, int mtrunc, const uin
     }
 }
 
-// FFT over `n` rows truncated to `mtrunc` (skew = fftSkew, bias -1).
+// Words of one level of leo_fft_need bits with `groups` groups.
+__device__ __forceinline__ int need_words(int groups) { return (groups + 31) >> 5; }
+
+// FFT over `n` rows truncated to `mtrunc` (skew = fftSkew, bias -1); `need`
+// (leo_fft_need bits, or null) prunes groups without output rows.
 template <class F>
-__device__ __forceinline__ void fft(const Tile &w, int n, int mtrunc, const LeoDev &f) {
+__device__ __forceinline__ void fft(const Tile &w, int n, int mtrunc, const LeoDev &f,
+                                    const uint32_t *need = nullptr) {
     int dist4 = n, dist = n >> 2;
     while (dist != 0) {
-        step4<F, false>(w, n, 0, log2i(dist), mtrunc, f.skew, -1, f);
+        step4<F, false>(w, n, 0, log2i(dist), mtrunc, f.skew, -1, f, need);
+        if (need) need += need_words(n / (4 * dist));
         dist4 = dist;
         dist >>= 2;
     }
     if (dist4 == 2) {
         for_rows(w, (mtrunc + 1) / 2, [&](int q, int u) {
+            if (need && !((need[q >> 5] >> (q & 31)) & 1u)) return;
             const int r = 2 * q;
             typename F::E x = Lds<F>::load(w, r, u), y = Lds<F>::load(w, r + 1, u);
             fft2<F>(x, y, f.skew[r], f);

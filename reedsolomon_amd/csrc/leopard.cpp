@@ -68,7 +68,7 @@ struct LeoPattern {
     // rows of the erased shards, their reveal logs, and row -> output index.
     bool outs_ready = false;
     int nout = 0;
-    DeviceBuffer d_rows, d_logs, d_outidx;
+    DeviceBuffer d_rows, d_logs, d_outidx, d_need;
 };
 
 struct LeopardState {
@@ -176,6 +176,30 @@ bool all_aligned(const std::vector<uint8_t *> &p) {
 
 // Uploads a pointer table into `buf` (synchronously ordered on the stream;
 // the host copy stays alive in the state until the call has synchronised).
+}  // namespace
+
+std::vector<uint32_t> leo_fft_need(int n, const std::vector<int> &out_rows) {
+    std::vector<uint32_t> v;
+    auto level = [&](int gsize) {  // one bit per group of gsize rows
+        const size_t off = v.size();
+        v.resize(off + (n / gsize + 31) / 32, 0u);
+        for (int r : out_rows) {
+            const int g = r / gsize;
+            v[off + g / 32] |= 1u << (g % 32);
+        }
+    };
+    int dist4 = n, dist = n >> 2;
+    while (dist != 0) {  // the FFT's radix-4 steps (leopard.go:696-735)
+        level(4 * dist);
+        dist4 = dist;
+        dist >>= 2;
+    }
+    if (dist4 == 2) level(2);  // its final radix-2 layer
+    return v;
+}
+
+namespace {
+
 // Row pointer table of a call in s.ptrs; skipped when the device copy already
 // holds this table (repeated calls on the same buffers).  The copy is ordered
 // on the codec's stream after every kernel that read the previous table.
@@ -351,6 +375,10 @@ int reconstruct_device(rsx_codec *c, const std::vector<uint8_t *> &shards, const
             logs.push_back(static_cast<uint16_t>(s.field->modulus - pat->errlocs[pos]));
         }
         const int no = static_cast<int>(rows.size());
+        const std::vector<uint32_t> need = leo_fft_need(n, rows);
+        if ((err = pat->d_need.reserve(std::max<size_t>(need.size(), 1) * sizeof(uint32_t)))) return err;
+        RSX_HIP(hipMemcpyAsync(pat->d_need.ptr, need.data(), need.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                               c->stream));
         if ((err = pat->d_rows.reserve(std::max(no, 1) * sizeof(int)))) return err;
         if ((err = pat->d_logs.reserve(std::max(no, 1) * sizeof(uint16_t)))) return err;
         if ((err = pat->d_outidx.reserve(out_index.size() * sizeof(int)))) return err;
@@ -377,7 +405,7 @@ int reconstruct_device(rsx_codec *c, const std::vector<uint8_t *> &shards, const
     if (!slice && s.fused && s.bits == 16 && kQuarterTiles) slice = leo_quarter_slice(n);
     if (slice) {
         const LeoDecodeArgs a{lptr, pat->d_errlocs.as<uint16_t>(), pat->d_rows.as<int>(), pat->d_logs.as<uint16_t>(), optr,
-                              size, n, nout, ifft_trunc, m + k, slice};
+                              size, n, nout, ifft_trunc, m + k, slice, pat->d_need.as<uint32_t>()};
         if (nout) RSX_HIP(leo_decode_fused(*s.dev, a, c->stream));
         return RSX_OK;
     }
@@ -391,7 +419,8 @@ int reconstruct_device(rsx_codec *c, const std::vector<uint8_t *> &shards, const
             const size_t cols = std::min(cw, size - off);
             const LeoMultipassArgs a{lptr, pat->d_errlocs.as<uint16_t>(), pat->d_outidx.as<int>(), pat->d_logs.as<uint16_t>(),
                                      optr, s.work.as<uint8_t>(), s.work2.as<uint8_t>(), off,
-                                     static_cast<int>(cols / 64), n, ifft_trunc, m + k, mC, mP, mA};
+                                     static_cast<int>(cols / 64), n, ifft_trunc, m + k, mC, mP, mA,
+                                     pat->d_need.as<uint32_t>()};
             RSX_HIP(leo_decode_multipass(*s.dev, a, c->stream));
         }
         return RSX_OK;
@@ -689,6 +718,8 @@ int leopard_plan_item(rsx_codec *c, int op, uint8_t *const *shards, const size_t
     out.errl_off = aux_append(aux, pat->errlocs.data(), pat->errlocs.size() * sizeof(uint16_t));
     out.rows_off = aux_append(aux, rows.data(), rows.size() * sizeof(int));
     out.logs_off = aux_append(aux, logs.data(), logs.size() * sizeof(uint16_t));
+    const std::vector<uint32_t> need = leo_fft_need(n, rows);
+    out.need_off = aux_append(aux, need.data(), need.size() * sizeof(uint32_t));
     const int nout = static_cast<int>(rows.size());
     out.task.op = 1;
     out.task.dec = LeoDecodeArgs{nullptr, nullptr, nullptr, nullptr, nullptr, size, n, nout,

@@ -35,7 +35,7 @@ struct LeoItemPlan {
     size_t lds = 0;
     size_t regs = 0;  // bytes a workgroup holds in registers (derivative / group accumulator)
     uint32_t slices = 0;
-    size_t ptr_off = 0, errl_off = 0, rows_off = 0, logs_off = 0;
+    size_t ptr_off = 0, errl_off = 0, rows_off = 0, logs_off = 0, need_off = 0;
     size_t parity_pool = 0;  // encode: pool index of the first parity pointer
     int verify_item = -1;    // Verify: the plan item whose flag a parity mismatch sets
     bool empty = false;      // nothing to do (no erasure to fill)

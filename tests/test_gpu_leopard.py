@@ -196,6 +196,47 @@ def test_largest_in_field_gf16_shapes(k, m):
     assert np.array_equal(rebuilt, np.stack([want[i] for i in lost]))
 
 
+# Sparse erasures: the final FFT skips butterfly groups that hold no erased
+# row; every schedule (one tile GF(2^8) / GF(2^16), quarter tile n = 4096,
+# three passes n = 8192) against the oracle, per call and batched.
+SPARSE_CASES = [(8, 192, 56, 64 * 37), (16, 100, 20, 64 * 41), (16, 1000, 100, 64 * 3),
+                (16, 2000, 50, 64 * 2), (16, 2100, 1100, 128)]
+
+
+@pytest.mark.parametrize("bits,k,m,size", SPARSE_CASES)
+def test_sparse_erasures_prune_fft_exactly(bits, k, m, size):
+    rng = np.random.default_rng(size + 7 * k)
+    data = [rng.integers(0, 256, size, dtype=np.uint8) for _ in range(k)]
+    o, want = oracle_encode(bits, k, m, data)
+    enc = rs.New(k, m, opt(bits))
+    n = k + m
+    sets = [[0], [k - 1], [k], [n - 1], sorted(rng.choice(n, 2, replace=False).tolist()),
+            sorted(rng.choice(k, 3, replace=False).tolist())]
+    items = []
+    for lost in sets:
+        for data_only in (False, True):
+            shards = [None if i in lost else dev(want[i]) for i in range(n)]
+            (enc.ReconstructData if data_only else enc.Reconstruct)(shards)
+            for i in lost:
+                if data_only and i >= k:
+                    continue
+                assert np.array_equal(host(shards[i]), want[i]), (lost, data_only, i)
+        work = [dev(want[i]) if i not in lost else torch.zeros(size, dtype=torch.uint8, device="cuda")
+                for i in range(n)]
+        lens_zero = list(lost)
+        items.append((work, lens_zero))
+    # batched: one plan with every sparse pattern (shapes the batch kernel takes)
+    b = rs.Batch(0)
+    plan_items = [(enc, rs.OP_RECONSTRUCT, [None if i in lz else w[i] for i in range(n)], w) for w, lz in items]
+    status = b.run([(e, op, sh) for e, op, sh, _ in plan_items])
+    for (e, op, sh, w), st, (_, lz) in zip(plan_items, status, items):
+        if st == rs.ErrNotSupported.code:
+            continue
+        assert st == 0
+        for i in lz:
+            assert np.array_equal(host(sh[i]), want[i]), (lz, i)
+
+
 SCHEDULE_CASES = [(8, 192, 56, 64 * 37), (8, 17, 3, 64 * 129), (8, 128, 128, 64 * 5), (8, 3, 2, 64 * 1000),
                   (16, 100, 20, 64 * 41), (16, 300, 300, 64 * 3), (16, 33, 7, 64 * 300), (16, 5, 3, 64 * 777),
                   (16, 1000, 100, 64 * 600), (16, 1500, 600, 64 * 5), (16, 2000, 50, 64 * 2),
