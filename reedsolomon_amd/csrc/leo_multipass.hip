@@ -75,26 +75,30 @@ __device__ __forceinline__ void store_tile(const Tile &w, int nrows, RowPtr rp) 
 // One radix-4 step over tiles whose local row l = j*A + a is global row
 // lo0 + a + j*P: butterflies at global distance `dist` (>= P) connect local
 // rows j, j + dj, j + 2dj, j + 3dj with dj = dist / P.  `w2`, when given, gets
-// the same butterflies (same skews).
-template <class F, bool INV>
-__device__ __forceinline__ void strided_step4(const Tile &w, const Tile *w2, int ashift, int J, int P, int dist,
-                                              int mtrunc, const LeoDev &f) {
-    const int dj = dist / P;
-    const int gcount = min(J / (4 * dj), (mtrunc + 4 * dist - 1) / (4 * dist));
-    if (gcount <= 0) return;
-    const int nq = gcount * dj;  // quads per lo
-    const int qs = log2i(nq);
+// the same butterflies (same skews).  Lanes take a (the lo run) fastest, then
+// the unit, then the quad q; UNI: a wave's 64 lanes share one q (A << ushift
+// >= 64), so the skews are wave-uniform and the multiply records are scalar
+// loads.
+template <class F, bool INV, bool UNI>
+__device__ __forceinline__ void strided_quads(const Tile &w, const Tile *w2, int ashift, int P, int dist,
+                                              int dj, int nq, const LeoDev &f) {
     const int A = 1 << ashift;
-    const int items = (A << qs) << w.ushift;
+    const int items = (A << log2i(nq)) << w.ushift;
+    const int umask = (1 << w.ushift) - 1;
     for (int it = threadIdx.x; it < items; it += blockDim.x) {
         const int a = it & (A - 1);
-        const int q = (it >> ashift) & ((1 << qs) - 1);
-        const int u = it >> (ashift + qs);
+        const int u = (it >> ashift) & umask;
+        const int q = it >> (ashift + w.ushift);
         if (q >= nq || u >= w.ucount) continue;
         const int rj = (q / dj) * 4 * dj;
         const int j = rj + q % dj;
         const int iend = rj * P + dist;  // global r + dist
-        const unsigned lm01 = f.skew[iend - 1], lm02 = f.skew[iend + dist - 1], lm23 = f.skew[iend + 2 * dist - 1];
+        unsigned lm01 = f.skew[iend - 1], lm02 = f.skew[iend + dist - 1], lm23 = f.skew[iend + 2 * dist - 1];
+        if (UNI) {
+            lm01 = __builtin_amdgcn_readfirstlane(lm01);
+            lm02 = __builtin_amdgcn_readfirstlane(lm02);
+            lm23 = __builtin_amdgcn_readfirstlane(lm23);
+        }
         const int l0 = (j << ashift) + a, ld = dj << ashift;
         for (int t = 0; t < (w2 ? 2 : 1); ++t) {
             const Tile &x = t ? *w2 : w;
@@ -117,6 +121,17 @@ __device__ __forceinline__ void strided_step4(const Tile &w, const Tile *w2, int
             Lds<F>::store(x, l0 + 3 * ld, u, x3);
         }
     }
+}
+
+template <class F, bool INV>
+__device__ __forceinline__ void strided_step4(const Tile &w, const Tile *w2, int ashift, int J, int P, int dist,
+                                              int mtrunc, const LeoDev &f) {
+    const int dj = dist / P;
+    const int gcount = min(J / (4 * dj), (mtrunc + 4 * dist - 1) / (4 * dist));
+    if (gcount <= 0) return;
+    const int nq = gcount * dj;  // quads per lo
+    if (ashift + w.ushift >= 6) strided_quads<F, INV, true>(w, w2, ashift, P, dist, dj, nq, f);
+    else strided_quads<F, INV, false>(w, w2, ashift, P, dist, dj, nq, f);
     __syncthreads();
 }
 

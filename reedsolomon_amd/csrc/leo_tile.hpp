@@ -131,14 +131,15 @@ __device__ __forceinline__ void for_units(const Tile &t, int nrows, Fn fn) {
 // in k_step4.
 // `need` (one bit per group of 4*dist rows of the whole transform, or null)
 // skips the groups no output row depends on.
-template <class F, bool INV>
-__device__ __forceinline__ void step4(const Tile &w, int rows, int base, int dshift, int mtrunc,
-                                      const uint16_t *skew, int bias, const LeoDev &f,
-                                      const uint32_t *need = nullptr) {
+// UNI: dist >= 64, so the 64 consecutive q of a wave lie in one group and its
+// three skews are wave-uniform; taking them from the first lane puts the
+// multiply records in scalar registers (scalar loads) instead of 64 per-lane
+// vector loads of the same 80 bytes.
+template <class F, bool INV, bool UNI>
+__device__ __forceinline__ void step4_groups(const Tile &w, int base, int dshift, int groups,
+                                             const uint16_t *skew, int bias, const LeoDev &f,
+                                             const uint32_t *need) {
     const int dist = 1 << dshift;
-    const int left = mtrunc - base;
-    if (left <= 0) return;  // uniform across the block: no barrier is skipped by a subset
-    const int groups = min(rows >> (dshift + 2), (left + 4 * dist - 1) >> (dshift + 2));
     for_rows(w, groups << dshift, [&](int q, int u) {
         const int r = (q >> dshift) << (dshift + 2);
         if (need) {
@@ -147,8 +148,12 @@ __device__ __forceinline__ void step4(const Tile &w, int rows, int base, int dsh
         }
         const int i = r + (q & (dist - 1));
         const int iend = base + r + dist;
-        const unsigned lm01 = skew[iend + bias], lm02 = skew[iend + dist + bias],
-                       lm23 = skew[iend + 2 * dist + bias];
+        unsigned lm01 = skew[iend + bias], lm02 = skew[iend + dist + bias], lm23 = skew[iend + 2 * dist + bias];
+        if (UNI) {
+            lm01 = __builtin_amdgcn_readfirstlane(lm01);
+            lm02 = __builtin_amdgcn_readfirstlane(lm02);
+            lm23 = __builtin_amdgcn_readfirstlane(lm23);
+        }
         typename F::E x0 = Lds<F>::load(w, i, u), x1 = Lds<F>::load(w, i + dist, u),
                       x2 = Lds<F>::load(w, i + 2 * dist, u), x3 = Lds<F>::load(w, i + 3 * dist, u);
         if (INV) {
@@ -167,6 +172,17 @@ __device__ __forceinline__ void step4(const Tile &w, int rows, int base, int dsh
         Lds<F>::store(w, i + 2 * dist, u, x2);
         Lds<F>::store(w, i + 3 * dist, u, x3);
     });
+}
+
+template <class F, bool INV>
+__device__ __forceinline__ void step4(const Tile &w, int rows, int base, int dshift, int mtrunc,
+                                      const uint16_t *skew, int bias, const LeoDev &f,
+                                      const uint32_t *need = nullptr) {
+    const int left = mtrunc - base;
+    if (left <= 0) return;  // uniform across the block: no barrier is skipped by a subset
+    const int groups = min(rows >> (dshift + 2), (left + (4 << dshift) - 1) >> (dshift + 2));
+    if (dshift >= 6) step4_groups<F, INV, true>(w, base, dshift, groups, skew, bias, f, need);
+    else step4_groups<F, INV, false>(w, base, dshift, groups, skew, bias, f, need);
     __syncthreads();
 }
 
