@@ -279,8 +279,12 @@ bool leo_multipass_shape(int n, int *C, int *P, int *A) {
     return true;
 }
 
+// Column chunk of one pass sequence: each work array is at most 512 MiB, so
+// k=4096 m=1024 shards of 64 KiB run as one chunk (three launches per call).
+// Fewer, larger launches beat Infinity-Cache-sized chunks: 8 chunks of 8 KiB
+// (64 MiB arrays) took 1.69 ms for that reconstruct, one chunk 1.52 ms.
 size_t leo_multipass_chunk(int n, size_t size) {
-    size_t cw = (size_t{64} << 20) / static_cast<size_t>(n);  // two work arrays within 128 MiB
+    size_t cw = (size_t{512} << 20) / static_cast<size_t>(n);
     cw -= cw % kSlice;
     cw = std::max<size_t>(cw, kSlice);
     return std::min(cw, size);

@@ -145,6 +145,34 @@ def test_leopard16_4096_1024_64kib():
         assert torch.equal(work[i], orig[i]), i
 
 
+def test_leopard16_three_pass_column_chunks():
+    # n = 8192 reconstructs run the three passes per column chunk of at most
+    # 64 KiB (512 MiB work arrays): 2 x 64 KiB + 192 bytes is three chunks, the
+    # last one three 64-byte slices wide.
+    k, m, size = 4096, 1024, 2 * (64 << 10) + 192
+    g = torch.Generator(device="cuda")
+    g.manual_seed(8192)
+    buf = torch.empty((k + m, size), dtype=torch.uint8, device="cuda")
+    buf[:k].random_(0, 256, generator=g)
+    buf[k:].zero_()
+    shards = [buf[i] for i in range(k + m)]
+    enc = rs.New(k, m, rs.WithLeopardGF16(True))
+    enc.Encode(shards)
+    sl = slice(size - 128, size)  # the ragged last chunk against the oracle
+    cols = buf[:, sl].cpu().numpy()
+    _, want = oracle_encode(16, k, m, [cols[i].copy() for i in range(k)])
+    for i in range(m):
+        assert np.array_equal(cols[k + i], want[k + i]), i
+    orig = buf.clone()
+    rng = np.random.default_rng(8192)
+    lost = sorted(rng.choice(k + m, m, replace=False).tolist())
+    gone = set(lost)
+    work = [None if i in gone else shards[i] for i in range(k + m)]
+    enc.Reconstruct(work)
+    for i in lost:
+        assert torch.equal(work[i], orig[i]), i
+
+
 def test_leopard8_192_56_64kib():
     # The Leopard GF(2^8) slice of the served stream at (192, 56) 64 KiB
     # ((200, 56) indexes past the reference's tables).
@@ -164,12 +192,6 @@ def test_leopard8_192_56_64kib():
         assert np.array_equal(host(work[i]), want[i]), i
 
 
-# Shapes x sizes that give several column slices and a narrower last slice,
-# run on both schedules: rows of a column slice in LDS (the default: one
-# tile where the rows fit, three tiled passes for taller GF(2^16)
-# reconstructs) and one launch per butterfly layer (RSX_LEOPARD_LAYERED=1).
-# GF(2^16) with n >= 2048 rows reconstructs in three LDS-tiled passes; 64*600
-# bytes at n = 2048 spans two column chunks of the work arrays.
 @pytest.mark.parametrize("k,m", [(32768, 32768), (64512, 1024)])
 def test_largest_in_field_gf16_shapes(k, m):
     # k + ceilPow2(m) = 65536: every field position is a shard.  (32768, 32768)
@@ -237,6 +259,11 @@ def test_sparse_erasures_prune_fft_exactly(bits, k, m, size):
             assert np.array_equal(host(sh[i]), want[i]), (lz, i)
 
 
+# Shapes x sizes that give several column slices and a narrower last slice,
+# run on both schedules: rows of a column slice in LDS (the default: one
+# tile where the rows fit, a quarter-chunk tile for GF(2^16) reconstructs
+# with n = 4096, three tiled passes for n >= 8192) and one launch per
+# butterfly layer (RSX_LEOPARD_LAYERED=1).
 SCHEDULE_CASES = [(8, 192, 56, 64 * 37), (8, 17, 3, 64 * 129), (8, 128, 128, 64 * 5), (8, 3, 2, 64 * 1000),
                   (16, 100, 20, 64 * 41), (16, 300, 300, 64 * 3), (16, 33, 7, 64 * 300), (16, 5, 3, 64 * 777),
                   (16, 1000, 100, 64 * 600), (16, 1500, 600, 64 * 5), (16, 2000, 50, 64 * 2),
