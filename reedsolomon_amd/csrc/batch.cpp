@@ -78,12 +78,13 @@ struct rsx_batch {
 
     // Leopard items, per field (0: GF(2^8), 1: GF(2^16)), run by the
     // LDS-resident batch kernel after the classic passes.
-    std::vector<rsx::LeoItemPlan> leo[2];
-    std::vector<rsx::LeoBatchTask> leo_tasks[2];
-    DeviceBuffer d_leo_tasks[2], d_leo_aux;
-    size_t leo_lds[2] = {0, 0};
-    size_t leo_deriv[2] = {0, 0};  // largest register bytes (LeoItemPlan::regs) of the tasks
-    uint32_t leo_slices[2] = {0, 0};
+    // Leopard items by launch kind: GF(2^8), GF(2^16), GF(2^16) quarter-chunk tiles.
+    std::vector<rsx::LeoItemPlan> leo[3];
+    std::vector<rsx::LeoBatchTask> leo_tasks[3];
+    DeviceBuffer d_leo_tasks[3], d_leo_aux;
+    size_t leo_lds[3] = {0, 0, 0};
+    size_t leo_deriv[3] = {0, 0, 0};  // largest register bytes (LeoItemPlan::regs) of the tasks
+    uint32_t leo_slices[3] = {0, 0, 0};
 };
 
 namespace {
@@ -176,7 +177,7 @@ int enqueue(rsx_batch *b, hipStream_t s) {
     const bool verify = b->has_verify;
     if (verify)
         RSX_HIP(launch_fill_u32(b->d_flags.as<uint32_t>(), static_cast<size_t>(b->nitems), 0u, s));
-    const bool leo = !b->leo_tasks[0].empty() || !b->leo_tasks[1].empty();
+    const bool leo = !b->leo_tasks[0].empty() || !b->leo_tasks[1].empty() || !b->leo_tasks[2].empty();
     const bool fork = leo && !b->launches.empty();
     const int spread = rsx_batch::kSpreadClasses
                            ? std::min<int>(static_cast<int>(b->launches.size()) - 1, rsx_batch::kClassStreams)
@@ -188,12 +189,12 @@ int enqueue(rsx_batch *b, hipStream_t s) {
         RSX_HIP(hipStreamWaitEvent(b->side, b->fork, 0));
         ls = b->side;
     }
-    for (int fi = 0; fi < 2; ++fi) {
+    for (int fi = 0; fi < 3; ++fi) {
         if (b->leo_tasks[fi].empty()) continue;
         const unsigned g = std::min<unsigned>(static_cast<unsigned>(std::max(1, b->cus * 8)), b->leo_slices[fi]);
         RSX_HIP(leo_batch_launch(*b->leo[fi][0].dev, b->d_leo_tasks[fi].as<LeoBatchTask>(),
                                  static_cast<int>(b->leo_tasks[fi].size()), b->leo_slices[fi], b->leo_lds[fi], g,
-                                 b->leo_deriv[fi], ls));
+                                 b->leo_deriv[fi], fi == 2, ls));
     }
     if (fork) RSX_HIP(hipEventRecord(b->join, b->side));
     int next_side = 0;
@@ -353,7 +354,7 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
     b->has_verify = false;
     b->nitems = n;
     b->is_verify.assign(static_cast<size_t>(std::max(n, 0)), 0);
-    for (int fi = 0; fi < 2; ++fi) {
+    for (int fi = 0; fi < 3; ++fi) {
         b->leo[fi].clear();
         b->leo_tasks[fi].clear();
         b->leo_lds[fi] = 0;
@@ -406,7 +407,7 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
                 lp.verify_item = it;
                 b->is_verify[it] = 1;
             }
-            if (!lp.empty) b->leo[lp.bits == 16 ? 1 : 0].push_back(lp);
+            if (!lp.empty) b->leo[lp.kind].push_back(lp);
             continue;
         }
         const int k = c->k, total = c->total;
@@ -560,7 +561,7 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
         if ((err = b->d_leo_aux.reserve(laux.size()))) return err;
         RSX_HIP(hipMemcpyAsync(b->d_leo_aux.ptr, laux.data(), laux.size(), hipMemcpyHostToDevice, b->stream));
     }
-    for (int fi = 0; fi < 2; ++fi) {
+    for (int fi = 0; fi < 3; ++fi) {
         uint32_t slice0 = 0;
         for (const LeoItemPlan &lp : b->leo[fi]) {
             LeoBatchTask t = lp.task;

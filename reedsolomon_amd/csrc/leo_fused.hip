@@ -126,8 +126,11 @@ __global__ __launch_bounds__(kFB) void k_leo_batch(const LeoDev f, const LeoBatc
         const uint32_t end = lo + 1 < ntasks ? tasks[lo + 1].slice0 : nslices;
         const uint32_t l = g - tasks[lo].slice0, count = end - tasks[lo].slice0;
         if (tasks[lo].op == 0) {
-            const LeoEncodeArgs a = tasks[lo].enc;
-            encode_slice<F, MAXD>(f, a, lds, slice_start(l, count, a.slice));
+            // Encode tasks use whole-chunk or unit tiles only.
+            if constexpr (!std::is_same<F, F16Q>::value) {
+                const LeoEncodeArgs a = tasks[lo].enc;
+                encode_slice<F, MAXD>(f, a, lds, slice_start(l, count, a.slice));
+            }
         } else {
             const LeoDecodeArgs a = tasks[lo].dec;
             decode_slice<F, MAXD>(f, a, lds, slice_start(l, count, a.slice));
@@ -209,7 +212,7 @@ hipError_t leo_encode_fused(const LeoDev &f, const LeoEncodeArgs &a, hipStream_t
 }
 
 hipError_t leo_batch_launch(const LeoDev &f, const LeoBatchTask *tasks, int ntasks, uint32_t nslices, size_t lds,
-                            unsigned grid, size_t deriv_bytes, hipStream_t s) {
+                            unsigned grid, size_t deriv_bytes, bool quarter, hipStream_t s) {
     if (ntasks <= 0 || nslices == 0) return hipSuccess;
     // The smaller register budget when every task's derivative (reconstruct)
     // or group accumulator (encode) fits it.
@@ -220,7 +223,10 @@ hipError_t leo_batch_launch(const LeoDev &f, const LeoBatchTask *tasks, int ntas
         if ((e = allow_lds(k_leo_batch<FT, MD>, lds)) != hipSuccess) return e;                       \
         hipLaunchKernelGGL((k_leo_batch<FT, MD>), dim3(grid), dim3(kFB), lds, s, f, tasks, ntasks, nslices); \
     } while (0)
-    if (f.bits == 16) {
+    if (quarter) {
+        if (f.bits != 16) return hipErrorInvalidValue;
+        RSX_LAUNCH(F16Q, kDerivMax);
+    } else if (f.bits == 16) {
         if (small) RSX_LAUNCH(F16, kDerivMax / 2);
         else RSX_LAUNCH(F16, kDerivMax);
     } else {

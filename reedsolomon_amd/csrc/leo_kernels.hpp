@@ -122,8 +122,10 @@ struct LeoBatchTask {
 // Runs every task's slices (tasks sorted by slice0, all of one field) with
 // `grid` persistent blocks of 256 threads and `lds` bytes (the largest task).
 // deriv_bytes: the largest n * slice of the reconstruct tasks (0 if none).
+// quarter: GF(2^16) reconstruct tasks whose slices are 16-byte quarter
+// chunks (every task of the launch).
 hipError_t leo_batch_launch(const LeoDev &f, const LeoBatchTask *tasks, int ntasks, uint32_t nslices, size_t lds,
-                            unsigned grid, size_t deriv_bytes, hipStream_t s);
+                            unsigned grid, size_t deriv_bytes, bool quarter, hipStream_t s);
 
 // Slice width for `regions` LDS regions of `rows` rows, radix-4 steps over
 // `step_rows` rows and shards of `size` bytes, or 0 when even the narrowest

@@ -631,6 +631,7 @@ int leopard_plan_item(rsx_codec *c, int op, uint8_t *const *shards, const size_t
     LeopardState &s = *c->leo;
     out = LeoItemPlan{};
     out.bits = s.bits;
+    out.kind = s.bits == 16 ? 1 : 0;
     out.dev = s.dev;
     if (nshards != total) return fail(RSX_ERR_TOO_FEW_SHARDS, "too few shards given");
     const bool rec = op == RSX_OP_RECONSTRUCT || op == RSX_OP_RECONSTRUCT_DATA;
@@ -692,8 +693,9 @@ int leopard_plan_item(rsx_codec *c, int op, uint8_t *const *shards, const size_t
         (void)f;
         return RSX_OK;
     }
-    const int slice = leo_fused_slice(s.bits, n, 1, n, kBatchedSize, std::min(kLeoFusedEncodeLds,
-                                                                               kLeoFusedDecodeLds / n * (n + 1)));
+    int slice = leo_fused_slice(s.bits, n, 1, n, kBatchedSize, std::min(kLeoFusedEncodeLds,
+                                                                         kLeoFusedDecodeLds / n * (n + 1)));
+    if (!slice && s.bits == 16 && kQuarterTiles && (slice = leo_quarter_slice(n)) != 0) out.kind = 2;
     if (!slice) return fail(RSX_ERR_NOT_SUPPORTED, "Leopard shape too tall for the batched kernel");
     LeoPattern *pat = nullptr;
     {

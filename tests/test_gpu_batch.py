@@ -169,8 +169,11 @@ def _leo_codeword(rng, bits, k, m, size):
     return full
 
 
+# GF(2^16) (2000, 50) and (1100, 1000) reconstruct n = 4096 work rows in
+# quarter-chunk tiles (a launch of their own inside the plan).
 LEO_CASES = [(8, 192, 56, 65536), (8, 10, 3, 64 * 37), (8, 64, 64, 4096), (16, 100, 20, 64 * 41),
-             (16, 300, 300, 192), (16, 33, 7, 64 * 300), (16, 1000, 100, 128)]
+             (16, 300, 300, 192), (16, 33, 7, 64 * 300), (16, 1000, 100, 128), (16, 2000, 50, 64 * 5),
+             (16, 1100, 1000, 64 * 3)]
 
 
 def test_batched_leopard_items_match_oracle_and_plain():
@@ -231,8 +234,8 @@ def test_batched_leopard_items_match_oracle_and_plain():
 
 def test_batched_leopard_unsupported_items_are_reported():
     rng = np.random.default_rng(9)
-    # GF(2^16) reconstruct with n = 4096 work rows: not one LDS tile.
-    k, m, size = 3000, 100, 128
+    # GF(2^16) reconstruct with n = 8192 work rows: not one LDS tile.
+    k, m, size = 5000, 100, 128
     enc = rs.New(k, m, rs.WithLeopardGF16(True))
     full = _leo_codeword(rng, 16, k, m, size)
     work = [None if i < 3 else torch.from_numpy(full[i]).to("cuda") for i in range(k + m)]
