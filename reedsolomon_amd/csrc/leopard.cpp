@@ -174,8 +174,6 @@ bool all_aligned(const std::vector<uint8_t *> &p) {
     return (bits & 15) == 0;
 }
 
-// Uploads a pointer table into `buf` (synchronously ordered on the stream;
-// the host copy stays alive in the state until the call has synchronised).
 }  // namespace
 
 std::vector<uint32_t> leo_fft_need(int n, const std::vector<int> &out_rows) {
@@ -203,6 +201,10 @@ namespace {
 // Row pointer table of a call in s.ptrs; skipped when the device copy already
 // holds this table (repeated calls on the same buffers).  The copy is ordered
 // on the codec's stream after every kernel that read the previous table.
+// Uploads the call's device pointer table, ordered on the codec stream.  A
+// copy from pageable host memory has consumed its source when the call
+// returns; the host copy is kept only to skip re-uploading an unchanged
+// table (repeated calls on the same buffers).
 int upload_ptrs(rsx_codec *c, std::vector<uint8_t *> &&p) {
     LeopardState &s = *c->leo;
     const size_t bytes = std::max<size_t>(p.size(), 1) * sizeof(void *);

@@ -302,3 +302,43 @@ def test_lds_and_layered_schedules_match_oracle(bits, k, m, size, monkeypatch):
         enc.Reconstruct(work)
         for i in range(k + m):
             assert np.array_equal(host(work[i]), ref[i]), (layered, i)
+
+
+@pytest.mark.parametrize("bits,k,m", [(16, 100, 20), (8, 32, 16)])
+def test_one_leopard_codec_shared_by_threads(bits, k, m):
+    # An Encoder is safe for concurrent use (reedsolomon_test.go:2158-2196):
+    # threads encode, verify and reconstruct different stripes (different
+    # pointer tables and erasure patterns) with one Leopard codec.
+    import threading
+    size = 64 * 9
+    enc = rs.New(k, m, opt(bits))
+    rng = np.random.default_rng(k + bits)
+    stripes = [oracle_encode(bits, k, m, [rng.integers(0, 256, size, dtype=np.uint8) for _ in range(k)])[1]
+               for _ in range(6)]
+    errors = []
+
+    def work(t):
+        try:
+            for rep in range(4):
+                ref = stripes[(t + rep) % len(stripes)]
+                shards = [dev(x) for x in ref[:k]] + [torch.zeros(size, dtype=torch.uint8, device="cuda")
+                                                      for _ in range(m)]
+                enc.Encode(shards)
+                if not enc.Verify(shards):
+                    errors.append((t, rep, "verify"))
+                lost = {(t * 7 + rep) % (k + m), (t + 5 * rep + 1) % (k + m), (3 * t + rep + 2) % (k + m)}
+                for i in lost:
+                    shards[i] = None
+                enc.Reconstruct(shards)
+                for i in range(k + m):
+                    if not np.array_equal(host(shards[i]), ref[i]):
+                        errors.append((t, rep, i))
+        except Exception as e:  # surfaced below
+            errors.append(repr(e))
+
+    threads = [threading.Thread(target=work, args=(t,)) for t in range(6)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    assert not errors, errors[:5]
