@@ -179,7 +179,8 @@ static unsigned threads_for(int bits, int rows, int slice, size_t lds) {
 
 // Only while two workgroups fit per CU: at n = 8192 (one 512-thread
 // workgroup per CU) the three-pass schedule is faster (1.79 vs 1.87 ms for
-// k=4096 m=1024 64 KiB); at n = 4096 the quarter tile is 3-6% faster.
+// k=4096 m=1024 64 KiB with the passes in 8 column chunks, 1.55 ms in one);
+// at n = 4096 the quarter tile is 3-6% faster.
 int leo_quarter_slice(int n) {
     const size_t lds = static_cast<size_t>(16) * (n + 1);
     const unsigned threads = threads_for(16, n, 16, lds);
