@@ -188,12 +188,21 @@ int leo_quarter_slice(int n) {
     return lds <= (80u << 10) && deriv <= static_cast<size_t>(kDerivMax) * 16 * threads ? 16 : 0;
 }
 
+int leo_quarter_encode_slice(int m, int groups) {
+    const size_t lds = static_cast<size_t>(16) * (m + 1);
+    const unsigned threads = threads_for(16, m, 16, lds);
+    const size_t acc = groups > 1 ? static_cast<size_t>(m) * 16 : 0;
+    return lds <= kLeoFusedEncodeLds && acc <= static_cast<size_t>(kDerivMax) * 16 * threads ? 16 : 0;
+}
+
 hipError_t leo_encode_fused(const LeoDev &f, const LeoEncodeArgs &a, hipStream_t s) {
-    // Several groups accumulate m * slice bytes in registers.
+    // Several groups accumulate m * slice bytes in registers (kDerivMax
+    // 16-byte slots per thread).
     const size_t acc = a.groups > 1 ? static_cast<size_t>(a.m) * a.slice : 0;
-    if (acc > kLeoFusedDecodeLds) return hipErrorInvalidValue;
     const size_t lds = static_cast<size_t>(a.slice) * (a.m + 1);
     const unsigned threads = threads_for(f.bits, a.m, a.slice, lds);
+    if (acc > static_cast<size_t>(kDerivMax) * 16 * threads) return hipErrorInvalidValue;
+    if (a.slice == 16 && (f.bits != 16 || lds > kLeoFusedEncodeLds)) return hipErrorInvalidValue;
     const bool small = acc <= kLeoDerivSmall;
     hipError_t e;
 #define RSX_LAUNCH(FT, MD)                                                                              \
@@ -201,7 +210,9 @@ hipError_t leo_encode_fused(const LeoDev &f, const LeoEncodeArgs &a, hipStream_t
         if ((e = allow_lds(k_encode_fused<FT, MD>, lds)) != hipSuccess) return e;                        \
         hipLaunchKernelGGL((k_encode_fused<FT, MD>), dim3(grid_for(a.size, a.slice)), dim3(threads), lds, s, f, a); \
     } while (0)
-    if (f.bits == 16) {
+    if (a.slice == 16) {
+        RSX_LAUNCH(F16Q, kDerivMax);
+    } else if (f.bits == 16) {
         if (small) RSX_LAUNCH(F16, kDerivMax / 2);
         else RSX_LAUNCH(F16, kDerivMax);
     } else {

@@ -134,6 +134,10 @@ int leo_fused_slice(int bits, int rows, int regions, int step_rows, size_t size,
 // 16 when a GF(2^16) reconstruct of n work rows runs as one LDS tile of
 // quarter-chunk slices (8 elements of every row; n = 4096), else 0.
 int leo_quarter_slice(int n);
+// 16 when a GF(2^16) encode with `groups` groups of m rows runs as one LDS
+// tile of quarter-chunk slices (m up to 8192; with several groups their XOR
+// is held in registers, 16 slots per thread), else 0.
+int leo_quarter_encode_slice(int m, int groups);
 hipError_t leo_encode_fused(const LeoDev &f, const LeoEncodeArgs &a, hipStream_t s);
 hipError_t leo_decode_fused(const LeoDev &f, const LeoDecodeArgs &a, hipStream_t s);
 
