@@ -126,11 +126,8 @@ __global__ __launch_bounds__(kFB) void k_leo_batch(const LeoDev f, const LeoBatc
         const uint32_t end = lo + 1 < ntasks ? tasks[lo + 1].slice0 : nslices;
         const uint32_t l = g - tasks[lo].slice0, count = end - tasks[lo].slice0;
         if (tasks[lo].op == 0) {
-            // Encode tasks use whole-chunk or unit tiles only.
-            if constexpr (!std::is_same<F, F16Q>::value) {
-                const LeoEncodeArgs a = tasks[lo].enc;
-                encode_slice<F, MAXD>(f, a, lds, slice_start(l, count, a.slice));
-            }
+            const LeoEncodeArgs a = tasks[lo].enc;
+            encode_slice<F, MAXD>(f, a, lds, slice_start(l, count, a.slice));
         } else {
             const LeoDecodeArgs a = tasks[lo].dec;
             decode_slice<F, MAXD>(f, a, lds, slice_start(l, count, a.slice));

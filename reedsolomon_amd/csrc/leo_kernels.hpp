@@ -122,8 +122,8 @@ struct LeoBatchTask {
 // Runs every task's slices (tasks sorted by slice0, all of one field) with
 // `grid` persistent blocks of 256 threads and `lds` bytes (the largest task).
 // deriv_bytes: the largest n * slice of the reconstruct tasks (0 if none).
-// quarter: GF(2^16) reconstruct tasks whose slices are 16-byte quarter
-// chunks (every task of the launch).
+// quarter: GF(2^16) tasks whose slices are 16-byte quarter chunks (every
+// task of the launch).
 hipError_t leo_batch_launch(const LeoDev &f, const LeoBatchTask *tasks, int ntasks, uint32_t nslices, size_t lds,
                             unsigned grid, size_t deriv_bytes, bool quarter, hipStream_t s);
 

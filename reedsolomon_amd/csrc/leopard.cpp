@@ -684,7 +684,12 @@ int leopard_plan_item(rsx_codec *c, int op, uint8_t *const *shards, const size_t
     if (!rec) {
         const int groups = (k + m - 1) / m;
         // Other items fill the device: size the slice for full workgroups.
-        const int slice = leo_fused_slice(s.bits, m, 1, m, kBatchedSize, encode_lds_budget(m, groups));
+        int slice = leo_fused_slice(s.bits, m, 1, m, kBatchedSize, encode_lds_budget(m, groups));
+        // Quarter-chunk tiles; the batch kernel's 256 threads hold the groups'
+        // XOR up to m = 4096 (one group: up to m = 8192).
+        if (!slice && s.bits == 16 && kQuarterTiles && (groups == 1 || m <= 4096) &&
+            (slice = leo_quarter_encode_slice(m, groups)) != 0)
+            out.kind = 2;
         if (!slice) return fail(RSX_ERR_NOT_SUPPORTED, "Leopard shape too tall for the batched kernel");
         out.ptr_off = pool.size();
         for (int g = 0; g < groups; ++g)

@@ -30,7 +30,7 @@ int leopard_reconstruct(rsx_codec *c, uint8_t *const *shards, const size_t *lens
 // `logs_off` (bytes) into the plan's auxiliary buffer.
 struct LeoItemPlan {
     int bits = 0;
-    int kind = 0;  // batch launch: 0 GF(2^8), 1 GF(2^16), 2 GF(2^16) quarter-chunk tiles (n = 4096 reconstruct)
+    int kind = 0;  // batch launch: 0 GF(2^8), 1 GF(2^16), 2 GF(2^16) quarter-chunk tiles
     const LeoDev *dev = nullptr;
     LeoBatchTask task{};
     size_t lds = 0;
@@ -47,8 +47,9 @@ struct LeoItemPlan {
 // _RECONSTRUCT_DATA) and plans it for the LDS-resident batch kernel.
 // Returns the item's status; RSX_ERR_NOT_SUPPORTED where the batched form
 // does not apply (shards not 16-byte aligned, or rows that do not fit one
-// LDS tile of whole chunks, units or, for GF(2^16) reconstructs with 4096
-// work rows, quarter chunks).
+// LDS tile of whole chunks, units or quarter chunks: GF(2^16) reconstructs
+// with more than 4096 work rows, encodes with m = ceilPow2(parity) above
+// 8192, or above 4096 with several groups).
 int leopard_plan_item(rsx_codec *c, int op, uint8_t *const *shards, const size_t *lens, int nshards,
                       std::vector<const uint8_t *> &pool, std::vector<uint8_t> &aux, LeoItemPlan &out,
                       uint8_t *filled);

@@ -232,6 +232,26 @@ def test_batched_leopard_items_match_oracle_and_plain():
     assert b.num_tasks() > 0
 
 
+@pytest.mark.parametrize("k,m,size", [(5000, 3000, 64 * 2), (3000, 5000, 64 * 3)])
+def test_batched_tall_gf16_encode_and_verify_match_oracle(k, m, size):
+    # m = ceilPow2(parity) of 4096 with two groups, and 8192 with one: the
+    # quarter-chunk launch of the plan.
+    rng = np.random.default_rng(k)
+    enc = rs.New(k, m, rs.WithLeopardGF16(True))
+    full = _leo_codeword(rng, 16, k, m, size)
+    dev = [torch.from_numpy(x).to("cuda") for x in full]
+    stale = dev[:k] + [torch.zeros(size, dtype=torch.uint8, device="cuda") for _ in range(m)]
+    good = [x.clone() for x in dev]
+    bad = [x.clone() for x in dev]
+    bad[k + m - 1][size - 1] ^= 0x40
+    b = rs.Batch(0)
+    st = b.run([(enc, rs.OP_ENCODE, stale), (enc, rs.OP_VERIFY, good), (enc, rs.OP_VERIFY, bad)])
+    assert st == [0, 0, 0], st
+    for i in range(k, k + m):
+        assert np.array_equal(stale[i].cpu().numpy(), full[i]), i
+    assert b.verify_result(1) and not b.verify_result(2)
+
+
 def test_batched_leopard_unsupported_items_are_reported():
     rng = np.random.default_rng(9)
     # GF(2^16) reconstruct with n = 8192 work rows: not one LDS tile.
