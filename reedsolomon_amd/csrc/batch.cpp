@@ -37,6 +37,9 @@ struct rsx_batch {
     // to the launch stream, so their tails overlap other classes instead of
     // idling the device; large classes run in order on the launch stream.
     static constexpr int kClassStreams = 3;
+    // Work arrays of a plan's three-pass Leopard items (two of n x shard
+    // bytes each); items past the cap get ErrNotSupported.
+    static constexpr size_t kPassWorkCap = size_t{4} << 30;
     static constexpr bool kSpreadClasses = true;
     hipStream_t cls[kClassStreams] = {};
     hipEvent_t cls_join[kClassStreams] = {};
@@ -85,6 +88,15 @@ struct rsx_batch {
     size_t leo_lds[3] = {0, 0, 0};
     size_t leo_deriv[3] = {0, 0, 0};  // largest register bytes (LeoItemPlan::regs) of the tasks
     uint32_t leo_slices[3] = {0, 0, 0};
+
+    // GF(2^16) reconstructs with 8192 or more work rows: three launches (one
+    // per pass) over all such items, each with its two work arrays in d_mp_work.
+    std::vector<rsx::LeoItemPlan> leo_mp;
+    std::vector<rsx::LeoPassTask> mp_tasks;
+    DeviceBuffer d_mp_tasks, d_mp_work;
+    size_t mp_work_bytes = 0;
+    uint32_t mp_blocks[3] = {0, 0, 0};
+    int mp_max_p = 0;
 };
 
 namespace {
@@ -177,7 +189,8 @@ int enqueue(rsx_batch *b, hipStream_t s) {
     const bool verify = b->has_verify;
     if (verify)
         RSX_HIP(launch_fill_u32(b->d_flags.as<uint32_t>(), static_cast<size_t>(b->nitems), 0u, s));
-    const bool leo = !b->leo_tasks[0].empty() || !b->leo_tasks[1].empty() || !b->leo_tasks[2].empty();
+    const bool leo = !b->leo_tasks[0].empty() || !b->leo_tasks[1].empty() || !b->leo_tasks[2].empty() ||
+                     !b->mp_tasks.empty();
     const bool fork = leo && !b->launches.empty();
     const int spread = rsx_batch::kSpreadClasses
                            ? std::min<int>(static_cast<int>(b->launches.size()) - 1, rsx_batch::kClassStreams)
@@ -196,6 +209,9 @@ int enqueue(rsx_batch *b, hipStream_t s) {
                                  static_cast<int>(b->leo_tasks[fi].size()), b->leo_slices[fi], b->leo_lds[fi], g,
                                  b->leo_deriv[fi], fi == 2, ls));
     }
+    if (!b->mp_tasks.empty())
+        RSX_HIP(leo_decode_multipass_batch(*b->leo_mp[0].dev, b->d_mp_tasks.as<LeoPassTask>(),
+                                           static_cast<int>(b->mp_tasks.size()), b->mp_blocks, b->mp_max_p, ls));
     if (fork) RSX_HIP(hipEventRecord(b->join, b->side));
     int next_side = 0;
     for (size_t i = 0; i < b->launches.size(); ++i) {
@@ -361,6 +377,11 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
         b->leo_deriv[fi] = 0;
         b->leo_slices[fi] = 0;
     }
+    b->leo_mp.clear();
+    b->mp_tasks.clear();
+    b->mp_work_bytes = 0;
+    b->mp_max_p = 0;
+    for (uint32_t &x : b->mp_blocks) x = 0;
     std::vector<uint8_t> laux;  // Leopard error locators / output rows / logs
     // Every table the plan reads is copied into `htab` (host) when first
     // seen and uploaded once; keyed by (host table, codec, decode-cache
@@ -407,7 +428,22 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
                 lp.verify_item = it;
                 b->is_verify[it] = 1;
             }
-            if (!lp.empty) b->leo[lp.kind].push_back(lp);
+            if (lp.empty) continue;
+            if (lp.kind == 3) {
+                // Work arrays of the three-pass items, within a cap per plan.
+                const size_t wb = (lp.work_bytes + 255) / 256 * 256;
+                if (b->mp_work_bytes + wb > rsx_batch::kPassWorkCap) {
+                    if (fl) std::memset(fl, 0, c->total);
+                    b->is_verify[it] = 0;
+                    status[it] = fail(RSX_ERR_NOT_SUPPORTED, "three-pass work arrays over the plan's cap");
+                    continue;
+                }
+                lp.mp.work = reinterpret_cast<uint8_t *>(b->mp_work_bytes);  // offset, patched below
+                b->mp_work_bytes += wb;
+                b->leo_mp.push_back(lp);
+            } else {
+                b->leo[lp.kind].push_back(lp);
+            }
             continue;
         }
         const int k = c->k, total = c->total;
@@ -596,6 +632,36 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
         RSX_HIP(hipMemcpyAsync(b->d_leo_tasks[fi].ptr, b->leo_tasks[fi].data(), bytes, hipMemcpyHostToDevice,
                                b->stream));
     }
+    if (!b->leo_mp.empty()) {
+        if ((err = b->d_mp_work.reserve(b->mp_work_bytes))) return err;
+        uint8_t *aux = b->d_leo_aux.as<uint8_t>();
+        for (const LeoItemPlan &lp : b->leo_mp) {
+            LeoPassTask t{};
+            t.a = lp.mp;
+            uint8_t *const *pp = b->d_ptrs.as<uint8_t *>() + lp.ptr_off;
+            t.a.rows = pp;
+            t.a.out = pp + t.a.n;
+            t.a.errlocs = reinterpret_cast<const uint16_t *>(aux + lp.errl_off);
+            t.a.out_index = reinterpret_cast<const int *>(aux + lp.outidx_off);
+            t.a.out_logs = reinterpret_cast<const uint16_t *>(aux + lp.logs_off);
+            t.a.need = reinterpret_cast<const uint32_t *>(aux + lp.need_off);
+            t.a.work = b->d_mp_work.as<uint8_t>() + reinterpret_cast<size_t>(lp.mp.work);
+            t.a.work2 = t.a.work + static_cast<size_t>(t.a.n) * static_cast<size_t>(t.a.slices) * 64;
+            uint32_t nb[3];
+            leo_pass_blocks(t.a, nb);
+            for (int p = 0; p < 3; ++p) {
+                if (static_cast<uint64_t>(b->mp_blocks[p]) + nb[p] > 0x7FFFFFFFull)
+                    return fail(RSX_ERR_INVALID_INPUT, "batch too large for one plan");
+                t.block0[p] = b->mp_blocks[p];
+                b->mp_blocks[p] += nb[p];
+            }
+            b->mp_max_p = std::max(b->mp_max_p, t.a.P);
+            b->mp_tasks.push_back(t);
+        }
+        const size_t bytes = b->mp_tasks.size() * sizeof(LeoPassTask);
+        if ((err = b->d_mp_tasks.reserve(bytes))) return err;
+        RSX_HIP(hipMemcpyAsync(b->d_mp_tasks.ptr, b->mp_tasks.data(), bytes, hipMemcpyHostToDevice, b->stream));
+    }
     RSX_HIP(hipStreamSynchronize(b->stream));
     if (b->tuned) {
         if ((err = build_launches(b, b->tile))) return err;
@@ -642,6 +708,7 @@ int rsx_batch_num_tasks(const rsx_batch *b) {
     size_t n = 0;
     for (auto &v : b->tasks) n += v.size();
     for (auto &v : b->leo_tasks) n += v.size();
+    n += b->mp_tasks.size();
     return static_cast<int>(n);
 }
 

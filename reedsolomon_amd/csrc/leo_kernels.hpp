@@ -160,8 +160,20 @@ struct LeoMultipassArgs {
 
 // Tile shapes for n rows; false when n is out of the multipass range.
 bool leo_multipass_shape(int n, int *C, int *P, int *A);
-// Column chunk (bytes) whose two work arrays stay within 128 MiB.
+// Column chunk (bytes) whose two work arrays stay within 512 MiB each.
 size_t leo_multipass_chunk(int n, size_t size);
 hipError_t leo_decode_multipass(const LeoDev &f, const LeoMultipassArgs &a, hipStream_t s);
+
+// Batched three-pass reconstructs: every task's whole column (col0 = 0) in
+// one launch per pass; block0[p] is the task's first workgroup in pass p
+// (tasks sorted by it), its workgroup count leo_pass_blocks.
+struct LeoPassTask {
+    LeoMultipassArgs a;
+    uint32_t block0[3];
+};
+void leo_pass_blocks(const LeoMultipassArgs &a, uint32_t blocks[3]);
+// blocks: total workgroups per pass; max_p: the largest P of the tasks.
+hipError_t leo_decode_multipass_batch(const LeoDev &f, const LeoPassTask *tasks, int ntasks, const uint32_t blocks[3],
+                                      int max_p, hipStream_t s);
 
 }  // namespace rsx

@@ -135,11 +135,11 @@ __device__ __forceinline__ void strided_step4(const Tile &w, const Tile *w2, int
     __syncthreads();
 }
 
+// Pass bodies: workgroup (s, by) = 64-byte column slice s, row block by.
 template <class F>
-__global__ __launch_bounds__(kMB1) void k_dec_pass1(const LeoDev f, const LeoMultipassArgs a) {
+__device__ __forceinline__ void pass1(const LeoDev &f, const LeoMultipassArgs &a, int s, int by) {
     extern __shared__ uint4 lds_raw[];
-    const int s = blockIdx.x;
-    const int base = blockIdx.y * a.C;
+    const int base = by * a.C;
     const Tile w{reinterpret_cast<uint8_t *>(lds_raw), a.C + 1, 1, 2};
     load_rows<F>(w, a.rows + base, a.errlocs + base, a.C, a.col0 + static_cast<size_t>(s) * kSlice, f);
     for (int dshift = 0; (4 << dshift) <= a.C; dshift += 2)
@@ -148,10 +148,9 @@ __global__ __launch_bounds__(kMB1) void k_dec_pass1(const LeoDev f, const LeoMul
 }
 
 template <class F>
-__global__ __launch_bounds__(kMB2) void k_dec_pass2(const LeoDev f, const LeoMultipassArgs a) {
+__device__ __forceinline__ void pass2(const LeoDev &f, const LeoMultipassArgs &a, int s, int by) {
     extern __shared__ uint4 lds_raw[];
-    const int s = blockIdx.x;
-    const int lo0 = blockIdx.y * a.A;
+    const int lo0 = by * a.A;
     const int J = a.n / a.P;
     const int ashift = log2i(a.A), jshift = log2i(J);
     const int R = a.A * J;
@@ -201,10 +200,9 @@ __global__ __launch_bounds__(kMB2) void k_dec_pass2(const LeoDev f, const LeoMul
 }
 
 template <class F, int MAXD>
-__global__ __launch_bounds__(kMB) void k_dec_pass3(const LeoDev f, const LeoMultipassArgs a) {
+__device__ __forceinline__ void pass3(const LeoDev &f, const LeoMultipassArgs &a, int s, int by) {
     extern __shared__ uint4 lds_raw[];
-    const int s = blockIdx.x;
-    const int base = blockIdx.y * a.P;
+    const int base = by * a.P;
     const Tile w{reinterpret_cast<uint8_t *>(lds_raw), a.P + 1, 1, 2};
     load_tile<F>(w, a.P, [&](int l) { return wrow(a.work2, a.n, s, base + l); });  // v
     __syncthreads();
@@ -265,7 +263,62 @@ __global__ __launch_bounds__(kMB) void k_dec_pass3(const LeoDev f, const LeoMult
     });
 }
 
+template <class F>
+__global__ __launch_bounds__(kMB1) void k_dec_pass1(const LeoDev f, const LeoMultipassArgs a) {
+    pass1<F>(f, a, blockIdx.x, blockIdx.y);
+}
+template <class F>
+__global__ __launch_bounds__(kMB2) void k_dec_pass2(const LeoDev f, const LeoMultipassArgs a) {
+    pass2<F>(f, a, blockIdx.x, blockIdx.y);
+}
+template <class F, int MAXD>
+__global__ __launch_bounds__(kMB) void k_dec_pass3(const LeoDev f, const LeoMultipassArgs a) {
+    pass3<F, MAXD>(f, a, blockIdx.x, blockIdx.y);
+}
+
+// Batched: one 1-D grid per pass over the workgroups of every task; a block
+// finds its task by binary search over the tasks' first blocks of this pass,
+// then its (slice, row block) with the slice fastest, as in the 2-D grids.
+template <int PASS>
+__device__ __forceinline__ int pass_task(const LeoPassTask *tasks, int ntasks, uint32_t g) {
+    int lo = 0, hi = ntasks - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tasks[mid].block0[PASS] <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+template <class F>
+__global__ __launch_bounds__(kMB1) void k_dec_pass1_batch(const LeoDev f, const LeoPassTask *tasks, int ntasks) {
+    const int t = pass_task<0>(tasks, ntasks, blockIdx.x);
+    const LeoMultipassArgs a = tasks[t].a;
+    const uint32_t l = blockIdx.x - tasks[t].block0[0];
+    pass1<F>(f, a, static_cast<int>(l % a.slices), static_cast<int>(l / a.slices));
+}
+template <class F>
+__global__ __launch_bounds__(kMB2) void k_dec_pass2_batch(const LeoDev f, const LeoPassTask *tasks, int ntasks) {
+    const int t = pass_task<1>(tasks, ntasks, blockIdx.x);
+    const LeoMultipassArgs a = tasks[t].a;
+    const uint32_t l = blockIdx.x - tasks[t].block0[1];
+    pass2<F>(f, a, static_cast<int>(l % a.slices), static_cast<int>(l / a.slices));
+}
+template <class F, int MAXD>
+__global__ __launch_bounds__(kMB) void k_dec_pass3_batch(const LeoDev f, const LeoPassTask *tasks, int ntasks) {
+    const int t = pass_task<2>(tasks, ntasks, blockIdx.x);
+    const LeoMultipassArgs a = tasks[t].a;
+    const uint32_t l = blockIdx.x - tasks[t].block0[2];
+    pass3<F, MAXD>(f, a, static_cast<int>(l % a.slices), static_cast<int>(l / a.slices));
+}
+
 }  // namespace
+
+void leo_pass_blocks(const LeoMultipassArgs &a, uint32_t blocks[3]) {
+    const uint32_t sl = static_cast<uint32_t>(a.slices);
+    blocks[0] = sl * static_cast<uint32_t>(a.n / a.C);
+    blocks[1] = sl * static_cast<uint32_t>(a.P / a.A);
+    blocks[2] = sl * static_cast<uint32_t>(a.n / a.P);
+}
 
 bool leo_multipass_shape(int n, int *C, int *P, int *A) {
     if (n < 2048) return false;
@@ -316,6 +369,34 @@ hipError_t leo_decode_multipass(const LeoDev &f, const LeoMultipassArgs &a, hipS
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (small) hipLaunchKernelGGL((k_dec_pass3<F16, kDerivMax / 2>), dim3(slices, a.n / a.P), dim3(kMB), lds3, s, f, a);
     else hipLaunchKernelGGL((k_dec_pass3<F16, kDerivMax>), dim3(slices, a.n / a.P), dim3(kMB), lds3, s, f, a);
+    return hipGetLastError();
+}
+
+hipError_t leo_decode_multipass_batch(const LeoDev &f, const LeoPassTask *tasks, int ntasks, const uint32_t blocks[3],
+                                      int max_p, hipStream_t s) {
+    if (f.bits != 16 || ntasks <= 0) return hipErrorInvalidValue;
+    const size_t lds1 = static_cast<size_t>(kSlice) * (1024 + 1);
+    const size_t lds2 = 2 * static_cast<size_t>(kSlice) * (512 + 1);
+    const size_t lds3 = static_cast<size_t>(kSlice) * (max_p + 1);
+    hipError_t e;
+    auto allow = [](const void *k, size_t lds) {
+        return lds <= (64u << 10) ? hipSuccess
+                                   : hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                         static_cast<int>(lds));
+    };
+    const bool small = max_p * (kSlice / 16) <= (kDerivMax / 2) * kMB;
+    if ((e = allow(reinterpret_cast<const void *>(k_dec_pass1_batch<F16>), lds1)) != hipSuccess) return e;
+    if ((e = allow(reinterpret_cast<const void *>(k_dec_pass2_batch<F16>), lds2)) != hipSuccess) return e;
+    if ((e = allow(small ? reinterpret_cast<const void *>(k_dec_pass3_batch<F16, kDerivMax / 2>)
+                         : reinterpret_cast<const void *>(k_dec_pass3_batch<F16, kDerivMax>),
+                   lds3)) != hipSuccess)
+        return e;
+    hipLaunchKernelGGL(k_dec_pass1_batch<F16>, dim3(blocks[0]), dim3(kMB1), lds1, s, f, tasks, ntasks);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_dec_pass2_batch<F16>, dim3(blocks[1]), dim3(kMB2), lds2, s, f, tasks, ntasks);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (small) hipLaunchKernelGGL((k_dec_pass3_batch<F16, kDerivMax / 2>), dim3(blocks[2]), dim3(kMB), lds3, s, f, tasks, ntasks);
+    else hipLaunchKernelGGL((k_dec_pass3_batch<F16, kDerivMax>), dim3(blocks[2]), dim3(kMB), lds3, s, f, tasks, ntasks);
     return hipGetLastError();
 }
 

@@ -710,7 +710,9 @@ int leopard_plan_item(rsx_codec *c, int op, uint8_t *const *shards, const size_t
     int slice = leo_fused_slice(s.bits, n, 1, n, kBatchedSize, std::min(kLeoFusedEncodeLds,
                                                                          kLeoFusedDecodeLds / n * (n + 1)));
     if (!slice && s.bits == 16 && kQuarterTiles && (slice = leo_quarter_slice(n)) != 0) out.kind = 2;
-    if (!slice) return fail(RSX_ERR_NOT_SUPPORTED, "Leopard shape too tall for the batched kernel");
+    int mC = 0, mP = 0, mA = 0;
+    const bool passes = !slice && s.bits == 16 && s.fused && leo_multipass_shape(n, &mC, &mP, &mA);
+    if (!slice && !passes) return fail(RSX_ERR_NOT_SUPPORTED, "Leopard shape too tall for the batched kernel");
     LeoPattern *pat = nullptr;
     {
         std::lock_guard<std::mutex> lock(c->mu);
@@ -737,6 +739,19 @@ int leopard_plan_item(rsx_codec *c, int op, uint8_t *const *shards, const size_t
     const std::vector<uint32_t> need = leo_fft_need(n, rows);
     out.need_off = aux_append(aux, need.data(), need.size() * sizeof(uint32_t));
     const int nout = static_cast<int>(rows.size());
+    if (passes) {
+        // Three passes over the whole column in the batch's work arrays.
+        std::vector<int> out_index(static_cast<size_t>(n), -1);
+        for (int q = 0; q < nout; ++q) out_index[rows[q]] = q;
+        out.outidx_off = aux_append(aux, out_index.data(), out_index.size() * sizeof(int));
+        out.kind = 3;
+        out.mp = LeoMultipassArgs{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0,
+                                  static_cast<int>(size / 64), n, pat->input_count, m + k, mC, mP, mA, nullptr};
+        out.work_bytes = 2 * static_cast<size_t>(n) * size;
+        out.slices = static_cast<uint32_t>(size / 64);
+        out.empty = nout == 0;
+        return RSX_OK;
+    }
     out.task.op = 1;
     out.task.dec = LeoDecodeArgs{nullptr, nullptr, nullptr, nullptr, nullptr, size, n, nout,
                                  s.bits == 16 ? pat->input_count : m + k, m + k, slice};

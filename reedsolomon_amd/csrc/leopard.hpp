@@ -30,7 +30,12 @@ int leopard_reconstruct(rsx_codec *c, uint8_t *const *shards, const size_t *lens
 // `logs_off` (bytes) into the plan's auxiliary buffer.
 struct LeoItemPlan {
     int bits = 0;
-    int kind = 0;  // batch launch: 0 GF(2^8), 1 GF(2^16), 2 GF(2^16) quarter-chunk tiles
+    // batch launch: 0 GF(2^8), 1 GF(2^16), 2 GF(2^16) quarter-chunk tiles,
+    // 3 GF(2^16) three-pass reconstruct (n >= 8192 work rows)
+    int kind = 0;
+    LeoMultipassArgs mp{};    // kind 3: pointers are pool / aux offsets until the plan patches them
+    size_t outidx_off = 0;    // kind 3: aux offset of the n-entry output index
+    size_t work_bytes = 0;    // kind 3: bytes of its two work arrays
     const LeoDev *dev = nullptr;
     LeoBatchTask task{};
     size_t lds = 0;
@@ -46,10 +51,10 @@ struct LeoItemPlan {
 // RSX_OP_ENCODE / _VERIFY (planned as an encode) / _RECONSTRUCT /
 // _RECONSTRUCT_DATA) and plans it for the LDS-resident batch kernel.
 // Returns the item's status; RSX_ERR_NOT_SUPPORTED where the batched form
-// does not apply (shards not 16-byte aligned, or rows that do not fit one
-// LDS tile of whole chunks, units or quarter chunks: GF(2^16) reconstructs
-// with more than 4096 work rows, encodes with m = ceilPow2(parity) above
-// 8192, or above 4096 with several groups).
+// does not apply (shards not 16-byte aligned, or GF(2^16) encodes with
+// m = ceilPow2(parity) above 8192, or above 4096 with several groups).
+// GF(2^16) reconstructs with 8192 or more work rows are planned as three-pass
+// items (kind 3).
 int leopard_plan_item(rsx_codec *c, int op, uint8_t *const *shards, const size_t *lens, int nshards,
                       std::vector<const uint8_t *> &pool, std::vector<uint8_t> &aux, LeoItemPlan &out,
                       uint8_t *filled);

@@ -252,13 +252,48 @@ def test_batched_tall_gf16_encode_and_verify_match_oracle(k, m, size):
     assert b.verify_result(1) and not b.verify_result(2)
 
 
+def test_batched_three_pass_reconstructs_rebuild_the_codeword():
+    # GF(2^16) reconstructs with 8192, 16384 and 32768 work rows (P = 512,
+    # 1024, 512) in one plan beside a one-tile Leopard item and a classic
+    # item: one launch per pass over all three-pass items.
+    rng = np.random.default_rng(31)
+    stripes = []
+    for k, m, size, op in ((5000, 100, 128, rs.OP_RECONSTRUCT), (12000, 3000, 64, rs.OP_RECONSTRUCT_DATA),
+                           (9000, 5000, 64, rs.OP_RECONSTRUCT), (100, 20, 64 * 3, rs.OP_RECONSTRUCT)):
+        enc = rs.New(k, m, rs.WithLeopardGF16(True))
+        full = _leo_codeword(rng, 16, k, m, size)
+        stripes.append((enc, k, m, full, set(rng.choice(k + m, m, replace=False).tolist()), op))
+    cdata = [rng.integers(0, 256, 4096, dtype=np.uint8) for _ in range(10)]
+    cref = cdata + [np.zeros(4096, np.uint8) for _ in range(4)]
+    rsref.Classic(10, 4).encode(cref)
+    stripes.append((rs.New(10, 4), 10, 4, cref, {2, 11}, rs.OP_RECONSTRUCT))
+    b = rs.Batch(0)
+    for use_graph in (True, False):
+        b.set_graph(use_graph)
+        items = []
+        for enc, k, m, full, lost, op in stripes:
+            items.append((enc, op, [None if i in lost else torch.from_numpy(full[i]).to("cuda")
+                                    for i in range(k + m)]))
+        st = b.run(items)
+        assert st == [0] * len(items), st
+        for idx, ((enc, k, m, full, lost, op), (_, _, work)) in enumerate(zip(stripes, items)):
+            for i in lost:
+                if op == rs.OP_RECONSTRUCT or i < k:
+                    assert np.array_equal(work[i].cpu().numpy(), full[i]), (use_graph, idx, i)
+                else:
+                    assert work[i] is None, (use_graph, idx, i)
+
+
 def test_batched_leopard_unsupported_items_are_reported():
     rng = np.random.default_rng(9)
-    # GF(2^16) reconstruct with n = 8192 work rows: not one LDS tile.
-    k, m, size = 5000, 100, 128
+    # GF(2^16) reconstruct on shards that are not 16-byte aligned.
+    k, m, size = 100, 20, 128
     enc = rs.New(k, m, rs.WithLeopardGF16(True))
     full = _leo_codeword(rng, 16, k, m, size)
-    work = [None if i < 3 else torch.from_numpy(full[i]).to("cuda") for i in range(k + m)]
+    base16 = torch.zeros((k + m) * size + 1, dtype=torch.uint8, device="cuda")
+    work = [None if i < 3 else base16[1 + i * size: 1 + (i + 1) * size] for i in range(k + m)]
+    for i in range(3, k + m):
+        work[i].copy_(torch.from_numpy(full[i]))
     # unaligned GF(2^8) encode
     enc8 = rs.New(4, 2, rs.WithLeopardGF(True))
     pool = torch.zeros(6 * 64 + 1, dtype=torch.uint8, device="cuda")
