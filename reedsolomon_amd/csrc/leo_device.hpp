@@ -90,7 +90,9 @@ struct F16 {
         return a.x | a.y | a.z | a.w | b.x | b.y | b.z | b.w;
     }
     // (lo, hi) of one dword pair times the table's element, XORed into (xl, xh).
-    __device__ static void mac1(uint32_t &xl, uint32_t &xh, uint32_t L, uint32_t H, ConstU32 *t) {
+    // T: the multiply record in memory (ConstU32 *) or in registers.
+    template <class T>
+    __device__ static void mac1(uint32_t &xl, uint32_t &xh, uint32_t L, uint32_t H, const T &t) {
         const uint32_t a0 = L & 0x07070707u, a1 = (L >> 3) & 0x07070707u, a2 = (L >> 6) & 0x03030303u;
         const uint32_t b0 = H & 0x07070707u, b1 = (H >> 3) & 0x07070707u, b2 = (H >> 6) & 0x03030303u;
         uint32_t r = xor3(xl, sel(t[1], t[0], a0), sel(t[5], t[4], a1));
@@ -100,7 +102,8 @@ struct F16 {
         r = xor3(r, sel(t[9], t[9], a2), sel(t[13], t[12], b0));
         xh = xor3(r, sel(t[17], t[16], b1), sel(t[19], t[19], b2));
     }
-    __device__ static void mul_add(E &x, const E &y, ConstU32 *t) {
+    template <class T>
+    __device__ static void mul_add(E &x, const E &y, const T &t) {
         mac1(x.lo.x, x.hi.x, y.lo.x, y.hi.x, t);
         mac1(x.lo.y, x.hi.y, y.lo.y, y.hi.y, t);
         mac1(x.lo.z, x.hi.z, y.lo.z, y.hi.z, t);
@@ -136,7 +139,8 @@ struct F16Q {
     __device__ static uint32_t differ(const E &x, const E &y) {
         return (x.lo.x ^ y.lo.x) | (x.lo.y ^ y.lo.y) | (x.hi.x ^ y.hi.x) | (x.hi.y ^ y.hi.y);
     }
-    __device__ static void mul_add(E &x, const E &y, ConstU32 *t) {
+    template <class T>
+    __device__ static void mul_add(E &x, const E &y, const T &t) {
         F16::mac1(x.lo.x, x.hi.x, y.lo.x, y.hi.x, t);
         F16::mac1(x.lo.y, x.hi.y, y.lo.y, y.hi.y, t);
     }
@@ -163,11 +167,13 @@ struct F8 {
         const uint4 a = xor4(x, y);
         return a.x | a.y | a.z | a.w;
     }
-    __device__ static uint32_t mac1(uint32_t x, uint32_t v, ConstU32 *t) {
+    template <class T>
+    __device__ static uint32_t mac1(uint32_t x, uint32_t v, const T &t) {
         const uint32_t g0 = v & 0x07070707u, g1 = (v >> 3) & 0x07070707u, g2 = (v >> 6) & 0x03030303u;
         return xor3(xor3(x, sel(t[1], t[0], g0), sel(t[3], t[2], g1)), sel(t[4], t[4], g2), 0u);
     }
-    __device__ static void mul_add(E &x, const E &y, ConstU32 *t) {
+    template <class T>
+    __device__ static void mul_add(E &x, const E &y, const T &t) {
         x.x = mac1(x.x, y.x, t);
         x.y = mac1(x.y, y.y, t);
         x.z = mac1(x.z, y.z, t);
@@ -191,6 +197,17 @@ template <class F>
 __device__ __forceinline__ void ifft2(typename F::E &x, typename F::E &y, unsigned lm, const LeoDev &f) {
     F::xor_into(y, x);
     if (lm != F::kMod) F::mul_add(x, y, table<F>(f, lm));
+}
+// The same with the multiply record of lm already in registers.
+template <class F>
+__device__ __forceinline__ void fft2r(typename F::E &x, typename F::E &y, unsigned lm, const uint32_t *t) {
+    if (lm != F::kMod) F::mul_add(x, y, t);
+    F::xor_into(y, x);
+}
+template <class F>
+__device__ __forceinline__ void ifft2r(typename F::E &x, typename F::E &y, unsigned lm, const uint32_t *t) {
+    F::xor_into(y, x);
+    if (lm != F::kMod) F::mul_add(x, y, t);
 }
 
 }  // namespace leo

@@ -49,7 +49,7 @@ __device__ __forceinline__ void encode_slice(const LeoDev &f, const LeoEncodeArg
     for (int g = 0; g < a.groups; ++g) {
         const int mtrunc = g == 0 ? (a.k < a.m ? a.k : a.m) : (a.k - g * a.m < a.m ? a.k - g * a.m : a.m);
         load_rows<F>(w, a.data + static_cast<size_t>(g) * a.m, nullptr, a.m, col, f);
-        ifft<F>(w, a.m, mtrunc, f.skew + a.m - 1 + g * a.m, 0, f);
+        ifft<F, true>(w, a.m, mtrunc, f.skew + a.m - 1 + g * a.m, 0, f);
         if (a.groups == 1) break;
         const bool first = g == 0, last = g == a.groups - 1;
 #pragma unroll
@@ -64,7 +64,7 @@ __device__ __forceinline__ void encode_slice(const LeoDev &f, const LeoEncodeArg
         }
         __syncthreads();  // before the next group's rows (or the FFT) reuse the tile
     }
-    fft<F>(w, a.m, a.p, f);
+    fft<F, true>(w, a.m, a.p, f);
     if (a.flag) compare_rows<F>(w, a.parity, a.p, col, a.flag, a.flag_value);
     else store_rows<F>(w, a.parity, nullptr, nullptr, a.p, col, f);
     __syncthreads();  // the next slice reuses the rows

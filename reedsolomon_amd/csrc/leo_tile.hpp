@@ -131,11 +131,43 @@ __device__ __forceinline__ void for_units(const Tile &t, int nrows, Fn fn) {
 // in k_step4.
 // `need` (one bit per group of 4*dist rows of the whole transform, or null)
 // skips the groups no output row depends on.
+// Lane `SRC` of each aligned group of G lanes (a DPP quad for G = 4, a DPP
+// row for G = 16) broadcast to the group.
+template <int G, int SRC>
+__device__ __forceinline__ uint32_t group_bcast(uint32_t v) {
+    if constexpr (G == 4) return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), SRC * 0x55, 0xF, 0xF, false));
+    else return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x150 + SRC, 0xF, 0xF, false));
+}
+
+// The three multiply records of a butterfly group whose G lanes are aligned
+// and all active: lanes 0-2 of the group load one record each, the group
+// shares them by broadcasts (one record load per lane instead of three).
+template <class F, int G>
+__device__ __forceinline__ void group_records(const LeoDev &f, unsigned lm01, unsigned lm02, unsigned lm23,
+                                              uint32_t (&rec)[3][F::kTW]) {
+    const int r = threadIdx.x & (G - 1);
+    uint32_t v[F::kTW];
+#pragma unroll
+    for (int o = 0; o < F::kTW; ++o) v[o] = 0;
+    if (r < 3) {
+        ConstU32 *t = table<F>(f, r == 0 ? lm01 : r == 1 ? lm02 : lm23);
+#pragma unroll
+        for (int o = 0; o < F::kTW; ++o) v[o] = t[o];
+    }
+#pragma unroll
+    for (int o = 0; o < F::kTW; ++o) {
+        rec[0][o] = group_bcast<G, 0>(v[o]);
+        rec[1][o] = group_bcast<G, 1>(v[o]);
+        rec[2][o] = group_bcast<G, 2>(v[o]);
+    }
+}
+
 // UNI: dist >= 64, so the 64 consecutive q of a wave lie in one group and its
 // three skews are wave-uniform; taking them from the first lane puts the
 // multiply records in scalar registers (scalar loads) instead of 64 per-lane
 // vector loads of the same 80 bytes.
-template <class F, bool INV, bool UNI>
+// SHARED (G = dist = 4 or 16 lanes per group): records shared by group_records.
+template <class F, bool INV, bool UNI, int SHARED = 0>
 __device__ __forceinline__ void step4_groups(const Tile &w, int base, int dshift, int groups,
                                              const uint16_t *skew, int bias, const LeoDev &f,
                                              const uint32_t *need) {
@@ -156,7 +188,21 @@ __device__ __forceinline__ void step4_groups(const Tile &w, int base, int dshift
         }
         typename F::E x0 = Lds<F>::load(w, i, u), x1 = Lds<F>::load(w, i + dist, u),
                       x2 = Lds<F>::load(w, i + 2 * dist, u), x3 = Lds<F>::load(w, i + 3 * dist, u);
-        if (INV) {
+        if constexpr (SHARED != 0) {
+            uint32_t rec[3][F::kTW];
+            group_records<F, SHARED>(f, lm01, lm02, lm23, rec);
+            if (INV) {
+                ifft2r<F>(x0, x1, lm01, rec[0]);
+                ifft2r<F>(x2, x3, lm23, rec[2]);
+                ifft2r<F>(x0, x2, lm02, rec[1]);
+                ifft2r<F>(x1, x3, lm02, rec[1]);
+            } else {
+                fft2r<F>(x0, x2, lm02, rec[1]);
+                fft2r<F>(x1, x3, lm02, rec[1]);
+                fft2r<F>(x0, x1, lm01, rec[0]);
+                fft2r<F>(x2, x3, lm23, rec[2]);
+            }
+        } else if (INV) {
             ifft2<F>(x0, x1, lm01, f);
             ifft2<F>(x2, x3, lm23, f);
             ifft2<F>(x0, x2, lm02, f);
@@ -174,7 +220,10 @@ __device__ __forceinline__ void step4_groups(const Tile &w, int base, int dshift
     });
 }
 
-template <class F, bool INV>
+// SHR: groups of 4 / 16 lanes share their multiply records (group_records).
+// It pays in the encoder (C3 encode 0.38 -> 0.37 ms, (2000,50) 64 KiB 0.31 ->
+// 0.26 ms); in the decoders the extra registers cost about as much as it saves.
+template <class F, bool INV, bool SHR = false>
 __device__ __forceinline__ void step4(const Tile &w, int rows, int base, int dshift, int mtrunc,
                                       const uint16_t *skew, int bias, const LeoDev &f,
                                       const uint32_t *need = nullptr) {
@@ -182,18 +231,20 @@ __device__ __forceinline__ void step4(const Tile &w, int rows, int base, int dsh
     if (left <= 0) return;  // uniform across the block: no barrier is skipped by a subset
     const int groups = min(rows >> (dshift + 2), (left + (4 << dshift) - 1) >> (dshift + 2));
     if (dshift >= 6) step4_groups<F, INV, true>(w, base, dshift, groups, skew, bias, f, need);
+    else if (SHR && dshift == 4) step4_groups<F, INV, false, 16>(w, base, dshift, groups, skew, bias, f, need);
+    else if (SHR && dshift == 2) step4_groups<F, INV, false, 4>(w, base, dshift, groups, skew, bias, f, need);
     else step4_groups<F, INV, false>(w, base, dshift, groups, skew, bias, f, need);
     __syncthreads();
 }
 
 // IFFT over `n` rows truncated to `mtrunc`; `skew` + `bias` as in leo_step4,
 // the final radix-2 layer (odd log2 n) uses skew[dist + bias].
-template <class F>
+template <class F, bool SHR = false>
 __device__ __forceinline__ void ifft(const Tile &w, int n, int mtrunc, const uint16_t *skew, int bias,
                                      const LeoDev &f) {
     int dshift = 0;
     while ((4 << dshift) <= n) {
-        step4<F, true>(w, n, 0, dshift, mtrunc, skew, bias, f);
+        step4<F, true, SHR>(w, n, 0, dshift, mtrunc, skew, bias, f);
         dshift += 2;
     }
     const int dist = 1 << dshift;
@@ -214,12 +265,12 @@ __device__ __forceinline__ int need_words(int groups) { return (groups + 31) >> 
 
 // FFT over `n` rows truncated to `mtrunc` (skew = fftSkew, bias -1); `need`
 // (leo_fft_need bits, or null) prunes groups without output rows.
-template <class F>
+template <class F, bool SHR = false>
 __device__ __forceinline__ void fft(const Tile &w, int n, int mtrunc, const LeoDev &f,
                                     const uint32_t *need = nullptr) {
     int dist4 = n, dist = n >> 2;
     while (dist != 0) {
-        step4<F, false>(w, n, 0, log2i(dist), mtrunc, f.skew, -1, f, need);
+        step4<F, false, SHR>(w, n, 0, log2i(dist), mtrunc, f.skew, -1, f, need);
         if (need) need += need_words(n / (4 * dist));
         dist4 = dist;
         dist >>= 2;
