@@ -85,7 +85,12 @@ __device__ __forceinline__ void strided_quads(const Tile &w, const Tile *w2, int
     const int A = 1 << ashift;
     const int items = (A << log2i(nq)) << w.ushift;
     const int umask = (1 << w.ushift) - 1;
-    for (int it = threadIdx.x; it < items; it += blockDim.x) {
+    // With w2 the two tiles are separate items (the second tile's after the
+    // first's): a step over both keeps twice the lanes busy.  items is a
+    // multiple of 64 where UNI holds, so a wave stays within one tile.
+    for (int it2 = threadIdx.x; it2 < (w2 ? 2 * items : items); it2 += blockDim.x) {
+        const int t = it2 >= items;
+        const int it = it2 - t * items;
         const int a = it & (A - 1);
         const int u = (it >> ashift) & umask;
         const int q = it >> (ashift + w.ushift);
@@ -100,7 +105,7 @@ __device__ __forceinline__ void strided_quads(const Tile &w, const Tile *w2, int
             lm23 = __builtin_amdgcn_readfirstlane(lm23);
         }
         const int l0 = (j << ashift) + a, ld = dj << ashift;
-        for (int t = 0; t < (w2 ? 2 : 1); ++t) {
+        {
             const Tile &x = t ? *w2 : w;
             typename F::E x0 = Lds<F>::load(x, l0, u), x1 = Lds<F>::load(x, l0 + ld, u),
                           x2 = Lds<F>::load(x, l0 + 2 * ld, u), x3 = Lds<F>::load(x, l0 + 3 * ld, u);
