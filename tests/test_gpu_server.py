@@ -82,17 +82,19 @@ CLASSIC = [(10, 4), (6, 3), (16, 4), (4, 2), (17, 3), (20, 12)]
 SIZES = [1000, 4096, 65536, 1 << 18, 1 << 20]
 
 
-def _make_requests(rng, n, leopard_every=0):
+def _make_requests(rng, n, leopard_every=0, leo_shape=(8, 192, 56, 65536)):
     """Returns [(encoder, op, shards_solo, shards_mix, note)] with identical
     independent inputs for the solo and the mixed run."""
     codecs = {s: rs.New(*s) for s in CLASSIC}
     oracles = {s: rsref.Classic(*s) for s in CLASSIC}
-    leo = rs.New(192, 56, rs.WithLeopardGF(True)) if leopard_every else None
-    leo_o = rsref.Leopard(192, 56, bits=8) if leopard_every else None
+    lbits, lk, lm, lsize = leo_shape
+    lopt = rs.WithLeopardGF16(True) if lbits == 16 else rs.WithLeopardGF(True)
+    leo = rs.New(lk, lm, lopt) if leopard_every else None
+    leo_o = rsref.Leopard(lk, lm, bits=lbits) if leopard_every else None
     out = []
     for n_ in range(n):
         if leopard_every and n_ % leopard_every == leopard_every - 1:
-            k, m, size, enc, o = 192, 56, 65536, leo, leo_o
+            k, m, size, enc, o = lk, lm, lsize, leo, leo_o
         else:
             k, m = CLASSIC[int(rng.integers(0, len(CLASSIC)))]
             size = int(rng.choice(SIZES))
@@ -148,9 +150,12 @@ def _check_same(req, vok, solo_ok):
         assert np.array_equal(mix[i].cpu().numpy(), ref[i]), (op, k, m, size, i)
 
 
-def test_served_mix_equals_solo_runs():
+# GF(2^8) (192, 56) 64 KiB as in the served stream; GF(2^16) (5000, 100)
+# reconstructs n = 8192 work rows (the plan's three-pass launches).
+@pytest.mark.parametrize("leo_shape", [(8, 192, 56, 65536), (16, 5000, 100, 128)])
+def test_served_mix_equals_solo_runs(leo_shape):
     rng = np.random.default_rng(5150)
-    reqs = _make_requests(rng, 96, leopard_every=12)
+    reqs = _make_requests(rng, 96, leopard_every=12, leo_shape=leo_shape)
     solo_ok = [_run_solo(enc, op, solo) for enc, op, solo, _, _ in reqs]
     torch.cuda.synchronize()
     srv = rs.Server(0, hbm_budget=256 << 20)
