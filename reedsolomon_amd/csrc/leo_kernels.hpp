@@ -164,6 +164,21 @@ bool leo_multipass_shape(int n, int *C, int *P, int *A);
 size_t leo_multipass_chunk(int n, size_t size);
 hipError_t leo_decode_multipass(const LeoDev &f, const LeoMultipassArgs &a, hipStream_t s);
 
+// ---- GF(2^16) encode in three LDS-tiled passes (m >= 16384 rows) ----
+struct LeoEncPassArgs {
+    const uint8_t *const *rows;  // pass 1: the m row pointers of group g (null past k)
+    uint8_t *const *parity;      // pass 3: the p parity shards
+    uint8_t *work, *work2;       // m * 64 bytes per slice each: group IFFT / accumulator
+    size_t col0;                 // first shard byte of this column chunk
+    int slices, m, p;
+    int mtrunc;                  // rows of group g (IFFT truncation)
+    int skew_off;                // group g's IFFT skew: fftSkew + m - 1 + g * m
+    int first, last;             // group g is the first / last one
+    int C, P, A;                 // leo_multipass_shape(m)
+};
+// pass 1 / 2 once per group in order, then pass 3.
+hipError_t leo_encode_multipass(const LeoDev &f, const LeoEncPassArgs &a, int pass, hipStream_t s);
+
 // Batched three-pass reconstructs: every task's whole column (col0 = 0) in
 // one launch per pass; block0[p] is the task's first workgroup in pass p
 // (tasks sorted by it), its workgroup count leo_pass_blocks.

@@ -81,7 +81,7 @@ __device__ __forceinline__ void store_tile(const Tile &w, int nrows, RowPtr rp) 
 // loads.
 template <class F, bool INV, bool UNI>
 __device__ __forceinline__ void strided_quads(const Tile &w, const Tile *w2, int ashift, int P, int dist,
-                                              int dj, int nq, const LeoDev &f) {
+                                              int dj, int nq, const uint16_t *sk, const LeoDev &f) {
     const int A = 1 << ashift;
     const int items = (A << log2i(nq)) << w.ushift;
     const int umask = (1 << w.ushift) - 1;
@@ -98,7 +98,7 @@ __device__ __forceinline__ void strided_quads(const Tile &w, const Tile *w2, int
         const int rj = (q / dj) * 4 * dj;
         const int j = rj + q % dj;
         const int iend = rj * P + dist;  // global r + dist
-        unsigned lm01 = f.skew[iend - 1], lm02 = f.skew[iend + dist - 1], lm23 = f.skew[iend + 2 * dist - 1];
+        unsigned lm01 = sk[iend], lm02 = sk[iend + dist], lm23 = sk[iend + 2 * dist];
         if (UNI) {
             lm01 = __builtin_amdgcn_readfirstlane(lm01);
             lm02 = __builtin_amdgcn_readfirstlane(lm02);
@@ -128,15 +128,18 @@ __device__ __forceinline__ void strided_quads(const Tile &w, const Tile *w2, int
     }
 }
 
+// sk: skew table with its bias applied (the butterfly group ending at global
+// row r + dist reads sk[r + dist]): fftSkew - 1 for the decoder's IFFT and
+// every FFT, fftSkew + m - 1 + g*m for encoder group g's IFFT.
 template <class F, bool INV>
 __device__ __forceinline__ void strided_step4(const Tile &w, const Tile *w2, int ashift, int J, int P, int dist,
-                                              int mtrunc, const LeoDev &f) {
+                                              int mtrunc, const uint16_t *sk, const LeoDev &f) {
     const int dj = dist / P;
     const int gcount = min(J / (4 * dj), (mtrunc + 4 * dist - 1) / (4 * dist));
     if (gcount <= 0) return;
     const int nq = gcount * dj;  // quads per lo
-    if (ashift + w.ushift >= 6) strided_quads<F, INV, true>(w, w2, ashift, P, dist, dj, nq, f);
-    else strided_quads<F, INV, false>(w, w2, ashift, P, dist, dj, nq, f);
+    if (ashift + w.ushift >= 6) strided_quads<F, INV, true>(w, w2, ashift, P, dist, dj, nq, sk, f);
+    else strided_quads<F, INV, false>(w, w2, ashift, P, dist, dj, nq, sk, f);
     __syncthreads();
 }
 
@@ -167,7 +170,8 @@ __device__ __forceinline__ void pass2(const LeoDev &f, const LeoMultipassArgs &a
     __syncthreads();
     // IFFT layers with distance >= C.
     int dist = a.C;
-    for (; 4 * dist <= a.n; dist *= 4) strided_step4<F, true>(in, nullptr, ashift, J, a.P, dist, a.ifft_trunc, f);
+    for (; 4 * dist <= a.n; dist *= 4)
+        strided_step4<F, true>(in, nullptr, ashift, J, a.P, dist, a.ifft_trunc, f.skew - 1, f);
     if (dist < a.n) {
         // final radix-2 layer: rows j, j + dj for j < dj, skew[dist - 1]
         const int dj = dist / a.P;
@@ -199,7 +203,8 @@ __device__ __forceinline__ void pass2(const LeoDev &f, const LeoMultipassArgs &a
         __syncthreads();
     }
     // FFT steps with distance >= P on both tiles.
-    for (int d = a.n >> 2; d >= a.P; d >>= 2) strided_step4<F, false>(in, &y, ashift, J, a.P, d, a.fft_trunc, f);
+    for (int d = a.n >> 2; d >= a.P; d >>= 2)
+        strided_step4<F, false>(in, &y, ashift, J, a.P, d, a.fft_trunc, f.skew - 1, f);
     store_tile<F>(in, R, [&](int l) { return wrow(a.work2, a.n, s, grow(l)); });  // v
     store_tile<F>(y, R, [&](int l) { return wrow(a.work, a.n, s, grow(l)); });    // u
 }
@@ -266,6 +271,109 @@ __device__ __forceinline__ void pass3(const LeoDev &f, const LeoMultipassArgs &a
         F::mul_add(v, Lds<F>::load(w, t, u), table<F>(f, a.out_logs[q]));
         F::gstore(a.out[q] + col, u, v);
     });
+}
+
+// ---- Encoder (leopard.go:169-260) for m taller than one LDS tile ----
+// parity = FFT_m(XOR over groups g of IFFT_m^g(data rows g*m ...)), the same
+// three-way split: pass 1 tiles C consecutive rows of group g (IFFT layers <
+// C) into work; pass 2 tiles rows lo + j*P (IFFT layers >= C), XORs into the
+// accumulator work2 and, after the last group, runs the FFT steps >= P on
+// it; pass 3 tiles P consecutive accumulator rows (FFT steps < P) and stores
+// rows < p to the parity shards.
+template <class F>
+__device__ __forceinline__ void enc_pass1(const LeoDev &f, const LeoEncPassArgs &a, int s, int by) {
+    extern __shared__ uint4 lds_raw[];
+    const int base = by * a.C;
+    const Tile w{reinterpret_cast<uint8_t *>(lds_raw), a.C + 1, 1, 2};
+    load_rows<F>(w, a.rows + base, nullptr, a.C, a.col0 + static_cast<size_t>(s) * kSlice, f);
+    for (int dshift = 0; (4 << dshift) <= a.C; dshift += 2)
+        step4<F, true>(w, a.C, base, dshift, a.mtrunc, f.skew + a.skew_off, 0, f);
+    store_tile<F>(w, a.C, [&](int l) { return wrow(a.work, a.m, s, base + l); });
+}
+
+template <class F>
+__device__ __forceinline__ void enc_pass2(const LeoDev &f, const LeoEncPassArgs &a, int s, int by) {
+    extern __shared__ uint4 lds_raw[];
+    const int lo0 = by * a.A;
+    const int J = a.m / a.P;
+    const int ashift = log2i(a.A);
+    const int R = a.A * J;
+    uint8_t *lds = reinterpret_cast<uint8_t *>(lds_raw);
+    const Tile in{lds, R + 1, 1, 2};
+    const Tile acc{lds + kSlice * (R + 1), R + 1, 1, 2};
+    auto grow = [&](int l) { return lo0 + (l & (a.A - 1)) + (l >> ashift) * a.P; };
+    load_tile<F>(in, R, [&](int l) { return wrow(a.work, a.m, s, grow(l)); });
+    if (!a.first) load_tile<F>(acc, R, [&](int l) { return wrow(a.work2, a.m, s, grow(l)); });
+    __syncthreads();
+    const uint16_t *isk = f.skew + a.skew_off;
+    int dist = a.C;
+    for (; 4 * dist <= a.m; dist *= 4) strided_step4<F, true>(in, nullptr, ashift, J, a.P, dist, a.mtrunc, isk, f);
+    if (dist < a.m) {
+        // final radix-2 layer: rows j, j + dj for j < dj, skew isk[dist]
+        const int dj = dist / a.P;
+        const unsigned lm = isk[dist];
+        const int items = (R / 2) << in.ushift;
+        for (int it = threadIdx.x; it < items; it += blockDim.x) {
+            const int u = it & 1, p = it >> 1;
+            const int l0 = p, l1 = p + (dj << ashift);
+            typename F::E x0 = Lds<F>::load(in, l0, u), x1 = Lds<F>::load(in, l1, u);
+            ifft2<F>(x0, x1, lm, f);
+            Lds<F>::store(in, l0, u, x0);
+            Lds<F>::store(in, l1, u, x1);
+        }
+        __syncthreads();
+    }
+    if (!a.first) {
+        for (int it = threadIdx.x; it < R * (kSlice / 16); it += blockDim.x) {
+            const int l = it % R, sl = it / R;
+            *slot(in, l, sl) = xor4(*slot(in, l, sl), *slot(acc, l, sl));
+        }
+        __syncthreads();
+    }
+    if (a.last)
+        for (int d = a.m >> 2; d >= a.P; d >>= 2)
+            strided_step4<F, false>(in, nullptr, ashift, J, a.P, d, a.p, f.skew - 1, f);
+    store_tile<F>(in, R, [&](int l) { return wrow(a.work2, a.m, s, grow(l)); });
+}
+
+template <class F>
+__device__ __forceinline__ void enc_pass3(const LeoDev &f, const LeoEncPassArgs &a, int s, int by) {
+    extern __shared__ uint4 lds_raw[];
+    const int base = by * a.P;
+    if (base >= a.p) return;  // no parity row in this tile (uniform per workgroup)
+    const Tile w{reinterpret_cast<uint8_t *>(lds_raw), a.P + 1, 1, 2};
+    load_tile<F>(w, a.P, [&](int l) { return wrow(a.work2, a.m, s, base + l); });
+    __syncthreads();
+    int dist4 = a.m, dist = a.m >> 2;
+    for (; dist != 0; dist4 = dist, dist >>= 2)
+        if (dist < a.P) step4<F, false>(w, a.P, base, log2i(dist), a.p, f.skew, -1, f);
+    if (dist4 == 2) {
+        const int pairs = min(a.P / 2, (a.p - base + 1) / 2);
+        for_rows(w, pairs, [&](int q, int u) {
+            const int r = 2 * q;
+            typename F::E x = Lds<F>::load(w, r, u), yv = Lds<F>::load(w, r + 1, u);
+            fft2<F>(x, yv, f.skew[base + r], f);
+            Lds<F>::store(w, r, u, x);
+            Lds<F>::store(w, r + 1, u, yv);
+        });
+        __syncthreads();
+    }
+    const int rows = min(a.P, a.p - base);
+    const size_t col = a.col0 + static_cast<size_t>(s) * kSlice;
+    for_units(w, rows, [&](int t, int u) { F::gstore(a.parity[base + t] + col, u, Lds<F>::load(w, t, u)); });
+}
+
+template <class F>
+__global__ __launch_bounds__(kMB1) void k_enc_pass1(const LeoDev f, const LeoEncPassArgs a) {
+    enc_pass1<F>(f, a, blockIdx.x, blockIdx.y);
+}
+template <class F>
+__global__ __launch_bounds__(kMB2) void k_enc_pass2(const LeoDev f, const LeoEncPassArgs a) {
+    enc_pass2<F>(f, a, blockIdx.x, blockIdx.y);
+}
+template <class F>
+__global__ __launch_bounds__(kMB) void k_enc_pass3(const LeoDev f, const LeoEncPassArgs a) {
+    enc_pass3<F>(f, a, blockIdx.x, blockIdx.y);
 }
 
 template <class F>
@@ -374,6 +482,31 @@ hipError_t leo_decode_multipass(const LeoDev &f, const LeoMultipassArgs &a, hipS
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (small) hipLaunchKernelGGL((k_dec_pass3<F16, kDerivMax / 2>), dim3(slices, a.n / a.P), dim3(kMB), lds3, s, f, a);
     else hipLaunchKernelGGL((k_dec_pass3<F16, kDerivMax>), dim3(slices, a.n / a.P), dim3(kMB), lds3, s, f, a);
+    return hipGetLastError();
+}
+
+hipError_t leo_encode_multipass(const LeoDev &f, const LeoEncPassArgs &a, int pass, hipStream_t s) {
+    if (f.bits != 16 || a.slices <= 0 || pass < 1 || pass > 3) return hipErrorInvalidValue;
+    const size_t lds1 = static_cast<size_t>(kSlice) * (a.C + 1);
+    const size_t lds2 = 2 * static_cast<size_t>(kSlice) * (a.A * (a.m / a.P) + 1);
+    const size_t lds3 = static_cast<size_t>(kSlice) * (a.P + 1);
+    auto allow = [](const void *k, size_t lds) {
+        return lds <= (64u << 10) ? hipSuccess
+                                   : hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                         static_cast<int>(lds));
+    };
+    hipError_t e;
+    const unsigned slices = static_cast<unsigned>(a.slices);
+    if (pass == 1) {
+        if ((e = allow(reinterpret_cast<const void *>(k_enc_pass1<F16>), lds1)) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_enc_pass1<F16>, dim3(slices, a.m / a.C), dim3(kMB1), lds1, s, f, a);
+    } else if (pass == 2) {
+        if ((e = allow(reinterpret_cast<const void *>(k_enc_pass2<F16>), lds2)) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_enc_pass2<F16>, dim3(slices, a.P / a.A), dim3(kMB2), lds2, s, f, a);
+    } else {
+        if ((e = allow(reinterpret_cast<const void *>(k_enc_pass3<F16>), lds3)) != hipSuccess) return e;
+        hipLaunchKernelGGL(k_enc_pass3<F16>, dim3(slices, a.m / a.P), dim3(kMB), lds3, s, f, a);
+    }
     return hipGetLastError();
 }
 

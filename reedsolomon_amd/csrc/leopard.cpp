@@ -161,6 +161,7 @@ constexpr size_t kEncodeBudget = 64u << 20;
 constexpr size_t kDecodeBudget = 64u << 20;
 constexpr bool kQuarterTiles = true;  // GF(2^16) reconstructs with n = 4096: one quarter-chunk tile
 constexpr size_t kQuarterEncodeMin = 4096;  // shard bytes: 256 quarter slices, one per CU
+constexpr int kPassEncodeMin = 16384;  // m from which encodes run the three tiled passes
 
 // LDS budget of a one-tile encode: the m rows of a slice, and with several
 // groups their XOR accumulator in registers (at most the derivative's
@@ -294,6 +295,38 @@ int encode_device(rsx_codec *c, const std::vector<uint8_t *> &data, const std::v
     if (slice) {
         const LeoEncodeArgs a{dptr, pptr, size, k, p, m, groups, slice, flag, flag_value};
         RSX_HIP(leo_encode_fused(*s.dev, a, c->stream));
+        return RSX_OK;
+    }
+    // m >= 16384: three tiled passes (per group: IFFT layers < C, then >= C
+    // and the XOR into the accumulator; after the last group the FFT).
+    int eC = 0, eP = 0, eA = 0;
+    if (s.fused && s.bits == 16 && m >= kPassEncodeMin && leo_multipass_shape(m, &eC, &eP, &eA)) {
+        const size_t cw = leo_multipass_chunk(m, size);
+        if ((err = s.work.reserve(cw * m))) return err;
+        if ((err = s.work2.reserve(cw * m))) return err;
+        for (size_t off = 0; off < size; off += cw) {
+            LeoEncPassArgs a{};
+            a.parity = pptr;
+            a.work = s.work.as<uint8_t>();
+            a.work2 = s.work2.as<uint8_t>();
+            a.col0 = off;
+            a.slices = static_cast<int>(std::min(cw, size - off) / 64);
+            a.m = m;
+            a.p = p;
+            a.C = eC;
+            a.P = eP;
+            a.A = eA;
+            for (int g = 0; g < groups; ++g) {
+                a.rows = dptr + static_cast<size_t>(g) * m;
+                a.mtrunc = std::min(m, k - g * m);
+                a.skew_off = m - 1 + g * m;
+                a.first = g == 0;
+                a.last = g == groups - 1;
+                RSX_HIP(leo_encode_multipass(*s.dev, a, 1, c->stream));
+                RSX_HIP(leo_encode_multipass(*s.dev, a, 2, c->stream));
+            }
+            RSX_HIP(leo_encode_multipass(*s.dev, a, 3, c->stream));
+        }
         return RSX_OK;
     }
     const size_t chunk = chunk_for(size, 2 * m, kEncodeBudget);

@@ -173,6 +173,35 @@ def test_leopard16_three_pass_column_chunks():
         assert torch.equal(work[i], orig[i]), i
 
 
+def test_leopard16_three_pass_encode_column_chunks():
+    # m = ceilPow2(10000) = 16384: the encoder's three passes run per column
+    # chunk of 32 KiB (512 MiB work arrays), two groups each; 32 KiB + 128
+    # bytes is two chunks, the second 128 bytes wide.
+    k, m, size = 20000, 10000, (32 << 10) + 128
+    g = torch.Generator(device="cuda")
+    g.manual_seed(16384)
+    buf = torch.empty((k + m, size), dtype=torch.uint8, device="cuda")
+    buf[:k].random_(0, 256, generator=g)
+    buf[k:].zero_()
+    shards = [buf[i] for i in range(k + m)]
+    enc = rs.New(k, m, rs.WithLeopardGF16(True))
+    enc.Encode(shards)
+    for sl in (slice(0, 128), slice(size - 128, size)):
+        cols = buf[:, sl].cpu().numpy()
+        _, want = oracle_encode(16, k, m, [cols[i].copy() for i in range(k)])
+        for i in range(m):
+            assert np.array_equal(cols[k + i], want[k + i]), (sl, i)
+    assert enc.Verify(shards)
+    orig = buf.clone()
+    rng = np.random.default_rng(16384)
+    lost = sorted(rng.choice(k + m, m, replace=False).tolist())
+    gone = set(lost)
+    work = [None if i in gone else shards[i] for i in range(k + m)]
+    enc.Reconstruct(work)
+    for i in lost:
+        assert torch.equal(work[i], orig[i]), i
+
+
 def test_leopard8_192_56_64kib():
     # The Leopard GF(2^8) slice of the served stream at (192, 56) 64 KiB
     # ((200, 56) indexes past the reference's tables).
@@ -264,8 +293,9 @@ def test_sparse_erasures_prune_fft_exactly(bits, k, m, size):
 # tile where the rows fit, a quarter-chunk tile for GF(2^16) reconstructs
 # with n = 4096, three tiled passes for n >= 8192; GF(2^16) encodes with
 # m = ceilPow2(parity) of 4096 and several groups, or 8192, in a quarter-chunk
-# tile from 4 KiB shards, one launch per layer below) and one launch per
-# butterfly layer everywhere (RSX_LEOPARD_LAYERED=1).
+# tile from 4 KiB shards, one launch per layer below; m >= 16384 in three
+# tiled passes) and one launch per butterfly layer everywhere
+# (RSX_LEOPARD_LAYERED=1).
 SCHEDULE_CASES = [(8, 192, 56, 64 * 37), (8, 17, 3, 64 * 129), (8, 128, 128, 64 * 5), (8, 3, 2, 64 * 1000),
                   (16, 100, 20, 64 * 41), (16, 300, 300, 64 * 3), (16, 33, 7, 64 * 300), (16, 5, 3, 64 * 777),
                   (16, 1000, 100, 64 * 600), (16, 1500, 600, 64 * 5), (16, 2000, 50, 64 * 2),
@@ -273,7 +303,7 @@ SCHEDULE_CASES = [(8, 192, 56, 64 * 37), (8, 17, 3, 64 * 129), (8, 128, 128, 64 
                   (16, 2100, 1100, 64 * 2), (16, 1100, 1100, 64 * 3), (16, 1000, 3000, 64 * 3),
                   (16, 1024, 1024, 64 * 21), (16, 5000, 3000, 64 * 3), (16, 3000, 5000, 64 * 2),
                   (16, 9000, 5000, 64), (16, 5000, 3000, 4096 + 64 * 3), (16, 3000, 5000, 4096 + 64 * 2),
-                  (16, 9000, 5000, 4096)]
+                  (16, 9000, 5000, 4096), (16, 20000, 10000, 64), (16, 8000, 16000, 128)]
 
 
 @pytest.mark.parametrize("bits,k,m,size", SCHEDULE_CASES)
