@@ -161,7 +161,7 @@ constexpr size_t kEncodeBudget = 64u << 20;
 constexpr size_t kDecodeBudget = 64u << 20;
 constexpr bool kQuarterTiles = true;  // GF(2^16) reconstructs with n = 4096: one quarter-chunk tile
 constexpr size_t kQuarterEncodeMin = 4096;  // shard bytes: 256 quarter slices, one per CU
-constexpr int kPassEncodeMin = 16384;  // m from which encodes run the three tiled passes
+constexpr int kPassEncodeMin = 2048;  // m from which encodes without a tile run the three passes
 
 // LDS budget of a one-tile encode: the m rows of a slice, and with several
 // groups their XOR accumulator in registers (at most the derivative's
@@ -254,11 +254,16 @@ int encode_tile_slice(const rsx_codec *c, size_t size) {
     const int m = ceil_pow2(c->m);
     const int groups = (c->k + m - 1) / m;
     if (!s.fused) return 0;
+    // GF(2^16) m >= 2048 below 4 KiB shards: too few tiles to fill the device;
+    // the three passes run 1.2-1.5x faster there (4096+4096 x 1 KiB 0.086 ->
+    // 0.056 ms), the tiles win from 16 KiB.
+    if (s.bits == 16 && m >= 2048 && size < kQuarterEncodeMin) return 0;
     const int slice = leo_fused_slice(s.bits, m, 1, m, size, encode_lds_budget(m, groups));
-    // GF(2^16) m = 4096 with several groups, or 8192: quarter-chunk slices,
-    // once the shard gives every CU one (4 KiB; 1.4-1.8x the layered schedule
-    // from there, 15% slower for 8192+8192 x 1 KiB: 64 workgroups).
-    if (!slice && s.bits == 16 && kQuarterTiles && size >= kQuarterEncodeMin) return leo_quarter_encode_slice(m, groups);
+    // GF(2^16) m = 4096 with several groups: quarter-chunk slices once the
+    // shard gives every CU one (4 KiB; about even with the three passes, which
+    // take m = 8192 at 1.3-1.5x the quarter tile and everything below 4 KiB).
+    if (!slice && s.bits == 16 && kQuarterTiles && m <= 4096 && size >= kQuarterEncodeMin)
+        return leo_quarter_encode_slice(m, groups);
     return slice;
 }
 
@@ -297,8 +302,9 @@ int encode_device(rsx_codec *c, const std::vector<uint8_t *> &data, const std::v
         RSX_HIP(leo_encode_fused(*s.dev, a, c->stream));
         return RSX_OK;
     }
-    // m >= 16384: three tiled passes (per group: IFFT layers < C, then >= C
-    // and the XOR into the accumulator; after the last group the FFT).
+    // No tile: three tiled passes (per group: IFFT layers < C, then >= C and
+    // the XOR into the accumulator; after the last group the FFT); 1.1-2.7x
+    // the layered schedule.
     int eC = 0, eP = 0, eA = 0;
     if (s.fused && s.bits == 16 && m >= kPassEncodeMin && leo_multipass_shape(m, &eC, &eP, &eA)) {
         const size_t cw = leo_multipass_chunk(m, size);

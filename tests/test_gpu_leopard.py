@@ -292,10 +292,9 @@ def test_sparse_erasures_prune_fft_exactly(bits, k, m, size):
 # run on both schedules: rows of a column slice in LDS (the default: one
 # tile where the rows fit, a quarter-chunk tile for GF(2^16) reconstructs
 # with n = 4096, three tiled passes for n >= 8192; GF(2^16) encodes with
-# m = ceilPow2(parity) of 4096 and several groups, or 8192, in a quarter-chunk
-# tile from 4 KiB shards, one launch per layer below; m >= 16384 in three
-# tiled passes) and one launch per butterfly layer everywhere
-# (RSX_LEOPARD_LAYERED=1).
+# m = ceilPow2(parity) of 4096 and several groups in a quarter-chunk tile from
+# 4 KiB shards, m >= 8192 and every m >= 2048 below 4 KiB in three tiled
+# passes) and one launch per butterfly layer everywhere (RSX_LEOPARD_LAYERED=1).
 SCHEDULE_CASES = [(8, 192, 56, 64 * 37), (8, 17, 3, 64 * 129), (8, 128, 128, 64 * 5), (8, 3, 2, 64 * 1000),
                   (16, 100, 20, 64 * 41), (16, 300, 300, 64 * 3), (16, 33, 7, 64 * 300), (16, 5, 3, 64 * 777),
                   (16, 1000, 100, 64 * 600), (16, 1500, 600, 64 * 5), (16, 2000, 50, 64 * 2),
