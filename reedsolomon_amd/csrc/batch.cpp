@@ -97,6 +97,19 @@ struct rsx_batch {
     size_t mp_work_bytes = 0;
     uint32_t mp_blocks[3] = {0, 0, 0};
     int mp_max_p = 0;
+
+    // GF(2^16) encodes above the tiles: per group index g, a pass-1 and a
+    // pass-2 launch over the items with more than g groups, then one pass-3
+    // launch over all; their work arrays also in d_mp_work.
+    struct EncLaunch {
+        size_t off;  // first task in d_emp_tasks
+        int ntasks, pass, max_p;
+        uint32_t blocks;
+    };
+    std::vector<rsx::LeoItemPlan> leo_emp;
+    std::vector<rsx::LeoEncPassTask> emp_tasks;
+    std::vector<EncLaunch> emp_launches;
+    DeviceBuffer d_emp_tasks;
 };
 
 namespace {
@@ -190,7 +203,7 @@ int enqueue(rsx_batch *b, hipStream_t s) {
     if (verify)
         RSX_HIP(launch_fill_u32(b->d_flags.as<uint32_t>(), static_cast<size_t>(b->nitems), 0u, s));
     const bool leo = !b->leo_tasks[0].empty() || !b->leo_tasks[1].empty() || !b->leo_tasks[2].empty() ||
-                     !b->mp_tasks.empty();
+                     !b->mp_tasks.empty() || !b->emp_launches.empty();
     const bool fork = leo && !b->launches.empty();
     const int spread = rsx_batch::kSpreadClasses
                            ? std::min<int>(static_cast<int>(b->launches.size()) - 1, rsx_batch::kClassStreams)
@@ -212,6 +225,9 @@ int enqueue(rsx_batch *b, hipStream_t s) {
     if (!b->mp_tasks.empty())
         RSX_HIP(leo_decode_multipass_batch(*b->leo_mp[0].dev, b->d_mp_tasks.as<LeoPassTask>(),
                                            static_cast<int>(b->mp_tasks.size()), b->mp_blocks, b->mp_max_p, ls));
+    for (const auto &L : b->emp_launches)
+        RSX_HIP(leo_encode_multipass_batch(*b->leo_emp[0].dev, b->d_emp_tasks.as<LeoEncPassTask>() + L.off, L.ntasks,
+                                           L.blocks, L.pass, L.max_p, ls));
     if (fork) RSX_HIP(hipEventRecord(b->join, b->side));
     int next_side = 0;
     for (size_t i = 0; i < b->launches.size(); ++i) {
@@ -379,6 +395,9 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
     }
     b->leo_mp.clear();
     b->mp_tasks.clear();
+    b->leo_emp.clear();
+    b->emp_tasks.clear();
+    b->emp_launches.clear();
     b->mp_work_bytes = 0;
     b->mp_max_p = 0;
     for (uint32_t &x : b->mp_blocks) x = 0;
@@ -429,7 +448,7 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
                 b->is_verify[it] = 1;
             }
             if (lp.empty) continue;
-            if (lp.kind == 3) {
+            if (lp.kind == 3 || lp.kind == 4) {
                 // Work arrays of the three-pass items, within a cap per plan.
                 const size_t wb = (lp.work_bytes + 255) / 256 * 256;
                 if (b->mp_work_bytes + wb > rsx_batch::kPassWorkCap) {
@@ -438,9 +457,10 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
                     status[it] = fail(RSX_ERR_NOT_SUPPORTED, "three-pass work arrays over the plan's cap");
                     continue;
                 }
-                lp.mp.work = reinterpret_cast<uint8_t *>(b->mp_work_bytes);  // offset, patched below
+                if (lp.kind == 3) lp.mp.work = reinterpret_cast<uint8_t *>(b->mp_work_bytes);  // offset, patched below
+                else lp.emp.work = reinterpret_cast<uint8_t *>(b->mp_work_bytes);
                 b->mp_work_bytes += wb;
-                b->leo_mp.push_back(lp);
+                (lp.kind == 3 ? b->leo_mp : b->leo_emp).push_back(lp);
             } else {
                 b->leo[lp.kind].push_back(lp);
             }
@@ -632,8 +652,49 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
         RSX_HIP(hipMemcpyAsync(b->d_leo_tasks[fi].ptr, b->leo_tasks[fi].data(), bytes, hipMemcpyHostToDevice,
                                b->stream));
     }
-    if (!b->leo_mp.empty()) {
+    if (!b->leo_mp.empty() || !b->leo_emp.empty())
         if ((err = b->d_mp_work.reserve(b->mp_work_bytes))) return err;
+    if (!b->leo_emp.empty()) {
+        int max_groups = 0, max_p = 0;
+        for (const LeoItemPlan &lp : b->leo_emp) {
+            max_groups = std::max(max_groups, lp.groups);
+            max_p = std::max(max_p, lp.emp.P);
+        }
+        // Task arrays in launch order: (pass 1, pass 2) for g = 0, 1, ..., then pass 3.
+        auto add_launch = [&](int pass, int g) -> int {
+            rsx_batch::EncLaunch L{b->emp_tasks.size(), 0, pass, max_p, 0};
+            for (const LeoItemPlan &lp : b->leo_emp) {
+                if (pass != 3 && g >= lp.groups) continue;
+                LeoEncPassTask t{};
+                t.a = lp.emp;
+                uint8_t *const *pp = b->d_ptrs.as<uint8_t *>();
+                t.a.rows = pp + lp.ptr_off + static_cast<size_t>(g) * t.a.m;
+                t.a.parity = pp + lp.parity_pool;
+                t.a.work = b->d_mp_work.as<uint8_t>() + reinterpret_cast<size_t>(lp.emp.work);
+                t.a.work2 = t.a.work + static_cast<size_t>(t.a.m) * static_cast<size_t>(t.a.slices) * 64;
+                t.a.col0 = 0;
+                t.a.mtrunc = std::min(t.a.m, lp.k - g * t.a.m);
+                t.a.skew_off = t.a.m - 1 + g * t.a.m;
+                t.a.first = g == 0;
+                t.a.last = g == lp.groups - 1;
+                t.block0 = L.blocks;
+                const uint64_t nb = leo_enc_pass_blocks(t.a, pass);
+                if (static_cast<uint64_t>(L.blocks) + nb > 0x7FFFFFFFull) return -1;
+                L.blocks += static_cast<uint32_t>(nb);
+                b->emp_tasks.push_back(t);
+                ++L.ntasks;
+            }
+            if (L.ntasks) b->emp_launches.push_back(L);
+            return 0;
+        };
+        for (int g = 0; g < max_groups; ++g)
+            if (add_launch(1, g) || add_launch(2, g)) return fail(RSX_ERR_INVALID_INPUT, "batch too large for one plan");
+        if (add_launch(3, 0)) return fail(RSX_ERR_INVALID_INPUT, "batch too large for one plan");
+        const size_t bytes = b->emp_tasks.size() * sizeof(LeoEncPassTask);
+        if ((err = b->d_emp_tasks.reserve(bytes))) return err;
+        RSX_HIP(hipMemcpyAsync(b->d_emp_tasks.ptr, b->emp_tasks.data(), bytes, hipMemcpyHostToDevice, b->stream));
+    }
+    if (!b->leo_mp.empty()) {
         uint8_t *aux = b->d_leo_aux.as<uint8_t>();
         for (const LeoItemPlan &lp : b->leo_mp) {
             LeoPassTask t{};
@@ -708,7 +769,7 @@ int rsx_batch_num_tasks(const rsx_batch *b) {
     size_t n = 0;
     for (auto &v : b->tasks) n += v.size();
     for (auto &v : b->leo_tasks) n += v.size();
-    n += b->mp_tasks.size();
+    n += b->mp_tasks.size() + b->leo_emp.size();
     return static_cast<int>(n);
 }
 

@@ -178,6 +178,16 @@ struct LeoEncPassArgs {
 };
 // pass 1 / 2 once per group in order, then pass 3.
 hipError_t leo_encode_multipass(const LeoDev &f, const LeoEncPassArgs &a, int pass, hipStream_t s);
+// Batched: one launch per pass and group index over the tasks of that launch
+// (sorted by block0; leo_enc_pass_blocks workgroups each); max_p: the largest
+// P of the pass-3 tasks.
+struct LeoEncPassTask {
+    LeoEncPassArgs a;
+    uint32_t block0;
+};
+uint32_t leo_enc_pass_blocks(const LeoEncPassArgs &a, int pass);
+hipError_t leo_encode_multipass_batch(const LeoDev &f, const LeoEncPassTask *tasks, int ntasks, uint32_t blocks,
+                                      int pass, int max_p, hipStream_t s);
 
 // Batched three-pass reconstructs: every task's whole column (col0 = 0) in
 // one launch per pass; block0[p] is the task's first workgroup in pass p

@@ -424,6 +424,39 @@ __global__ __launch_bounds__(kMB) void k_dec_pass3_batch(const LeoDev f, const L
     pass3<F, MAXD>(f, a, static_cast<int>(l % a.slices), static_cast<int>(l / a.slices));
 }
 
+// Batched encoder passes: one 1-D grid over the workgroups of every task
+// of the launch (tasks sorted by block0).
+__device__ __forceinline__ int enc_task(const LeoEncPassTask *tasks, int ntasks, uint32_t g) {
+    int lo = 0, hi = ntasks - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (tasks[mid].block0 <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+template <class F>
+__global__ __launch_bounds__(kMB1) void k_enc_pass1_batch(const LeoDev f, const LeoEncPassTask *tasks, int ntasks) {
+    const int t = enc_task(tasks, ntasks, blockIdx.x);
+    const LeoEncPassArgs a = tasks[t].a;
+    const uint32_t l = blockIdx.x - tasks[t].block0;
+    enc_pass1<F>(f, a, static_cast<int>(l % a.slices), static_cast<int>(l / a.slices));
+}
+template <class F>
+__global__ __launch_bounds__(kMB2) void k_enc_pass2_batch(const LeoDev f, const LeoEncPassTask *tasks, int ntasks) {
+    const int t = enc_task(tasks, ntasks, blockIdx.x);
+    const LeoEncPassArgs a = tasks[t].a;
+    const uint32_t l = blockIdx.x - tasks[t].block0;
+    enc_pass2<F>(f, a, static_cast<int>(l % a.slices), static_cast<int>(l / a.slices));
+}
+template <class F>
+__global__ __launch_bounds__(kMB) void k_enc_pass3_batch(const LeoDev f, const LeoEncPassTask *tasks, int ntasks) {
+    const int t = enc_task(tasks, ntasks, blockIdx.x);
+    const LeoEncPassArgs a = tasks[t].a;
+    const uint32_t l = blockIdx.x - tasks[t].block0;
+    enc_pass3<F>(f, a, static_cast<int>(l % a.slices), static_cast<int>(l / a.slices));
+}
+
 }  // namespace
 
 void leo_pass_blocks(const LeoMultipassArgs &a, uint32_t blocks[3]) {
@@ -431,6 +464,30 @@ void leo_pass_blocks(const LeoMultipassArgs &a, uint32_t blocks[3]) {
     blocks[0] = sl * static_cast<uint32_t>(a.n / a.C);
     blocks[1] = sl * static_cast<uint32_t>(a.P / a.A);
     blocks[2] = sl * static_cast<uint32_t>(a.n / a.P);
+}
+
+uint32_t leo_enc_pass_blocks(const LeoEncPassArgs &a, int pass) {
+    const uint32_t sl = static_cast<uint32_t>(a.slices);
+    return sl * static_cast<uint32_t>(pass == 1 ? a.m / a.C : pass == 2 ? a.P / a.A : a.m / a.P);
+}
+
+hipError_t leo_encode_multipass_batch(const LeoDev &f, const LeoEncPassTask *tasks, int ntasks, uint32_t blocks,
+                                      int pass, int max_p, hipStream_t s) {
+    if (f.bits != 16 || ntasks <= 0 || blocks == 0 || pass < 1 || pass > 3) return hipErrorInvalidValue;
+    const size_t lds = pass == 1 ? static_cast<size_t>(kSlice) * (1024 + 1)
+                       : pass == 2 ? 2 * static_cast<size_t>(kSlice) * (512 + 1)
+                                   : static_cast<size_t>(kSlice) * (max_p + 1);
+    const void *k = pass == 1 ? reinterpret_cast<const void *>(k_enc_pass1_batch<F16>)
+                    : pass == 2 ? reinterpret_cast<const void *>(k_enc_pass2_batch<F16>)
+                                : reinterpret_cast<const void *>(k_enc_pass3_batch<F16>);
+    if (lds > (64u << 10)) {
+        const hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+        if (e != hipSuccess) return e;
+    }
+    if (pass == 1) hipLaunchKernelGGL(k_enc_pass1_batch<F16>, dim3(blocks), dim3(kMB1), lds, s, f, tasks, ntasks);
+    else if (pass == 2) hipLaunchKernelGGL(k_enc_pass2_batch<F16>, dim3(blocks), dim3(kMB2), lds, s, f, tasks, ntasks);
+    else hipLaunchKernelGGL(k_enc_pass3_batch<F16>, dim3(blocks), dim3(kMB), lds, s, f, tasks, ntasks);
+    return hipGetLastError();
 }
 
 bool leo_multipass_shape(int n, int *C, int *P, int *A) {

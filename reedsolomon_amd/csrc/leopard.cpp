@@ -729,7 +729,9 @@ int leopard_plan_item(rsx_codec *c, int op, uint8_t *const *shards, const size_t
         if (!slice && s.bits == 16 && kQuarterTiles && (groups == 1 || m <= 4096) &&
             (slice = leo_quarter_encode_slice(m, groups)) != 0)
             out.kind = 2;
-        if (!slice) return fail(RSX_ERR_NOT_SUPPORTED, "Leopard shape too tall for the batched kernel");
+        int eC = 0, eP = 0, eA = 0;
+        const bool passes = !slice && s.bits == 16 && s.fused && leo_multipass_shape(m, &eC, &eP, &eA);
+        if (!slice && !passes) return fail(RSX_ERR_NOT_SUPPORTED, "Leopard shape too tall for the batched kernel");
         out.ptr_off = pool.size();
         for (int g = 0; g < groups; ++g)
             for (int j = 0; j < m; ++j) {
@@ -738,6 +740,22 @@ int leopard_plan_item(rsx_codec *c, int op, uint8_t *const *shards, const size_t
             }
         out.parity_pool = pool.size();
         for (int i = 0; i < p; ++i) pool.push_back(shards[k + i]);
+        if (passes) {
+            // Verify compares inside the one-tile kernels only.
+            if (op == RSX_OP_VERIFY) return fail(RSX_ERR_NOT_SUPPORTED, "batched Verify of a three-pass encode");
+            out.kind = 4;
+            out.k = k;
+            out.groups = groups;
+            out.emp.slices = static_cast<int>(size / 64);
+            out.emp.m = m;
+            out.emp.p = p;
+            out.emp.C = eC;
+            out.emp.P = eP;
+            out.emp.A = eA;
+            out.work_bytes = 2 * static_cast<size_t>(m) * size;
+            out.slices = static_cast<uint32_t>(size / 64);
+            return RSX_OK;
+        }
         out.task.op = 0;
         out.task.enc = LeoEncodeArgs{nullptr, nullptr, size, k, p, m, groups, slice};
         out.lds = static_cast<size_t>(slice) * (m + 1);

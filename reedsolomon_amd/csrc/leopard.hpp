@@ -33,9 +33,12 @@ struct LeoItemPlan {
     // batch launch: 0 GF(2^8), 1 GF(2^16), 2 GF(2^16) quarter-chunk tiles,
     // 3 GF(2^16) three-pass reconstruct (n >= 8192 work rows)
     int kind = 0;
+    // 4 GF(2^16) three-pass encode (m above the tiles)
     LeoMultipassArgs mp{};    // kind 3: pointers are pool / aux offsets until the plan patches them
+    LeoEncPassArgs emp{};     // kind 4: per-group fields and pointers are set by the plan
+    int k = 0, groups = 0;    // kind 4: data shards, m-row groups
     size_t outidx_off = 0;    // kind 3: aux offset of the n-entry output index
-    size_t work_bytes = 0;    // kind 3: bytes of its two work arrays
+    size_t work_bytes = 0;    // kinds 3 and 4: bytes of its two work arrays
     const LeoDev *dev = nullptr;
     LeoBatchTask task{};
     size_t lds = 0;
@@ -51,10 +54,9 @@ struct LeoItemPlan {
 // RSX_OP_ENCODE / _VERIFY (planned as an encode) / _RECONSTRUCT /
 // _RECONSTRUCT_DATA) and plans it for the LDS-resident batch kernel.
 // Returns the item's status; RSX_ERR_NOT_SUPPORTED where the batched form
-// does not apply (shards not 16-byte aligned, or GF(2^16) encodes with
-// m = ceilPow2(parity) above 8192, or above 4096 with several groups).
-// GF(2^16) reconstructs with 8192 or more work rows are planned as three-pass
-// items (kind 3).
+// does not apply (shards not 16-byte aligned).  GF(2^16) reconstructs with
+// 8192 or more work rows and encodes above the tiles are planned as three-pass
+// items (kinds 3 and 4).
 int leopard_plan_item(rsx_codec *c, int op, uint8_t *const *shards, const size_t *lens, int nshards,
                       std::vector<const uint8_t *> &pool, std::vector<uint8_t> &aux, LeoItemPlan &out,
                       uint8_t *filled);

@@ -284,6 +284,36 @@ def test_batched_three_pass_reconstructs_rebuild_the_codeword():
                     assert work[i] is None, (use_graph, idx, i)
 
 
+def test_batched_three_pass_encodes_match_oracle():
+    # GF(2^16) encodes above the tiles (m = 16384 with two groups and with
+    # one, m = 8192 with two) run the encoder's passes per group index, in a
+    # plan beside a one-tile encode and a three-pass reconstruct; Verify of
+    # such a shape is not taken by the batch.
+    rng = np.random.default_rng(16)
+    stripes = []
+    for k, m, size in ((20000, 10000, 64), (8000, 16000, 128), (10000, 5000, 64), (100, 20, 64 * 3)):
+        stripes.append((rs.New(k, m, rs.WithLeopardGF16(True)), k, m, _leo_codeword(rng, 16, k, m, size)))
+    b = rs.Batch(0)
+    for use_graph in (True, False):
+        b.set_graph(use_graph)
+        items = []
+        for enc, k, m, full in stripes:
+            items.append((enc, rs.OP_ENCODE, [torch.from_numpy(x).to("cuda") for x in full[:k]] +
+                          [torch.zeros(len(full[0]), dtype=torch.uint8, device="cuda") for _ in range(m)]))
+        enc, k, m, full = stripes[0]
+        lost = set(rng.choice(k + m, m, replace=False).tolist())
+        rec = [None if i in lost else torch.from_numpy(full[i]).to("cuda") for i in range(k + m)]
+        items.append((enc, rs.OP_RECONSTRUCT, rec))
+        items.append((enc, rs.OP_VERIFY, [torch.from_numpy(x).to("cuda") for x in full]))
+        st = b.run(items)
+        assert st == [0] * (len(items) - 1) + [rs.ErrNotSupported.code], st
+        for (enc_, k_, m_, full_), (_, _, sh) in zip(stripes, items):
+            for i in range(k_, k_ + m_):
+                assert np.array_equal(sh[i].cpu().numpy(), full_[i]), (use_graph, k_, m_, i)
+        for i in lost:
+            assert np.array_equal(rec[i].cpu().numpy(), full[i]), (use_graph, i)
+
+
 def test_batched_leopard_unsupported_items_are_reported():
     rng = np.random.default_rng(9)
     # GF(2^16) reconstruct on shards that are not 16-byte aligned.
