@@ -724,9 +724,10 @@ int leopard_plan_item(rsx_codec *c, int op, uint8_t *const *shards, const size_t
         const int groups = (k + m - 1) / m;
         // Other items fill the device: size the slice for full workgroups.
         int slice = leo_fused_slice(s.bits, m, 1, m, kBatchedSize, encode_lds_budget(m, groups));
-        // Quarter-chunk tiles; the batch kernel's 256 threads hold the groups'
-        // XOR up to m = 4096 (one group: up to m = 8192).
-        if (!slice && s.bits == 16 && kQuarterTiles && (groups == 1 || m <= 4096) &&
+        // Quarter-chunk tiles for m = 4096 (the batch kernel's 256 threads hold
+        // the groups' XOR); m = 8192 encodes take the three passes, as per call,
+        // and a one-group m = 8192 Verify the tile (it compares in the kernel).
+        if (!slice && s.bits == 16 && kQuarterTiles && (m <= 4096 || (groups == 1 && op == RSX_OP_VERIFY)) &&
             (slice = leo_quarter_encode_slice(m, groups)) != 0)
             out.kind = 2;
         int eC = 0, eP = 0, eA = 0;
