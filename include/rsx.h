@@ -158,10 +158,9 @@ int rsx_decode_cache_size(const rsx_codec *c);
  * size and erasure pattern): one launch per output-count class.  Leopard
  * codecs (leopard.go:139 / 393 / 421, leopard8.go likewise): one LDS-resident
  * launch per field, for 16-byte aligned shards whose rows fit one LDS tile
- * (GF(2^16) encodes above the tiles and reconstructs with n >= 8192 work
- * rows run three-pass launches over all such items, their work arrays within
- * 4 GiB per plan; Verify of such encode shapes, m = ceilPow2(parity) above
- * 4096 with several groups or above 8192, is not batched); other Leopard items get RSX_ERR_NOT_SUPPORTED and
+ * (GF(2^16) encodes and Verifies above the tiles and reconstructs with
+ * n >= 8192 work rows run three-pass launches over all such items, their
+ * work arrays within 4 GiB per plan); other Leopard items get RSX_ERR_NOT_SUPPORTED and
  * take the per-stripe call.  There is no single reference counterpart: each
  * item stands for one call of reedsolomon.go:686 / 857 / 1446 / 1634 and is
  * validated as that call is; status[i] receives its result.  Items of one

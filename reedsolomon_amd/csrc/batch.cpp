@@ -677,6 +677,10 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
                 t.a.skew_off = t.a.m - 1 + g * t.a.m;
                 t.a.first = g == 0;
                 t.a.last = g == lp.groups - 1;
+                if (pass == 3 && lp.verify_item >= 0) {  // Verify: pass 3 compares
+                    t.a.flag = b->d_flags.as<uint32_t>() + lp.verify_item;
+                    t.a.flag_value = 1;
+                }
                 t.block0 = L.blocks;
                 const uint64_t nb = leo_enc_pass_blocks(t.a, pass);
                 if (static_cast<uint64_t>(L.blocks) + nb > 0x7FFFFFFFull) return -1;

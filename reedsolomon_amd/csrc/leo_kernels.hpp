@@ -175,6 +175,8 @@ struct LeoEncPassArgs {
     int skew_off;                // group g's IFFT skew: fftSkew + m - 1 + g * m
     int first, last;             // group g is the first / last one
     int C, P, A;                 // leo_multipass_shape(m)
+    uint32_t *flag;              // pass 3: compare with the parity shards instead of storing;
+    uint32_t flag_value;         // a mismatch stores flag_value to *flag
 };
 // pass 1 / 2 once per group in order, then pass 3.
 hipError_t leo_encode_multipass(const LeoDev &f, const LeoEncPassArgs &a, int pass, hipStream_t s);

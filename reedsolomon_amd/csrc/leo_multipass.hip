@@ -360,6 +360,10 @@ __device__ __forceinline__ void enc_pass3(const LeoDev &f, const LeoEncPassArgs 
     }
     const int rows = min(a.P, a.p - base);
     const size_t col = a.col0 + static_cast<size_t>(s) * kSlice;
+    if (a.flag) {
+        compare_rows<F>(w, a.parity + base, rows, col, a.flag, a.flag_value);  // Verify
+        return;
+    }
     for_units(w, rows, [&](int t, int u) { F::gstore(a.parity[base + t] + col, u, Lds<F>::load(w, t, u)); });
 }
 

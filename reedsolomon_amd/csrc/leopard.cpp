@@ -267,10 +267,15 @@ int encode_tile_slice(const rsx_codec *c, size_t size) {
     return slice;
 }
 
+// Encodes without a tile run the three tiled passes (GF(2^16), m >= 2048).
+bool encode_three_pass(const rsx_codec *c) {
+    const LeopardState &s = *c->leo;
+    return s.fused && s.bits == 16 && ceil_pow2(c->m) >= kPassEncodeMin;
+}
+
 // Encode with device pointers (16-byte aligned) for data (k) and parity (p).
-// With `flag` (one-tile shapes only: encode_tile_slice != 0) the parity rows
-// are compared with `parity` instead of written, and a mismatch stores
-// flag_value to *flag.
+// With `flag` (one-tile and three-pass shapes) the parity rows are compared
+// with `parity` instead of written, and a mismatch stores flag_value to *flag.
 int encode_device(rsx_codec *c, const std::vector<uint8_t *> &data, const std::vector<uint8_t *> &parity,
                   size_t size, uint32_t *flag = nullptr, uint32_t flag_value = 0) {
     LeopardState &s = *c->leo;
@@ -296,7 +301,9 @@ int encode_device(rsx_codec *c, const std::vector<uint8_t *> &data, const std::v
     const uint8_t *const *dptr = reinterpret_cast<const uint8_t *const *>(s.ptrs.ptr);
     uint8_t *const *pptr = reinterpret_cast<uint8_t *const *>(s.ptrs.as<uint8_t *>() + static_cast<size_t>(groups) * m);
     const int slice = encode_tile_slice(c, size);
-    if (flag && !slice) return fail(RSX_ERR_INVALID_INPUT, "compare-mode encode needs the one-tile schedule");
+    int eC = 0, eP = 0, eA = 0;
+    const bool passes = !slice && encode_three_pass(c) && leo_multipass_shape(m, &eC, &eP, &eA);
+    if (flag && !slice && !passes) return fail(RSX_ERR_INVALID_INPUT, "compare-mode encode needs a tiled schedule");
     if (slice) {
         const LeoEncodeArgs a{dptr, pptr, size, k, p, m, groups, slice, flag, flag_value};
         RSX_HIP(leo_encode_fused(*s.dev, a, c->stream));
@@ -305,8 +312,7 @@ int encode_device(rsx_codec *c, const std::vector<uint8_t *> &data, const std::v
     // No tile: three tiled passes (per group: IFFT layers < C, then >= C and
     // the XOR into the accumulator; after the last group the FFT); 1.1-2.7x
     // the layered schedule.
-    int eC = 0, eP = 0, eA = 0;
-    if (s.fused && s.bits == 16 && m >= kPassEncodeMin && leo_multipass_shape(m, &eC, &eP, &eA)) {
+    if (passes) {
         const size_t cw = leo_multipass_chunk(m, size);
         if ((err = s.work.reserve(cw * m))) return err;
         if ((err = s.work2.reserve(cw * m))) return err;
@@ -322,6 +328,8 @@ int encode_device(rsx_codec *c, const std::vector<uint8_t *> &data, const std::v
             a.C = eC;
             a.P = eP;
             a.A = eA;
+            a.flag = flag;
+            a.flag_value = flag_value;
             for (int g = 0; g < groups; ++g) {
                 a.rows = dptr + static_cast<size_t>(g) * m;
                 a.mtrunc = std::min(m, k - g * m);
@@ -590,9 +598,10 @@ int leopard_verify(rsx_codec *c, uint8_t *const *shards, const size_t *lens, int
     std::vector<bool> use(total, true), load(total, true), save(total, false);
     Resolved r;
     if ((err = resolve(c, shards, total, size, use, load, r))) return err;
-    if (encode_tile_slice(c, size)) {
+    if (encode_tile_slice(c, size) || encode_three_pass(c)) {
         // The parity rows are recomputed in LDS and compared with the stored
-        // ones by the encode kernel itself: no temporaries, no compare launch.
+        // ones by the encode kernel itself (the one tile, or the encoder's
+        // pass 3): no temporaries, no compare launch.
         uint32_t *flag = nullptr, epoch = 0;
         if ((err = verify_flag_begin(c, &flag, &epoch))) return err;
         std::vector<uint8_t *> data(r.dev.begin(), r.dev.begin() + k), parity(r.dev.begin() + k, r.dev.end());
@@ -725,9 +734,8 @@ int leopard_plan_item(rsx_codec *c, int op, uint8_t *const *shards, const size_t
         // Other items fill the device: size the slice for full workgroups.
         int slice = leo_fused_slice(s.bits, m, 1, m, kBatchedSize, encode_lds_budget(m, groups));
         // Quarter-chunk tiles for m = 4096 (the batch kernel's 256 threads hold
-        // the groups' XOR); m = 8192 encodes take the three passes, as per call,
-        // and a one-group m = 8192 Verify the tile (it compares in the kernel).
-        if (!slice && s.bits == 16 && kQuarterTiles && (m <= 4096 || (groups == 1 && op == RSX_OP_VERIFY)) &&
+        // the groups' XOR); m = 8192 takes the three passes, as per call.
+        if (!slice && s.bits == 16 && kQuarterTiles && m <= 4096 &&
             (slice = leo_quarter_encode_slice(m, groups)) != 0)
             out.kind = 2;
         int eC = 0, eP = 0, eA = 0;
@@ -742,8 +750,6 @@ int leopard_plan_item(rsx_codec *c, int op, uint8_t *const *shards, const size_t
         out.parity_pool = pool.size();
         for (int i = 0; i < p; ++i) pool.push_back(shards[k + i]);
         if (passes) {
-            // Verify compares inside the one-tile kernels only.
-            if (op == RSX_OP_VERIFY) return fail(RSX_ERR_NOT_SUPPORTED, "batched Verify of a three-pass encode");
             out.kind = 4;
             out.k = k;
             out.groups = groups;

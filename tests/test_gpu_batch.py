@@ -288,7 +288,7 @@ def test_batched_three_pass_encodes_match_oracle():
     # GF(2^16) encodes above the tiles (m = 16384 with two groups and with
     # one, m = 8192 with two) run the encoder's passes per group index, in a
     # plan beside a one-tile encode and a three-pass reconstruct; Verify of
-    # such a shape is not taken by the batch.
+    # such a shape compares in pass 3.
     rng = np.random.default_rng(16)
     stripes = []
     for k, m, size in ((20000, 10000, 64), (8000, 16000, 128), (10000, 5000, 64), (100, 20, 64 * 3)):
@@ -305,8 +305,12 @@ def test_batched_three_pass_encodes_match_oracle():
         rec = [None if i in lost else torch.from_numpy(full[i]).to("cuda") for i in range(k + m)]
         items.append((enc, rs.OP_RECONSTRUCT, rec))
         items.append((enc, rs.OP_VERIFY, [torch.from_numpy(x).to("cuda") for x in full]))
+        bad = [torch.from_numpy(x).to("cuda") for x in full]
+        bad[k + m - 1][len(full[0]) - 1] ^= 0x10
+        items.append((enc, rs.OP_VERIFY, bad))
         st = b.run(items)
-        assert st == [0] * (len(items) - 1) + [rs.ErrNotSupported.code], st
+        assert st == [0] * len(items), st
+        assert b.verify_result(len(items) - 2) and not b.verify_result(len(items) - 1)
         for (enc_, k_, m_, full_), (_, _, sh) in zip(stripes, items):
             for i in range(k_, k_ + m_):
                 assert np.array_equal(sh[i].cpu().numpy(), full_[i]), (use_graph, k_, m_, i)
