@@ -164,7 +164,7 @@ bool leo_multipass_shape(int n, int *C, int *P, int *A);
 size_t leo_multipass_chunk(int n, size_t size);
 hipError_t leo_decode_multipass(const LeoDev &f, const LeoMultipassArgs &a, hipStream_t s);
 
-// ---- GF(2^16) encode in three LDS-tiled passes (m >= 16384 rows) ----
+// ---- GF(2^16) encode in three LDS-tiled passes (m >= 2048 rows without a tile) ----
 struct LeoEncPassArgs {
     const uint8_t *const *rows;  // pass 1: the m row pointers of group g (null past k)
     uint8_t *const *parity;      // pass 3: the p parity shards

@@ -1,5 +1,7 @@
-// leo_multipass.hip — Leopard GF(2^16) reconstruct for transforms taller than
-// one LDS tile (n >= 8192 rows; the passes also hold for n = 2048 and 4096).
+// leo_multipass.hip — Leopard GF(2^16) reconstruct and encode for transforms
+// taller than one LDS tile (reconstruct n >= 8192 rows, encode m >= 8192 or
+// small shards; the passes hold from 2048 rows).  The encoder's passes are
+// described at enc_pass1; the reconstruct's here.
 //
 // The n-row transform (leopard.go:421-647: scale by error locators, IFFT,
 // formal derivative, FFT, reveal) runs as three passes over 64-byte column
