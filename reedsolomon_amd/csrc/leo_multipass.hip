@@ -289,7 +289,7 @@ __device__ __forceinline__ void enc_pass1(const LeoDev &f, const LeoEncPassArgs 
     const Tile w{reinterpret_cast<uint8_t *>(lds_raw), a.C + 1, 1, 2};
     load_rows<F>(w, a.rows + base, nullptr, a.C, a.col0 + static_cast<size_t>(s) * kSlice, f);
     for (int dshift = 0; (4 << dshift) <= a.C; dshift += 2)
-        step4<F, true>(w, a.C, base, dshift, a.mtrunc, f.skew + a.skew_off, 0, f);
+        step4<F, true, true>(w, a.C, base, dshift, a.mtrunc, f.skew + a.skew_off, 0, f);
     store_tile<F>(w, a.C, [&](int l) { return wrow(a.work, a.m, s, base + l); });
 }
 
@@ -348,7 +348,7 @@ __device__ __forceinline__ void enc_pass3(const LeoDev &f, const LeoEncPassArgs 
     __syncthreads();
     int dist4 = a.m, dist = a.m >> 2;
     for (; dist != 0; dist4 = dist, dist >>= 2)
-        if (dist < a.P) step4<F, false>(w, a.P, base, log2i(dist), a.p, f.skew, -1, f);
+        if (dist < a.P) step4<F, false, true>(w, a.P, base, log2i(dist), a.p, f.skew, -1, f);
     if (dist4 == 2) {
         const int pairs = min(a.P / 2, (a.p - base + 1) / 2);
         for_rows(w, pairs, [&](int q, int u) {
