@@ -290,6 +290,13 @@ int rsx_gal_mul_slice(int device, uint8_t c, const uint8_t *in, uint8_t *out, si
 uint8_t rsx_inv(uint8_t e);
 /* LowLevel.GF16Mul, gf16.go:6 (Leopard GF(2^16), polynomial 0x1002D). */
 uint16_t rsx_gf16_mul(uint16_t a, uint16_t b);
+/* The Leopard field tables the library computes and uploads: expLUT / logLUT
+ * (order entries), fftSkew (order - 1) and logWalsh (order) of
+ * leopard.go:1104-1195 (bits = 16, order 65536) or leopard8.go:1029-1117
+ * (bits = 8, order 256; values widened to 16 bits).  Any pointer may be
+ * NULL.  No reference counterpart (the Go tables are package-private): it
+ * lets a caller check the field representation the engine computes in. */
+int rsx_leopard_tables(int bits, uint16_t *exp, uint16_t *log, uint16_t *skew, uint16_t *walsh);
 /* LowLevel.GF16MulSliceXor (nout = 1) / GF16MulSliceXor8 (nout = 8),
  * mulslice.go:45-95: outs[j] ^= scalars[j] * in over the Leopard split layout;
  * zero scalars are skipped.  len must be a multiple of 64 and every output

@@ -13,6 +13,13 @@
  * column-separable, so this restatement walks them in fixed 4 KiB chunks.
  * The reference's errorBitfield pruning of the final FFT only skips groups
  * whose outputs are not revealed, so the full truncated FFT is used here.
+ *
+ * Pinned (tests/test_leopard_field_pin.py): the field of both engines by
+ * brute-force carry-less products over the reference's Cantor basis and
+ * polynomial literals; fftSkew by the subspace-polynomial closed form;
+ * logWalsh by the Walsh-Hadamard definition; encode parity by Lagrange
+ * interpolation on small shapes; reconstruct by encode -> erase ->
+ * reconstruct = original (tests/test_oracle_leopard.py).
  */
 #include "rsref.h"
 

@@ -77,6 +77,8 @@ lib.rsx_inv.argtypes = [ctypes.c_uint8]
 lib.rsx_inv.restype = ctypes.c_uint8
 lib.rsx_gf16_mul.argtypes = [ctypes.c_uint16, ctypes.c_uint16]
 lib.rsx_gf16_mul.restype = ctypes.c_uint16
+_U16 = ctypes.POINTER(ctypes.c_uint16)
+lib.rsx_leopard_tables.argtypes = [ctypes.c_int, _U16, _U16, _U16, _U16]
 lib.rsx_gf16_mul_slice_xor.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_uint16), ctypes.c_int, _P, _PP, _SZ,
                                        ctypes.c_size_t]
 _READ_FN = ctypes.CFUNCTYPE(ctypes.c_longlong, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t)
@@ -740,6 +742,19 @@ class Server:
 def Inv(e: int) -> int:
     """mulslice.go:42: multiplicative inverse in GF(2^8) (Inv(0) = 0)."""
     return lib.rsx_inv(e)
+
+
+def leopard_tables(bits: int) -> dict:
+    """The Leopard field tables the engine computes in (leopard.go:1104-1195
+    for bits = 16, leopard8.go:1029-1117 for bits = 8): exp, log, skew
+    (fftSkew) and walsh (logWalsh) as uint16 arrays.  The Go tables are
+    package-private; this exposes them for checks."""
+    order = 1 << bits
+    t = dict(exp=np.zeros(order, np.uint16), log=np.zeros(order, np.uint16),
+             skew=np.zeros(order - 1, np.uint16), walsh=np.zeros(order, np.uint16))
+    ptr = {k: v.ctypes.data_as(_U16) for k, v in t.items()}
+    _check(lib.rsx_leopard_tables(bits, ptr["exp"], ptr["log"], ptr["skew"], ptr["walsh"]))
+    return t
 
 
 class LowLevel:

@@ -63,6 +63,16 @@ extern "C" {
 
 uint8_t rsx_inv(uint8_t e) { return gf().inv(e); }
 
+int rsx_leopard_tables(int bits, uint16_t *exp, uint16_t *log, uint16_t *skew, uint16_t *walsh) {
+    if (bits != 16 && bits != 8) return fail(RSX_ERR_INVALID_INPUT, "bits must be 16 or 8");
+    const LeoField &f = leo_field(bits);
+    if (exp) std::copy(f.exp.begin(), f.exp.end(), exp);
+    if (log) std::copy(f.log.begin(), f.log.end(), log);
+    if (skew) std::copy(f.skew.begin(), f.skew.end(), skew);
+    if (walsh) std::copy(f.walsh.begin(), f.walsh.end(), walsh);
+    return RSX_OK;
+}
+
 uint16_t rsx_gf16_mul(uint16_t a, uint16_t b) {
     // gf16.go:6-16: exp(addMod(log a, log b)) with the sum reduced once more.
     if (a == 0 || b == 0) return 0;

@@ -14,6 +14,9 @@ the reference's own tests as JSON.  Outputs:
                                    :948 gf2p811dMulMatrices
   tests/golden/leo8_gfni.npy       leopard8.go:1271 gf2p811dMulMatricesLeo8
   tests/golden/kat.json            known-answer vectors (sources cited inside)
+  tests/golden/leopard_consts.json leopard.go:100-105 bitwidth/order/modulus/
+                                   polynomial and :1105-1110 cantorBasis;
+                                   leopard8.go:96-104 and :1030-1032 likewise
 """
 from __future__ import annotations
 
@@ -43,6 +46,35 @@ def _block(src: str, name: str) -> str:
     raise ValueError(name)
 
 
+def _local_block(src: str, decl: str) -> str:
+    """Text of the composite literal `decl ... { ... }` inside a function body."""
+    start = src.index(decl)
+    brace = src.index("{", start)
+    return src[brace:src.index("}", brace) + 1]
+
+
+def _const(src: str, name: str) -> str:
+    m = re.search(r"^\s*" + re.escape(name) + r"\s*=\s*(.+?)\s*$", src, flags=re.M)
+    if not m:
+        raise ValueError(name)
+    return m.group(1)
+
+
+def leopard_consts(leo16: str, leo8: str) -> dict:
+    """Field constants of the two Leopard engines, read from the source text."""
+    out = {"_source": "leopard.go:100-105,1105-1110; leopard8.go:96-104,1030-1032"}
+    for tag, src, sfx, basis_decl in (("gf16", leo16, "", "cantorBasis := [bitwidth]ffe{"),
+                                      ("gf8", leo8, "8", "cantorBasis := [bitwidth8]ffe8{")):
+        bitwidth = int(_const(src, "bitwidth" + sfx), 0)
+        assert _const(src, "order" + sfx) == f"1 << bitwidth{sfx}"
+        assert _const(src, "modulus" + sfx) == f"order{sfx} - 1"
+        basis = _numbers(_local_block(src, basis_decl))
+        assert len(basis) == bitwidth, (tag, basis)
+        out[tag] = {"bitwidth": bitwidth, "order": 1 << bitwidth, "modulus": (1 << bitwidth) - 1,
+                    "polynomial": int(_const(src, "polynomial" + sfx), 0), "cantor_basis": basis}
+    return out
+
+
 def _numbers(text: str) -> list[int]:
     return [int(t, 0) for t in re.findall(r"0x[0-9a-fA-F]+|\d+", text)]
 
@@ -53,6 +85,9 @@ def main() -> int:
         return 1
     galois = open(os.path.join(REF, "galois.go")).read()
     leo8 = open(os.path.join(REF, "leopard8.go")).read()
+    leo16 = open(os.path.join(REF, "leopard.go")).read()
+    with open(os.path.join(HERE, "leopard_consts.json"), "w") as f:
+        json.dump(leopard_consts(leo16, leo8), f, indent=1)
 
     def arr(name, n, dtype=np.uint8):
         vals = _numbers(_block(galois, name))

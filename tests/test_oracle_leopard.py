@@ -12,8 +12,8 @@ oracle is pinned by:
     formal-derivative path is structurally independent of the encoder's
     grouped IFFT, so a wrong skew index or dropped group fails it), MDS over
     random k-subsets, Verify true/false, and the reference's error outcomes.
-Leopard parity bytes themselves are pinned only by the line-faithful
-restatement (no reference vector exists).
+The field representation, the skews and the encoder's parity bytes are
+pinned against mathematics in tests/test_leopard_field_pin.py.
 """
 import json
 import os
