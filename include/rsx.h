@@ -71,7 +71,9 @@ typedef struct rsx_options {
     int leopard;               /* RSX_LEOPARD_*                                   */
     int matrix;                /* RSX_MATRIX_*                                    */
     int fast_one_parity;       /* WithFastOneParityMatrix      options.go:260     */
-    int inversion_cache;       /* WithInversionCache (default 1) options.go:152   */
+    int inversion_cache;       /* WithInversionCache (default 1) options.go:152;
+                                  0: every reconstruct inverts afresh, no decode
+                                  plan is kept in HBM (classic engine)           */
     const uint8_t *custom_matrix; /* WithCustomMatrix, row-major custom_rows x custom_cols */
     int custom_rows;
     int custom_cols;
@@ -116,6 +118,10 @@ int rsx_reconstruct(rsx_codec *c, uint8_t *const *shards, const size_t *lens, in
                     uint8_t *filled);
 int rsx_reconstruct_data(rsx_codec *c, uint8_t *const *shards, const size_t *lens, int nshards,
                          uint8_t *filled);
+/* required has nrequired entries: k (ReconstructData of the required data
+ * shards) or TotalShards (any shard), else ErrInvalidInput.  With k entries a
+ * missing parity shard counts as not required (the reference indexes past a
+ * k-entry slice there and panics). */
 int rsx_reconstruct_some(rsx_codec *c, uint8_t *const *shards, const size_t *lens, int nshards,
                          const uint8_t *required, int nrequired, uint8_t *filled);
 

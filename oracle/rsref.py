@@ -218,9 +218,12 @@ class Classic:
         s = _Shards(shards, size)
         filled = np.zeros(max(len(shards), 1), np.uint8)
         if required is not None:
-            req = np.asarray(required, np.uint8)
+            # A k-entry mask: entries past k read as false (the reference
+            # indexes past the slice and panics when a parity shard is missing).
+            req = np.zeros(max(len(shards), len(required)), np.uint8)
+            req[:len(required)] = np.asarray(required, np.uint8)
             err = lib().rsref_reconstruct(_ptr(self.matrix), self.k, self.m, s.ptrs, s.lens, s.n,
-                                          int(data_only), _ptr(req), len(req), _ptr(filled))
+                                          int(data_only), _ptr(req), len(required), _ptr(filled))
         else:
             err = lib().rsref_reconstruct(_ptr(self.matrix), self.k, self.m, s.ptrs, s.lens, s.n,
                                           int(data_only), None, 0, _ptr(filled))

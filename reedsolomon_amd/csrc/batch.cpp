@@ -20,7 +20,7 @@ namespace rsx {
 int check_shards(const size_t *lens, int n, bool nilok);
 size_t shard_size(const size_t *lens, int n);
 int decode_plan(rsx_codec *c, const size_t *lens, bool data_only, const uint8_t *required,
-                const DecodePlan **plan);
+                std::shared_ptr<const DecodePlan> *plan);
 }  // namespace rsx
 
 using namespace rsx;
@@ -406,6 +406,7 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
     // seen and uploaded once; keyed by (host table, codec, decode-cache
     // epoch) so a table freed by a cache flush is never matched again.
     std::map<std::tuple<const void *, const rsx_codec *, uint64_t>, size_t> table_off;
+    std::vector<std::shared_ptr<const DecodePlan>> held_plans;  // decode plans read by this plan
     std::vector<uint32_t> htab;
     size_t fill_off = 0;
 
@@ -515,7 +516,7 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
                 status[it] = fail(RSX_ERR_TOO_FEW_SHARDS, "too few shards given");
                 continue;
             }
-            const DecodePlan *plan = nullptr;
+            std::shared_ptr<const DecodePlan> plan;
             {
                 std::lock_guard<std::mutex> lock(c->mu);
                 err = decode_plan(c, lens, data_only, nullptr, &plan);
@@ -532,6 +533,7 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
             }
             tables = plan->host_tables.data();
             mtot = static_cast<int>(plan->outputs.size());
+            held_plans.push_back(std::move(plan));  // `tables` stays valid and unique while planning
         } else {
             status[it] = fail(RSX_ERR_INVALID_INPUT, "unknown batch op");
             continue;

@@ -464,27 +464,32 @@ namespace rsx {
 // Finds or builds the decode plan for one erasure pattern.  The key is the
 // full presence mask plus the requested outputs, which determines both the
 // inverse (the reference keys its inversion tree by the invalid-row prefix,
-// inversion_tree.go:41-91) and the output rows.  Caller holds c->mu.
+// inversion_tree.go:41-91) and the output rows.  With WithInversionCache
+// (false) (options.go:152-157) every call inverts afresh and nothing is
+// kept beyond the call's own plan.  Caller holds c->mu.
 int decode_plan(rsx_codec *c, const size_t *lens, bool data_only, const uint8_t *required,
-                const DecodePlan **plan) {
+                std::shared_ptr<const DecodePlan> *plan) {
     const int k = c->k, total = c->total;
-    std::string key(1 + 2 * ((total + 7) / 8), '\0');
-    key[0] = static_cast<char>(data_only ? 1 : 0) | static_cast<char>(required ? 2 : 0);
-    for (int i = 0; i < total; ++i) {
-        if (lens[i]) key[1 + i / 8] |= static_cast<char>(1 << (i % 8));
-        if (required && required[i]) key[1 + (total + 7) / 8 + i / 8] |= static_cast<char>(1 << (i % 8));
+    std::string key;
+    if (c->inversion_cache) {
+        key.assign(1 + 2 * ((total + 7) / 8), '\0');
+        key[0] = static_cast<char>(data_only ? 1 : 0) | static_cast<char>(required ? 2 : 0);
+        for (int i = 0; i < total; ++i) {
+            if (lens[i]) key[1 + i / 8] |= static_cast<char>(1 << (i % 8));
+            if (required && required[i]) key[1 + (total + 7) / 8 + i / 8] |= static_cast<char>(1 << (i % 8));
+        }
+        auto it = c->decode_cache.find(key);
+        if (it != c->decode_cache.end()) {
+            *plan = it->second;
+            return RSX_OK;
+        }
+        if (c->decode_cache.size() >= kDecodeCacheCap) {
+            RSX_HIP(hipStreamSynchronize(c->stream));
+            c->decode_cache.clear();
+            ++c->decode_epoch;
+        }
     }
-    auto it = c->decode_cache.find(key);
-    if (it != c->decode_cache.end()) {
-        *plan = it->second.get();
-        return RSX_OK;
-    }
-    if (c->decode_cache.size() >= kDecodeCacheCap) {
-        RSX_HIP(hipStreamSynchronize(c->stream));
-        c->decode_cache.clear();
-        ++c->decode_epoch;
-    }
-    auto p = std::make_unique<DecodePlan>();
+    auto p = std::make_shared<DecodePlan>();
     // reedsolomon.go:1718-1730: the first k present rows.
     for (int r = 0; r < total && static_cast<int>(p->inputs.size()) < k; ++r)
         if (lens[r]) p->inputs.push_back(r);
@@ -516,8 +521,9 @@ int decode_plan(rsx_codec *c, const size_t *lens, bool data_only, const uint8_t 
         int err = upload_tables(rp, k, p->tables, p->host_tables, c->stream);
         if (err) return err;
     }
-    *plan = p.get();
-    c->decode_cache.emplace(std::move(key), std::move(p));
+    if (c->inversion_cache) c->decode_cache.emplace(std::move(key), p);
+    else c->uncached_plan = p;  // frees the previous call's plan (hipFree waits for the device)
+    *plan = std::move(p);
     return RSX_OK;
 }
 
@@ -545,7 +551,7 @@ int reconstruct_classic(rsx_codec *c, uint8_t *const *shards, const size_t *lens
     (void)hipSetDevice(c->device);
 
     std::lock_guard<std::mutex> lock(c->mu);
-    const DecodePlan *plan = nullptr;
+    std::shared_ptr<const DecodePlan> plan;
     if ((err = decode_plan(c, lens, data_only, required, &plan))) return err;
     const int nout = static_cast<int>(plan->outputs.size());
     std::vector<const void *> ptrs;
@@ -605,9 +611,13 @@ int rsx_reconstruct_some(rsx_codec *c, uint8_t *const *shards, const size_t *len
     std::memset(filled, 0, static_cast<size_t>(std::max(nshards, 0)));
     if (c->engine)
         return leopard_reconstruct(c, shards, lens, nshards, nrequired == c->total, filled);
-    if (nrequired != c->k && nrequired != c->total)
+    if ((nrequired != c->k && nrequired != c->total) || !required)
         return fail(RSX_ERR_INVALID_INPUT, "invalid input");
-    return reconstruct_classic(c, shards, lens, nshards, nrequired == c->k, required, nrequired,
+    // A k-entry mask leaves the parity shards not required (the reference
+    // indexes required[i] for every missing shard and panics past k).
+    std::vector<uint8_t> req(static_cast<size_t>(c->total), 0);
+    std::copy(required, required + nrequired, req.begin());
+    return reconstruct_classic(c, shards, lens, nshards, nrequired == c->k, req.data(), c->total,
                                filled);
 }
 

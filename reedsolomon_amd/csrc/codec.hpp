@@ -107,8 +107,13 @@ struct rsx_codec {
     std::vector<uint32_t> parity_tables_host;
 
     std::mutex mu;  // guards the caches and the staging buffers below
-    std::unordered_map<std::string, std::unique_ptr<rsx::DecodePlan>> decode_cache;
+    // Plans are shared so a batch planner that copied a pointer under `mu`
+    // keeps its plan alive after a flush.
+    std::unordered_map<std::string, std::shared_ptr<const rsx::DecodePlan>> decode_cache;
     uint64_t decode_epoch = 0;  // incremented whenever decode_cache is flushed
+    // WithInversionCache(false): the plan of the last call, held until the
+    // next call replaces it (its device tables are read after the call returns).
+    std::shared_ptr<const rsx::DecodePlan> uncached_plan;
     rsx::DeviceBuffer staging;  // host-memory calls: shards staged in HBM
     uint32_t *host_flag = nullptr;     // Verify: pinned word, the epoch of the last mismatch
     uint32_t *dev_flag = nullptr;      // Verify: host_flag's device address

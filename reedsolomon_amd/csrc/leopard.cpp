@@ -79,7 +79,9 @@ struct LeopardState {
     DeviceBuffer work, work2, ptrs, rows, stage, scratch, flag, cmp;
     std::vector<uint8_t *> h_ptrs;  // what `ptrs` holds on the device
     int enc_rows = 0;               // `rows` holds 0 .. enc_rows-1 (encode's output rows)
-    std::unordered_map<std::string, std::unique_ptr<LeoPattern>> patterns;
+    // Shared so a batch planner that looked a pattern up under the codec's
+    // lock keeps it alive if another thread flushes the map afterwards.
+    std::unordered_map<std::string, std::shared_ptr<LeoPattern>> patterns;
 };
 
 void LeopardDeleter::operator()(LeopardState *s) const { delete s; }
@@ -370,7 +372,7 @@ int encode_device(rsx_codec *c, const std::vector<uint8_t *> &data, const std::v
     return RSX_OK;
 }
 
-int pattern_for(rsx_codec *c, const uint8_t *erased, bool recover_all, LeoPattern **out) {
+int pattern_for(rsx_codec *c, const uint8_t *erased, bool recover_all, std::shared_ptr<LeoPattern> *out) {
     LeopardState &s = *c->leo;
     const int k = c->k, p = c->m, total = c->total;
     std::string key(1 + (total + 7) / 8, '\0');
@@ -379,22 +381,22 @@ int pattern_for(rsx_codec *c, const uint8_t *erased, bool recover_all, LeoPatter
         if (erased[i]) key[1 + i / 8] |= static_cast<char>(1 << (i % 8));
     auto it = s.patterns.find(key);
     if (it != s.patterns.end()) {
-        *out = it->second.get();
+        *out = it->second;
         return RSX_OK;
     }
     if (s.patterns.size() >= 4096) {
         RSX_HIP(hipStreamSynchronize(c->stream));
         s.patterns.clear();
     }
-    auto pat = std::make_unique<LeoPattern>();
+    auto pat = std::make_shared<LeoPattern>();
     pat->errlocs = leo_error_locators(*s.field, k, p, erased, &pat->input_count);
     int err = pat->d_errlocs.reserve(pat->errlocs.size() * 2);
     if (err) return err;
     RSX_HIP(hipMemcpyAsync(pat->d_errlocs.ptr, pat->errlocs.data(), pat->errlocs.size() * 2,
                            hipMemcpyHostToDevice, c->stream));
     RSX_HIP(hipStreamSynchronize(c->stream));
-    *out = pat.get();
-    s.patterns.emplace(std::move(key), std::move(pat));
+    s.patterns.emplace(std::move(key), pat);
+    *out = std::move(pat);
     return RSX_OK;
 }
 
@@ -405,7 +407,7 @@ int reconstruct_device(rsx_codec *c, const std::vector<uint8_t *> &shards, const
     LeopardState &s = *c->leo;
     const int k = c->k, p = c->m, total = c->total;
     const int m = ceil_pow2(p), n = ceil_pow2(m + k);
-    LeoPattern *pat = nullptr;
+    std::shared_ptr<LeoPattern> pat;
     int err = pattern_for(c, erased, recover_all, &pat);
     if (err) return err;
     // Row table for the scaled load: parity rows, zero rows, data rows, zero rows.
@@ -660,10 +662,15 @@ int leopard_reconstruct(rsx_codec *c, uint8_t *const *shards, const size_t *lens
     }
     Resolved r;
     if ((err = resolve(c, shards, total, size, use, load, r))) return err;
-    if ((err = reconstruct_device(c, r.dev, erased.data(), recover_all, size, filled))) return err;
-    if ((err = unstage(c, shards, r, save, size))) return err;
-    if (r.staged) RSX_HIP(hipStreamSynchronize(c->stream));
-    return RSX_OK;
+    err = reconstruct_device(c, r.dev, erased.data(), recover_all, size, filled);
+    if (!err) err = unstage(c, shards, r, save, size);
+    if (!err && r.staged) {
+        const hipError_t e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) err = hip_fail(e, "hipStreamSynchronize(c->stream)");
+    }
+    // filled marks only shards written by a call that succeeded.
+    if (err) std::memset(filled, 0, static_cast<size_t>(total));
+    return err;
 }
 
 namespace {
@@ -777,7 +784,7 @@ int leopard_plan_item(rsx_codec *c, int op, uint8_t *const *shards, const size_t
     int mC = 0, mP = 0, mA = 0;
     const bool passes = !slice && s.bits == 16 && s.fused && leo_multipass_shape(n, &mC, &mP, &mA);
     if (!slice && !passes) return fail(RSX_ERR_NOT_SUPPORTED, "Leopard shape too tall for the batched kernel");
-    LeoPattern *pat = nullptr;
+    std::shared_ptr<LeoPattern> pat;
     {
         std::lock_guard<std::mutex> lock(c->mu);
         if ((err = pattern_for(c, erased.data(), recover_all, &pat))) return err;

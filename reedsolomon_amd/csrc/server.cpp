@@ -341,19 +341,28 @@ int rsx_server_run_trace(rsx_server *s, const rsx_request *reqs, const uint64_t 
     }
     rsx_server_drain(s);
     Clock::time_point last = t0;
+    int rc = RSX_OK;
     {
         std::lock_guard<std::mutex> lock(s->mu);
         for (int i = 0; i < n; ++i) {
-            const Req &q = s->reqs[tickets[i]];
+            const auto it = s->reqs.find(tickets[i]);
+            if (it == s->reqs.end()) {  // already reported by a concurrent poll / wait
+                status[i] = RSX_ERR_INVALID_INPUT;
+                verify_ok[i] = 0;
+                latency_ns[i] = 0;
+                rc = rsx::fail(RSX_ERR_INVALID_INPUT, "ticket reported by another call");
+                continue;
+            }
+            const Req &q = it->second;
             status[i] = q.status;
             verify_ok[i] = q.verify_ok;
             latency_ns[i] = q.latency_ns;
             last = std::max(last, q.arrival + std::chrono::nanoseconds(q.latency_ns));
-            s->reqs.erase(tickets[i]);
+            s->reqs.erase(it);
         }
     }
     *makespan_ns = static_cast<uint64_t>(std::chrono::duration_cast<std::chrono::nanoseconds>(last - t0).count());
-    return RSX_OK;
+    return rc;
 }
 
 int rsx_server_stats(const rsx_server *s, uint64_t *batches, uint64_t *requests, uint64_t *max_inflight_bytes) {

@@ -179,6 +179,30 @@ def test_reconstruct_modes_vs_oracle():
     assert enc.DecodeCacheSize() > 0
 
 
+def test_inversion_cache_off():
+    # WithInversionCache(false) (options.go:152-157): every reconstruct
+    # inverts afresh and nothing is cached; results equal the oracle's,
+    # also when the same pattern repeats back to back (the previous call's
+    # plan is released while this one runs).
+    rng = np.random.default_rng(22)
+    k, m, size = 6, 3, 4096
+    data = rand_data(rng, k, size)
+    o, full = oracle_encode(k, m, data)
+    enc = rs.New(k, m, rs.WithInversionCache(False))
+    patterns = [{0}, {0}, {1, 7}, {2, 4, 8}, {1, 7}, {5}]
+    works = []
+    for lost in patterns:
+        ref = [None if i in lost else full[i].copy() for i in range(k + m)]
+        assert o.reconstruct(ref) == 0
+        work = [None if i in lost else dev(full[i]) for i in range(k + m)]
+        enc.Reconstruct(work)
+        works.append((lost, ref, work))
+    for lost, ref, work in works:
+        for i in range(k + m):
+            assert np.array_equal(host(work[i]), ref[i]), (lost, i)
+    assert enc.DecodeCacheSize() == 0
+
+
 def test_reconstruct_some_every_mask():
     # TestRSReconstructSome (reedsolomon_test.go:2319-2424)
     rng = np.random.default_rng(31)
@@ -186,10 +210,11 @@ def test_reconstruct_some_every_mask():
     data = rand_data(rng, k, 100)
     o, full = oracle_encode(k, m, data)
     enc = rs.New(k, m)
-    for req_len in (k, k + m):
+    # A k-entry mask with a parity shard missing ({0, 4}): the reference
+    # panics there; here the parity counts as not required.
+    for req_len, lost in ((k, {0, 2}), (k, {0, 4}), (k + m, {0, 4})):
         for mask in range(1 << req_len):
             req = [bool(mask >> i & 1) for i in range(req_len)]
-            lost = {0, 4} if req_len == k + m else {0, 2}
             ref = [None if i in lost else full[i].copy() for i in range(k + m)]
             assert o.reconstruct_some(ref, req) == 0
             work = [None if i in lost else dev(full[i]) for i in range(k + m)]

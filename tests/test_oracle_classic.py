@@ -204,6 +204,25 @@ def test_reconstruct_some_matches_required_mask():
                 assert work[i] is None
 
 
+def test_reconstruct_some_data_mask_with_parity_missing():
+    # A k-entry mask (ReconstructData of the required data shards) while a
+    # parity shard is missing: only the required data shards are rebuilt.
+    rng = np.random.default_rng(12)
+    k, m = 3, 2
+    codec = rsref.Classic(k, m)
+    shards = _rand_shards(rng, k, m, 50)
+    codec.encode(shards)
+    for mask in range(1 << k):
+        req = [bool(mask >> i & 1) for i in range(k)]
+        work = [None, shards[1].copy(), shards[2].copy(), shards[3].copy(), None]
+        assert codec.reconstruct_some(work, req) == 0
+        assert work[4] is None
+        if req[0]:
+            assert (work[0] == shards[0]).all()
+        else:
+            assert work[0] is None
+
+
 def test_encode_idx_equals_encode():
     rng = np.random.default_rng(5)
     k, m = 6, 3
