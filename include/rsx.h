@@ -125,6 +125,21 @@ int rsx_reconstruct_data(rsx_codec *c, uint8_t *const *shards, const size_t *len
 int rsx_reconstruct_some(rsx_codec *c, uint8_t *const *shards, const size_t *lens, int nshards,
                          const uint8_t *required, int nrequired, uint8_t *filled);
 
+/* No counterpart in the reference.  Reconstruct (data_only = 0) or
+ * ReconstructData (data_only = 1) of the classic engine, same results as
+ * rsx_reconstruct / rsx_reconstruct_data, for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): the call allocates nothing and copies nothing between host and
+ * device.  The decode tables go to a fixed 64 KiB device array per GPU (held
+ * until the process ends), so k x (shards rebuilt) must be <= 3276, else
+ * ErrNotSupported.  Shards must be device memory (else ErrMixedMemory) and
+ * the call is stream-ordered on the codec's stream, like rsx_reconstruct.
+ * Calls on one GPU whose work overlaps (different codecs on different
+ * streams) share that array and race: order them.  Leopard codecs:
+ * ErrNotSupported. */
+int rsx_reconstruct_noalloc(rsx_codec *c, uint8_t *const *shards, const size_t *lens, int nshards,
+                            int data_only, uint8_t *filled);
+
 /* Encoder.Update, reedsolomon.go:762-818 (Leopard: ErrNotSupported,
  * leopard.go:316).  shards: old data (NULL where unchanged) then the parity,
  * updated in place; new_data: k entries, NULL where unchanged.  A NULL

@@ -70,6 +70,7 @@ lib.rsx_verify.argtypes = [_P, _PP, _SZ, ctypes.c_int, ctypes.POINTER(ctypes.c_i
 lib.rsx_reconstruct.argtypes = [_P, _PP, _SZ, ctypes.c_int, _U8]
 lib.rsx_reconstruct_data.argtypes = [_P, _PP, _SZ, ctypes.c_int, _U8]
 lib.rsx_reconstruct_some.argtypes = [_P, _PP, _SZ, ctypes.c_int, _U8, ctypes.c_int, _U8]
+lib.rsx_reconstruct_noalloc.argtypes = [_P, _PP, _SZ, ctypes.c_int, ctypes.c_int, _U8]
 lib.rsx_update.argtypes = [_P, _PP, _SZ, ctypes.c_int, _PP, _SZ, ctypes.c_int]
 lib.rsx_decode_idx.argtypes = [_P, _PP, _SZ, ctypes.c_int, _U8, ctypes.c_int, _PP, _SZ, ctypes.c_int]
 lib.rsx_gal_mul_slice.argtypes = [ctypes.c_int, ctypes.c_uint8, _P, _P, ctypes.c_size_t, ctypes.c_int]
@@ -488,6 +489,14 @@ class Encoder:
 
     def ReconstructData(self, shards) -> None:
         self._reconstruct(lib.rsx_reconstruct_data, shards)
+
+    def ReconstructNoAlloc(self, shards, data_only: bool = False) -> None:
+        """Reconstruct (or ReconstructData) of device shards through
+        rsx_reconstruct_noalloc: the library allocates nothing during the call
+        (decode tables go to a fixed per-GPU scratch array).  No reference
+        counterpart.  Absent shards get output buffers here, as for
+        Reconstruct."""
+        self._reconstruct(lib.rsx_reconstruct_noalloc, shards, 1 if data_only else 0)
 
     def ReconstructSome(self, shards, required) -> None:
         req = (ctypes.c_uint8 * max(len(required), 1))(*[1 if r else 0 for r in required])

@@ -292,6 +292,7 @@ int rsx_new(int k, int m, const rsx_options *opt, rsx_codec **out) {
         for (int i = 0; i < m; ++i) rows[i] = c->matrix.row(k + i);
         err = upload_tables(rows, k, c->parity_tables, c->parity_tables_host, c->stream);
         if (err) return err;
+        c->noalloc_work.assign(static_cast<size_t>(2 * k + m) * k, 0);
     }
     *out = c.release();
     return RSX_OK;

@@ -119,6 +119,9 @@ struct rsx_codec {
     uint32_t *dev_flag = nullptr;      // Verify: host_flag's device address
     uint32_t verify_epoch = 0;         // Verify: value a mismatch of the current call stores
     rsx::DeviceBuffer aux_tables;  // Update / DecodeIdx: per-call coefficient tables
+    // rsx_reconstruct_noalloc: host workspace sized at New (k x 2k inversion
+    // array + m decode rows of k bytes), so that call allocates nothing.
+    std::vector<uint8_t> noalloc_work;
     rsx::DeviceBuffer one_table;   // the table of coefficient 1 (XOR passes)
 
     std::unique_ptr<rsx::LeopardState, rsx::LeopardDeleter> leo;

@@ -25,15 +25,9 @@ Matrix multiply(const Matrix &a, const Matrix &b) {
 // Gauss-Jordan on [m | I] (matrix.go:192-265): pivot by swapping with the
 // first lower row that has a nonzero entry, scale the pivot row to 1,
 // eliminate below, then above.
-bool invert(const Matrix &m, Matrix &out) {
+bool invert_in_place(uint8_t *a, int n) {
     const Gf256 &f = gf();
-    const int n = m.rows;
     const int w = 2 * n;
-    std::vector<uint8_t> a(static_cast<size_t>(n) * w, 0);
-    for (int r = 0; r < n; ++r) {
-        std::memcpy(&a[static_cast<size_t>(r) * w], m.row(r), n);
-        a[static_cast<size_t>(r) * w + n + r] = 1;
-    }
     auto A = [&](int r, int c) -> uint8_t & { return a[static_cast<size_t>(r) * w + c]; };
     for (int r = 0; r < n; ++r) {
         if (A(r, r) == 0) {
@@ -60,8 +54,20 @@ bool invert(const Matrix &m, Matrix &out) {
             if (s)
                 for (int c = 0; c < w; ++c) A(above, c) ^= f.mul[s][A(d, c)];
         }
+    return true;
+}
+
+bool invert(const Matrix &m, Matrix &out) {
+    const int n = m.rows;
+    const int w = 2 * n;
+    std::vector<uint8_t> a(static_cast<size_t>(n) * w, 0);
+    for (int r = 0; r < n; ++r) {
+        std::memcpy(&a[static_cast<size_t>(r) * w], m.row(r), n);
+        a[static_cast<size_t>(r) * w + n + r] = 1;
+    }
+    if (!invert_in_place(a.data(), n)) return false;
     out = Matrix(n, n);
-    for (int r = 0; r < n; ++r) std::memcpy(out.row(r), &A(r, n), n);
+    for (int r = 0; r < n; ++r) std::memcpy(out.row(r), &a[static_cast<size_t>(r) * w + n], n);
     return true;
 }
 

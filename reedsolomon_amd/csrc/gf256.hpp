@@ -57,6 +57,9 @@ struct Matrix {
 Matrix multiply(const Matrix &a, const Matrix &b);
 // Returns false when singular.
 bool invert(const Matrix &m, Matrix &out);
+// Gauss-Jordan on an n x 2n row-major array [m | I] (matrix.go:192-265):
+// on success the right half holds inv(m).  Allocates nothing.
+bool invert_in_place(uint8_t *aug, int n);
 
 // Coding matrix (k+m) x k of New; kind is one of RSX_MATRIX_* (fast_one_parity
 // applies only when m == 1).  custom, when non-null, holds the parity rows.
