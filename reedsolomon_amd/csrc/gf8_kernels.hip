@@ -22,21 +22,6 @@ namespace {
 
 constexpr int kBlock = 256;
 
-struct Split3 {
-    uint32_t g0, g1, g2;
-};
-
-__device__ __forceinline__ Split3 split(uint32_t x) {
-    return {x & 0x07070707u, (x >> 3) & 0x07070707u, (x >> 6) & 0x03030303u};
-}
-
-// Product of 4 packed bytes by one coefficient, given its 5-dword table.
-__device__ __forceinline__ uint32_t mul4(const Split3 &s, uint32_t t0, uint32_t t1, uint32_t t2,
-                                         uint32_t t3, uint32_t t4) {
-    return __builtin_amdgcn_perm(t1, t0, s.g0) ^ __builtin_amdgcn_perm(t3, t2, s.g1) ^
-           __builtin_amdgcn_perm(t4, t4, s.g2);
-}
-
 // Streaming pass: each wave owns R consecutive KiB-wide column chunks and
 // walks all k inputs (gf8_device.hpp); vectors past the last whole chunk (all
 // of them when R == 0) are done one per lane.
@@ -70,10 +55,8 @@ __global__ __launch_bounds__(kBlock) void gf8_pass_bytes(const Gf8Pass p, size_t
         for (int i = 0; i < p.nout; ++i) {
             uint32_t acc = p.mode == kGf8Accumulate ? p.out[i][b] : 0u;
             const uint32_t *tab = p.tables + static_cast<size_t>(p.o0 + i) * 5;
-            for (int j = 0; j < p.k; ++j, tab += static_cast<size_t>(p.mtot) * 5) {
-                const Split3 s = split(p.in[j][b]);
-                acc ^= mul4(s, tab[0], tab[1], tab[2], tab[3], tab[4]);
-            }
+            for (int j = 0; j < p.k; ++j, tab += static_cast<size_t>(p.mtot) * 5)
+                acc ^= gf8dev::mul_byte(p.in[j][b], make_uint4(tab[0], tab[1], tab[2], tab[3]), tab[4]);
             if (p.mode != kGf8Compare) p.out[i][b] = static_cast<uint8_t>(acc);
             else if (p.out[i][b] != static_cast<uint8_t>(acc)) *p.flag = p.flag_value;
         }
