@@ -156,7 +156,15 @@ struct LeoMultipassArgs {
     int n, ifft_trunc, fft_trunc;
     int C, P, A;                 // tile shapes from leo_multipass_shape
     const uint32_t *need;        // leo_fft_need bits, or null
+    // Bit b: rows [b*P, (b+1)*P) hold an output row (leo_pass3_blocks).  Pass 3
+    // runs only those row blocks and pass 2 stores only their rows.
+    uint64_t pass3_blocks;
 };
+
+// Bit b set when one of `out_rows` lies in rows [b*P, (b+1)*P) (n / P <= 64).
+uint64_t leo_pass3_blocks(int P, const std::vector<int> &out_rows);
+// Row blocks of pass 3 to launch: one past the highest set bit.
+int leo_pass3_grid(uint64_t blocks);
 
 // Tile shapes for n rows; false when n is out of the multipass range.
 bool leo_multipass_shape(int n, int *C, int *P, int *A);

@@ -47,8 +47,8 @@ __device__ __forceinline__ uint8_t *wrow(uint8_t *work, int n, int s, int r) {
     return work + (static_cast<size_t>(s) * n + r) * kSlice;
 }
 
-// Copies rows [0, nrows) of a tile from global rows rp(l) (64 bytes each),
-// several loads in flight per thread.
+// Copies rows [0, nrows) of a tile from global rows rp(l) (64 bytes each; a
+// null row reads as zero), several loads in flight per thread.
 template <class F, class RowPtr>
 __device__ __forceinline__ void load_tile(const Tile &w, int nrows, RowPtr rp) {
     constexpr int kBatch = 4;
@@ -59,7 +59,10 @@ __device__ __forceinline__ void load_tile(const Tile &w, int nrows, RowPtr rp) {
 #pragma unroll
         for (int j = 0; j < kBatch; ++j) {
             const int it = it0 + j * blockDim.x;
-            if (it < items) v[j] = F::gload(rp(it >> w.ushift), it & umask);
+            if (it < items) {
+                const uint8_t *row = rp(it >> w.ushift);
+                v[j] = row ? F::gload(row, it & umask) : F::zero();
+            }
         }
 #pragma unroll
         for (int j = 0; j < kBatch; ++j) {
@@ -69,9 +72,13 @@ __device__ __forceinline__ void load_tile(const Tile &w, int nrows, RowPtr rp) {
     }
 }
 
+// Stores rows [0, nrows) of a tile to global rows rp(l); null rows are skipped.
 template <class F, class RowPtr>
 __device__ __forceinline__ void store_tile(const Tile &w, int nrows, RowPtr rp) {
-    for_units(w, nrows, [&](int l, int u) { F::gstore(rp(l), u, Lds<F>::load(w, l, u)); });
+    for_units(w, nrows, [&](int l, int u) {
+        uint8_t *row = rp(l);
+        if (row) F::gstore(row, u, Lds<F>::load(w, l, u));
+    });
 }
 
 // One radix-4 step over tiles whose local row l = j*A + a is global row
@@ -150,6 +157,10 @@ template <class F>
 __device__ __forceinline__ void pass1(const LeoDev &f, const LeoMultipassArgs &a, int s, int by) {
     extern __shared__ uint4 lds_raw[];
     const int base = by * a.C;
+    // Rows from inputCount on are zero (leopard.go:488-495) and the IFFT
+    // layers of this pass stay within the block: a block past it stays zero
+    // and pass 2 reads it as zero without loading it.
+    if (base >= a.ifft_trunc) return;
     const Tile w{reinterpret_cast<uint8_t *>(lds_raw), a.C + 1, 1, 2};
     load_rows<F>(w, a.rows + base, a.errlocs + base, a.C, a.col0 + static_cast<size_t>(s) * kSlice, f);
     for (int dshift = 0; (4 << dshift) <= a.C; dshift += 2)
@@ -168,7 +179,13 @@ __device__ __forceinline__ void pass2(const LeoDev &f, const LeoMultipassArgs &a
     const Tile in{lds, R + 1, 1, 2};
     const Tile y{lds + kSlice * (R + 1), R + 1, 1, 2};
     auto grow = [&](int l) { return lo0 + (l & (a.A - 1)) + (l >> ashift) * a.P; };
-    load_tile<F>(in, R, [&](int l) { return wrow(a.work, a.n, s, grow(l)); });
+    // Pass 1 left the blocks of C rows from ceil(inputCount / C) * C zero
+    // without writing them.
+    const int zero_from = (a.ifft_trunc + a.C - 1) / a.C * a.C;
+    load_tile<F>(in, R, [&](int l) -> const uint8_t * {
+        const int r = grow(l);
+        return r < zero_from ? wrow(a.work, a.n, s, r) : nullptr;
+    });
     __syncthreads();
     // IFFT layers with distance >= C.
     int dist = a.C;
@@ -207,16 +224,20 @@ __device__ __forceinline__ void pass2(const LeoDev &f, const LeoMultipassArgs &a
     // FFT steps with distance >= P on both tiles.
     for (int d = a.n >> 2; d >= a.P; d >>= 2)
         strided_step4<F, false>(in, &y, ashift, J, a.P, d, a.fft_trunc, f.skew - 1, f);
-    store_tile<F>(in, R, [&](int l) { return wrow(a.work2, a.n, s, grow(l)); });  // v
-    store_tile<F>(y, R, [&](int l) { return wrow(a.work, a.n, s, grow(l)); });    // u
+    // Only the row blocks pass 3 runs are read again (row block = l >> ashift).
+    auto kept = [&](int l) { return ((a.pass3_blocks >> (l >> ashift)) & 1u) != 0; };
+    store_tile<F>(in, R, [&](int l) { return kept(l) ? wrow(a.work2, a.n, s, grow(l)) : nullptr; });  // v
+    store_tile<F>(y, R, [&](int l) { return kept(l) ? wrow(a.work, a.n, s, grow(l)) : nullptr; });    // u
 }
 
 template <class F, int MAXD>
 __device__ __forceinline__ void pass3(const LeoDev &f, const LeoMultipassArgs &a, int s, int by) {
     extern __shared__ uint4 lds_raw[];
+    // A row block without an output row writes nothing (uniform per workgroup).
+    if (!((a.pass3_blocks >> by) & 1u)) return;
     const int base = by * a.P;
     const Tile w{reinterpret_cast<uint8_t *>(lds_raw), a.P + 1, 1, 2};
-    load_tile<F>(w, a.P, [&](int l) { return wrow(a.work2, a.n, s, base + l); });  // v
+    load_tile<F>(w, a.P, [&](int l) -> const uint8_t * { return wrow(a.work2, a.n, s, base + l); });  // v
     __syncthreads();
     // z = u + L(v): L over the row bits < P.  Registers first, then LDS.
     {
@@ -501,11 +522,23 @@ bool leo_multipass_shape(int n, int *C, int *P, int *A) {
     int p = n >> 2;
     while (p > 1024) p >>= 2;
     const int J = n / p;
-    if (J > 512) return false;
+    if (J > 64) return false;  // pass 3's row blocks fit one 64-bit mask (never for n <= 65536)
     *C = 1024;
     *P = p;
     *A = 512 / J;
     return true;
+}
+
+uint64_t leo_pass3_blocks(int P, const std::vector<int> &out_rows) {
+    uint64_t b = 0;
+    for (int r : out_rows) b |= uint64_t{1} << (r / P);  // n / P <= 64 (leo_multipass_shape)
+    return b;
+}
+
+int leo_pass3_grid(uint64_t blocks) {
+    int g = 0;
+    while (g < 64 && (blocks >> g) != 0) ++g;
+    return g;
 }
 
 // Column chunk of one pass sequence: each work array is at most 512 MiB, so
@@ -539,12 +572,17 @@ hipError_t leo_decode_multipass(const LeoDev &f, const LeoMultipassArgs &a, hipS
                          : reinterpret_cast<const void *>(k_dec_pass3<F16, kDerivMax>),
                    lds3)) != hipSuccess)
         return e;
-    hipLaunchKernelGGL(k_dec_pass1<F16>, dim3(slices, a.n / a.C), dim3(kMB1), lds1, s, f, a);
+    if (a.pass3_blocks == 0) return hipSuccess;  // no output row
+    // Row blocks of pass 1 past inputCount and of pass 3 past the last one
+    // with an output row have nothing to do and are not launched.
+    const unsigned by1 = static_cast<unsigned>((std::min(a.ifft_trunc, a.n) + a.C - 1) / a.C);
+    const unsigned by3 = static_cast<unsigned>(leo_pass3_grid(a.pass3_blocks));
+    hipLaunchKernelGGL(k_dec_pass1<F16>, dim3(slices, by1), dim3(kMB1), lds1, s, f, a);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(k_dec_pass2<F16>, dim3(slices, a.P / a.A), dim3(kMB2), lds2, s, f, a);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    if (small) hipLaunchKernelGGL((k_dec_pass3<F16, kDerivMax / 2>), dim3(slices, a.n / a.P), dim3(kMB), lds3, s, f, a);
-    else hipLaunchKernelGGL((k_dec_pass3<F16, kDerivMax>), dim3(slices, a.n / a.P), dim3(kMB), lds3, s, f, a);
+    if (small) hipLaunchKernelGGL((k_dec_pass3<F16, kDerivMax / 2>), dim3(slices, by3), dim3(kMB), lds3, s, f, a);
+    else hipLaunchKernelGGL((k_dec_pass3<F16, kDerivMax>), dim3(slices, by3), dim3(kMB), lds3, s, f, a);
     return hipGetLastError();
 }
 

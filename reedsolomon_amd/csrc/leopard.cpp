@@ -68,6 +68,7 @@ struct LeoPattern {
     // rows of the erased shards, their reveal logs, and row -> output index.
     bool outs_ready = false;
     int nout = 0;
+    uint64_t pass3_blocks = 0;  // three-pass reconstruct: row blocks with an output row
     DeviceBuffer d_rows, d_logs, d_outidx, d_need;
 };
 
@@ -448,6 +449,8 @@ int reconstruct_device(rsx_codec *c, const std::vector<uint8_t *> &shards, const
         RSX_HIP(hipMemcpyAsync(pat->d_outidx.ptr, out_index.data(), out_index.size() * sizeof(int),
                                hipMemcpyHostToDevice, c->stream));
         pat->nout = no;
+        int pC = 0, pP = 0, pA = 0;
+        if (leo_multipass_shape(n, &pC, &pP, &pA)) pat->pass3_blocks = leo_pass3_blocks(pP, rows);
         pat->outs_ready = true;
     }
     const int nout = pat->nout;
@@ -478,7 +481,7 @@ int reconstruct_device(rsx_codec *c, const std::vector<uint8_t *> &shards, const
             const LeoMultipassArgs a{lptr, pat->d_errlocs.as<uint16_t>(), pat->d_outidx.as<int>(), pat->d_logs.as<uint16_t>(),
                                      optr, s.work.as<uint8_t>(), s.work2.as<uint8_t>(), off,
                                      static_cast<int>(cols / 64), n, ifft_trunc, m + k, mC, mP, mA,
-                                     pat->d_need.as<uint32_t>()};
+                                     pat->d_need.as<uint32_t>(), pat->pass3_blocks};
             RSX_HIP(leo_decode_multipass(*s.dev, a, c->stream));
         }
         return RSX_OK;
@@ -817,7 +820,8 @@ int leopard_plan_item(rsx_codec *c, int op, uint8_t *const *shards, const size_t
         out.outidx_off = aux_append(aux, out_index.data(), out_index.size() * sizeof(int));
         out.kind = 3;
         out.mp = LeoMultipassArgs{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0,
-                                  static_cast<int>(size / 64), n, pat->input_count, m + k, mC, mP, mA, nullptr};
+                                  static_cast<int>(size / 64), n, pat->input_count, m + k, mC, mP, mA, nullptr,
+                                  leo_pass3_blocks(mP, rows)};
         out.work_bytes = 2 * static_cast<size_t>(n) * size;
         out.slices = static_cast<uint32_t>(size / 64);
         out.empty = nout == 0;

@@ -288,6 +288,39 @@ def test_sparse_erasures_prune_fft_exactly(bits, k, m, size):
             assert np.array_equal(host(sh[i]), want[i]), (lz, i)
 
 
+# Three-pass reconstructs (n = 8192) skip pass-1 row blocks past inputCount
+# (all zero), and pass-3 row blocks without an erased row (pass 2 does not
+# store their rows).  Patterns: the last 1000 data shards erased (inputCount
+# = m + 2000, so blocks from 4096 stay zero), parity only (outputs in the
+# first block), one data shard in the middle, and a spread of both; per call
+# and batched, against the oracle.
+def test_three_pass_skips_empty_row_blocks():
+    k, m, size = 3000, 2000, 128
+    rng = np.random.default_rng(3000)
+    data = [rng.integers(0, 256, size, dtype=np.uint8) for _ in range(k)]
+    o, want = oracle_encode(16, k, m, data)
+    enc = rs.New(k, m, rs.WithLeopardGF16(True))
+    n = k + m
+    sets = [list(range(2000, 3000)), [k, k + 7, n - 1], [1500],
+            sorted(rng.choice(n, 300, replace=False).tolist())]
+    items = []
+    for lost in sets:
+        ref = [None if i in set(lost) else want[i].copy() for i in range(n)]
+        assert o.reconstruct(ref) == 0
+        shards = [None if i in set(lost) else dev(want[i]) for i in range(n)]
+        enc.Reconstruct(shards)
+        for i in lost:
+            assert np.array_equal(host(shards[i]), want[i]), (len(lost), i)
+        w = [dev(want[i]) if i not in set(lost) else torch.zeros(size, dtype=torch.uint8, device="cuda")
+             for i in range(n)]
+        items.append(([None if i in set(lost) else w[i] for i in range(n)], w, lost))
+    status = rs.Batch(0).run([(enc, rs.OP_RECONSTRUCT, sh) for sh, _, _ in items])
+    for (sh, w, lost), st in zip(items, status):
+        assert st == 0
+        for i in lost:
+            assert np.array_equal(host(sh[i]), want[i]), (len(lost), i)
+
+
 # Shapes x sizes that give several column slices and a narrower last slice,
 # run on both schedules: rows of a column slice in LDS (the default: one
 # tile where the rows fit, a quarter-chunk tile for GF(2^16) reconstructs
