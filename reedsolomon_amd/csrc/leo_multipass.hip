@@ -90,8 +90,8 @@ __device__ __forceinline__ void store_tile(const Tile &w, int nrows, RowPtr rp) 
 // loads.
 template <class F, bool INV, bool UNI>
 __device__ __forceinline__ void strided_quads(const Tile &w, const Tile *w2, int ashift, int P, int dist,
-                                              int dj, int nq, const uint16_t *sk, const LeoDev &f) {
-    const int A = 1 << ashift;
+                                              int djs, int nq, const uint16_t *sk, const LeoDev &f) {
+    const int A = 1 << ashift, dj = 1 << djs;
     const int items = (A << log2i(nq)) << w.ushift;
     const int umask = (1 << w.ushift) - 1;
     // With w2 the two tiles are separate items (the second tile's after the
@@ -104,20 +104,31 @@ __device__ __forceinline__ void strided_quads(const Tile &w, const Tile *w2, int
         const int u = (it >> ashift) & umask;
         const int q = it >> (ashift + w.ushift);
         if (q >= nq || u >= w.ucount) continue;
-        const int rj = (q / dj) * 4 * dj;
-        const int j = rj + q % dj;
+        const int rj = (q >> djs) << (djs + 2);
+        const int j = rj + (q & (dj - 1));
         const int iend = rj * P + dist;  // global r + dist
-        unsigned lm01 = sk[iend], lm02 = sk[iend + dist], lm23 = sk[iend + 2 * dist];
-        if (UNI) {
-            lm01 = __builtin_amdgcn_readfirstlane(lm01);
-            lm02 = __builtin_amdgcn_readfirstlane(lm02);
-            lm23 = __builtin_amdgcn_readfirstlane(lm23);
-        }
         const int l0 = (j << ashift) + a, ld = dj << ashift;
-        {
-            const Tile &x = t ? *w2 : w;
-            typename F::E x0 = Lds<F>::load(x, l0, u), x1 = Lds<F>::load(x, l0 + ld, u),
-                          x2 = Lds<F>::load(x, l0 + 2 * ld, u), x3 = Lds<F>::load(x, l0 + 3 * ld, u);
+        const Tile &x = t ? *w2 : w;
+        typename F::E x0 = Lds<F>::load(x, l0, u), x1 = Lds<F>::load(x, l0 + ld, u),
+                      x2 = Lds<F>::load(x, l0 + 2 * ld, u), x3 = Lds<F>::load(x, l0 + 3 * ld, u);
+        if constexpr (UNI) {
+            const unsigned lm01 = uniform_u16(sk, iend), lm02 = uniform_u16(sk, iend + dist),
+                           lm23 = uniform_u16(sk, iend + 2 * dist);
+            uint32_t rec[3][F::kTW];
+            uniform_records<F>(f, lm01, lm02, lm23, rec);
+            if (INV) {
+                ifft2r<F>(x0, x1, lm01, rec[0]);
+                ifft2r<F>(x2, x3, lm23, rec[2]);
+                ifft2r<F>(x0, x2, lm02, rec[1]);
+                ifft2r<F>(x1, x3, lm02, rec[1]);
+            } else {
+                fft2r<F>(x0, x2, lm02, rec[1]);
+                fft2r<F>(x1, x3, lm02, rec[1]);
+                fft2r<F>(x0, x1, lm01, rec[0]);
+                fft2r<F>(x2, x3, lm23, rec[2]);
+            }
+        } else {
+            const unsigned lm01 = sk[iend], lm02 = sk[iend + dist], lm23 = sk[iend + 2 * dist];
             if (INV) {
                 ifft2<F>(x0, x1, lm01, f);
                 ifft2<F>(x2, x3, lm23, f);
@@ -129,11 +140,11 @@ __device__ __forceinline__ void strided_quads(const Tile &w, const Tile *w2, int
                 fft2<F>(x0, x1, lm01, f);
                 fft2<F>(x2, x3, lm23, f);
             }
-            Lds<F>::store(x, l0, u, x0);
-            Lds<F>::store(x, l0 + ld, u, x1);
-            Lds<F>::store(x, l0 + 2 * ld, u, x2);
-            Lds<F>::store(x, l0 + 3 * ld, u, x3);
         }
+        Lds<F>::store(x, l0, u, x0);
+        Lds<F>::store(x, l0 + ld, u, x1);
+        Lds<F>::store(x, l0 + 2 * ld, u, x2);
+        Lds<F>::store(x, l0 + 3 * ld, u, x3);
     }
 }
 
@@ -143,12 +154,12 @@ __device__ __forceinline__ void strided_quads(const Tile &w, const Tile *w2, int
 template <class F, bool INV>
 __device__ __forceinline__ void strided_step4(const Tile &w, const Tile *w2, int ashift, int J, int P, int dist,
                                               int mtrunc, const uint16_t *sk, const LeoDev &f) {
-    const int dj = dist / P;
+    const int dj = dist / P, djs = log2i(dj);  // dist and P are powers of two
     const int gcount = min(J / (4 * dj), (mtrunc + 4 * dist - 1) / (4 * dist));
     if (gcount <= 0) return;
     const int nq = gcount * dj;  // quads per lo
-    if (ashift + w.ushift >= 6) strided_quads<F, INV, true>(w, w2, ashift, P, dist, dj, nq, sk, f);
-    else strided_quads<F, INV, false>(w, w2, ashift, P, dist, dj, nq, sk, f);
+    if (ashift + w.ushift >= 6) strided_quads<F, INV, true>(w, w2, ashift, P, dist, djs, nq, sk, f);
+    else strided_quads<F, INV, false>(w, w2, ashift, P, dist, djs, nq, sk, f);
     __syncthreads();
 }
 

@@ -125,12 +125,32 @@ __device__ __forceinline__ void for_units(const Tile &t, int nrows, Fn fn) {
     }
 }
 
-// One radix-4 step (two layers) at distance 2^dshift over the butterfly
-// groups r < mtrunc of a tile of `rows` consecutive rows whose local row 0 is
-// row `base` of the transform (skews and truncation use the global row), as
-// in k_step4.
-// `need` (one bit per group of 4*dist rows of the whole transform, or null)
-// skips the groups no output row depends on.
+// p[i] for a wave-uniform i, by a scalar load of the aligned dword holding
+// it (the skew tables are whole allocations, so every entry is 2-byte
+// aligned inside a 4-byte aligned buffer).
+__device__ __forceinline__ unsigned uniform_u16(const uint16_t *p, int i) {
+    i = __builtin_amdgcn_readfirstlane(i);
+    const uintptr_t a = reinterpret_cast<uintptr_t>(p + i);
+    const uint32_t d = *as_const(reinterpret_cast<const uint32_t *>(a & ~uintptr_t{3}));
+    return (d >> ((a & 2) << 3)) & 0xFFFFu;
+}
+
+// The three multiply records of a wave-uniform butterfly group, read
+// together (scalar loads, one wait) before any product uses them.  A skew
+// equal to the modulus reads a record that is never used (that product is
+// skipped, leopard.go:741-760).
+template <class F>
+__device__ __forceinline__ void uniform_records(const LeoDev &f, unsigned lm01, unsigned lm02, unsigned lm23,
+                                                uint32_t (&rec)[3][F::kTW]) {
+    ConstU32 *t0 = table<F>(f, lm01), *t1 = table<F>(f, lm02), *t2 = table<F>(f, lm23);
+#pragma unroll
+    for (int o = 0; o < F::kTW; ++o) {
+        rec[0][o] = t0[o];
+        rec[1][o] = t1[o];
+        rec[2][o] = t2[o];
+    }
+}
+
 // Lane `SRC` of each aligned group of G lanes (a DPP quad for G = 4, a DPP
 // row for G = 16) broadcast to the group.
 template <int G, int SRC>
@@ -162,6 +182,12 @@ __device__ __forceinline__ void group_records(const LeoDev &f, unsigned lm01, un
     }
 }
 
+// One radix-4 step (two layers) at distance 2^dshift over the butterfly
+// groups r < mtrunc of a tile of `rows` consecutive rows whose local row 0 is
+// row `base` of the transform (skews and truncation use the global row), as
+// in k_step4.
+// `need` (one bit per group of 4*dist rows of the whole transform, or null)
+// skips the groups no output row depends on.
 // UNI: dist >= 64, so the 64 consecutive q of a wave lie in one group and its
 // three skews are wave-uniform; taking them from the first lane puts the
 // multiply records in scalar registers (scalar loads) instead of 64 per-lane
@@ -180,15 +206,27 @@ __device__ __forceinline__ void step4_groups(const Tile &w, int base, int dshift
         }
         const int i = r + (q & (dist - 1));
         const int iend = base + r + dist;
-        unsigned lm01 = skew[iend + bias], lm02 = skew[iend + dist + bias], lm23 = skew[iend + 2 * dist + bias];
-        if (UNI) {
-            lm01 = __builtin_amdgcn_readfirstlane(lm01);
-            lm02 = __builtin_amdgcn_readfirstlane(lm02);
-            lm23 = __builtin_amdgcn_readfirstlane(lm23);
-        }
         typename F::E x0 = Lds<F>::load(w, i, u), x1 = Lds<F>::load(w, i + dist, u),
                       x2 = Lds<F>::load(w, i + 2 * dist, u), x3 = Lds<F>::load(w, i + 3 * dist, u);
-        if constexpr (SHARED != 0) {
+        if constexpr (UNI) {
+            const unsigned lm01 = uniform_u16(skew, iend + bias), lm02 = uniform_u16(skew, iend + dist + bias),
+                           lm23 = uniform_u16(skew, iend + 2 * dist + bias);
+            uint32_t rec[3][F::kTW];
+            uniform_records<F>(f, lm01, lm02, lm23, rec);
+            if (INV) {
+                ifft2r<F>(x0, x1, lm01, rec[0]);
+                ifft2r<F>(x2, x3, lm23, rec[2]);
+                ifft2r<F>(x0, x2, lm02, rec[1]);
+                ifft2r<F>(x1, x3, lm02, rec[1]);
+            } else {
+                fft2r<F>(x0, x2, lm02, rec[1]);
+                fft2r<F>(x1, x3, lm02, rec[1]);
+                fft2r<F>(x0, x1, lm01, rec[0]);
+                fft2r<F>(x2, x3, lm23, rec[2]);
+            }
+        } else if constexpr (SHARED != 0) {
+            const unsigned lm01 = skew[iend + bias], lm02 = skew[iend + dist + bias],
+                           lm23 = skew[iend + 2 * dist + bias];
             uint32_t rec[3][F::kTW];
             group_records<F, SHARED>(f, lm01, lm02, lm23, rec);
             if (INV) {
@@ -202,16 +240,20 @@ __device__ __forceinline__ void step4_groups(const Tile &w, int base, int dshift
                 fft2r<F>(x0, x1, lm01, rec[0]);
                 fft2r<F>(x2, x3, lm23, rec[2]);
             }
-        } else if (INV) {
-            ifft2<F>(x0, x1, lm01, f);
-            ifft2<F>(x2, x3, lm23, f);
-            ifft2<F>(x0, x2, lm02, f);
-            ifft2<F>(x1, x3, lm02, f);
         } else {
-            fft2<F>(x0, x2, lm02, f);
-            fft2<F>(x1, x3, lm02, f);
-            fft2<F>(x0, x1, lm01, f);
-            fft2<F>(x2, x3, lm23, f);
+            const unsigned lm01 = skew[iend + bias], lm02 = skew[iend + dist + bias],
+                           lm23 = skew[iend + 2 * dist + bias];
+            if (INV) {
+                ifft2<F>(x0, x1, lm01, f);
+                ifft2<F>(x2, x3, lm23, f);
+                ifft2<F>(x0, x2, lm02, f);
+                ifft2<F>(x1, x3, lm02, f);
+            } else {
+                fft2<F>(x0, x2, lm02, f);
+                fft2<F>(x1, x3, lm02, f);
+                fft2<F>(x0, x1, lm01, f);
+                fft2<F>(x2, x3, lm23, f);
+            }
         }
         Lds<F>::store(w, i, u, x0);
         Lds<F>::store(w, i + dist, u, x1);
