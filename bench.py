@@ -72,7 +72,8 @@ def _parse():
                     help="served stream: Poisson rate as a fraction of the measured saturation rate")
     ap.add_argument("--hbm-budget-gib", type=float, default=16.0,
                     help="served stream: shard bytes admitted in flight")
-    ap.add_argument("--only", choices=["rs10_4", "rs10_3", "mixed", "leopard16", "served", "stream", "reference"],
+    ap.add_argument("--only", choices=["rs10_4", "rs10_3", "mixed", "leopard16", "served", "stream", "reference",
+                                       "ceiling"],
                     help="run one secondary section alone and print its JSON (for profiling)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="with --encode-only: also copy 1 GiB HBM->HBM 5 times (counter calibration)")
@@ -185,12 +186,13 @@ def headline(args, torch, rs):
     enc = rs.New(k, m, rs.WithMI355X(0))
     h = enc.handle
     rng = np.random.default_rng(SEED)
-    erase, rec_bytes = [], 0
+    erase, rec_bytes, phys = [], 0, 0
     for s in range(nst):
         e = int(rng.integers(0, m + 1))
         lost = set(rng.choice(n, e, replace=False).tolist()) if e else set()
         erase.append(((ctypes.c_size_t * n)(*[0 if i in lost else S for i in range(n)]), e))
         rec_bytes += n * S if e else 0
+        phys += (k + e) * S if e else 0  # a reconstruct reads k shards and writes e
     filled = (ctypes.c_uint8 * n)()
     bytes_per_step = nst * n * S + rec_bytes
 
@@ -273,6 +275,7 @@ def headline(args, torch, rs):
     out = dict(
         bytes_per_step=bytes_per_step,
         plain_gbps=round(plain, 1), batched_gbps=round(batched, 1),
+        physical_gbps=round((nst * n * S + phys) * args.steps / t_batch / 1e9, 1),
         ms_plain=round(t_plain / args.steps * 1e3, 3), ms_batched=round(t_batch / args.steps * 1e3, 3),
         sample=sample,
         roofline={"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
@@ -284,6 +287,50 @@ def headline(args, torch, rs):
         tuning={"tile_bytes": tile, "blocks_per_cu": bpc},
     )
     del st, benc, brec
+    torch.cuda.empty_cache()
+    return out
+
+
+def ceiling(args, torch, rs):
+    """The device's streaming ceilings for the read/write mixes of the coding
+    passes (reedsolomon_amd/membench: the pass kernels' loads, stores and
+    chunk walk with an XOR in place of the products).  Each launch moves about
+    20 GiB; the best of three grid sizes is reported."""
+    path = os.path.join(os.path.dirname(os.path.abspath(rs.__file__)), "membench", "libmembench.so")
+    lib = ctypes.CDLL(path)
+    lib.membench_stream.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                    ctypes.c_void_p, ctypes.c_uint, ctypes.c_void_p]
+    total = 20 << 30
+    buf = torch.empty(total, dtype=torch.uint8, device="cuda")
+    buf.random_(0, 256)
+    flag = torch.zeros(1, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    side = torch.cuda.Stream()  # a real stream: events on it time the launches
+    stream = side.cuda_stream
+    out = {}
+    for name, nin, nout in (("copy_1r_1w", 1, 1), ("read_4r", 4, 0), ("write_1w", 0, 1), ("mix_4r_1w", 4, 1),
+                            ("mix_10r_4w", 10, 4), ("mix_16r_4w", 16, 4)):
+        region = total // (nin + nout) // 4096 * 4096
+        base = buf.data_ptr()
+        ins = (ctypes.c_void_p * 16)(*[base + j * region for j in range(nin)])
+        outs = (ctypes.c_void_p * 8)(*[base + (nin + i) * region for i in range(nout)])
+        moved = (nin + nout) * region
+        best = None
+        for grid in (1024, 2048, 4096):
+            def launch():
+                rc = lib.membench_stream(nin, nout, ins, outs, region, flag.data_ptr(), grid, stream)
+                if rc:
+                    raise RuntimeError(f"membench_stream error {rc}")
+            for _ in range(2):
+                launch()
+            t = event_time(torch, stream, launch, 5)
+            if best is None or t < best[0]:
+                best = (t, grid)
+        gbps = moved / best[0] / 1e9
+        out[name] = {"gbps": round(gbps, 1), "pct_hbm_peak": round(100 * gbps / HBM_PEAK_GBPS, 2),
+                     "avg_launch_us": round(best[0] * 1e6, 1), "bytes_per_launch": moved, "grid": best[1]}
+        _log(f"bench: HBM ceiling {name}: {gbps:.1f} GB/s")
+    del buf
     torch.cuda.empty_cache()
     return out
 
@@ -903,7 +950,7 @@ def main() -> int:
     torch.cuda.set_device(0)
     if args.only:
         fn = {"rs10_4": rs10_4, "rs10_3": config0, "mixed": mixed, "leopard16": leopard, "served": served,
-              "stream": stream, "reference": reference_shapes}[args.only]
+              "stream": stream, "reference": reference_shapes, "ceiling": ceiling}[args.only]
         sec = fn(args, torch, rs)
         sec.pop("sample", None)
         print(json.dumps({"section": args.only, **sec}), flush=True)
@@ -913,8 +960,9 @@ def main() -> int:
         head.pop("sample", None)
         print(json.dumps({"kernel": "batch_pass<4>", **head}), flush=True)
         return 0
-    mix = leo = srv = r104 = c0 = strm = refs = None
+    mix = leo = srv = r104 = c0 = strm = refs = ceil = None
     if not args.quick:
+        ceil = ceiling(args, torch, rs)
         r104 = rs10_4(args, torch, rs)
         c0 = config0(args, torch, rs)
         refs = reference_shapes(args, torch, rs)
@@ -953,6 +1001,10 @@ def main() -> int:
         "batched_over_plain": round(head["batched_gbps"] / head["plain_gbps"], 2),
         "parity_check_vs_oracle": checks.get("rs16_4"),
         "oracle_checks": checks,
+        "physical_gbps": head["physical_gbps"],
+        "pct_of_measured_ceiling": (round(100.0 * head["batched_gbps"] / ceil["mix_16r_4w"]["gbps"], 1)
+                                    if ceil else None),
+        "hbm_ceiling": ceil,
         "roofline": head["roofline"],
         "plain_encode_launch": head["plain_encode_launch"],
         "tuning": head["tuning"],

@@ -3,7 +3,9 @@
     python reedsolomon_amd/build.py
 
 Every .hip / .cpp file under csrc/ is compiled to an object for gfx950 and
-linked into one shared library exporting the C ABI of include/rsx.h.
+linked into one shared library exporting the C ABI of include/rsx.h.  The
+HBM streaming microbenchmark that bench.py uses for the device's read/write
+ceilings (membench/) is built beside it into membench/libmembench.so.
 """
 from __future__ import annotations
 
@@ -18,6 +20,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "librsx.so")
+MEMBENCH_SRC = os.path.join(HERE, "membench", "membench.hip")
+MEMBENCH_LIB = os.path.join(HERE, "membench", "libmembench.so")
 ARCH = os.environ.get("RSX_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
@@ -55,8 +59,14 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    tmp = MEMBENCH_LIB + ".tmp"
+    cmd = [HIPCC, *COMMON, "-x", "hip", "-shared", MEMBENCH_SRC, "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {MEMBENCH_SRC}:\n{r.stderr}")
+    os.replace(tmp, MEMBENCH_LIB)
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", MEMBENCH_LIB)
     return LIB
 
 

@@ -46,16 +46,21 @@ __global__ __launch_bounds__(kBlock, 2) void batch_pass(const BatchLaunch L) {
     uint32_t *hi = reinterpret_cast<uint32_t *>(lds + L.maxk * M);
     const Tables tab{lo, hi};
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int cached = -1;
+    // The staged tables depend only on the task's table array and first
+    // output (which fix k, mtot and nout), so tasks that share them (every
+    // encode of one codec) stage once per block.
+    const uint32_t *cached_tables = nullptr;
+    int cached_o0 = -1;
     for (uint32_t t = blockIdx.x; t < L.ntiles; t += gridDim.x) {
         const int ti = L.tile_task ? static_cast<int>(L.tile_task[t]) : find_task(L.tasks, L.ntasks, t);
         const BatchTask task = L.tasks[ti];
         const int k = task.k;
-        if (ti != cached) {
+        if (task.tables != cached_tables || task.o0 != cached_o0) {
             __syncthreads();
             stage_tables<M>(lo, hi, task.tables, k, task.mtot, task.o0, task.nout);
             __syncthreads();
-            cached = ti;
+            cached_tables = task.tables;
+            cached_o0 = task.o0;
         }
         const uint8_t *const *in = L.ptrs + task.in_off;
         uint8_t *const *out = (uint8_t *const *)(L.ptrs + task.out_off);
