@@ -295,11 +295,12 @@ def ceiling(args, torch, rs):
     """The device's streaming ceilings for the read/write mixes of the coding
     passes (reedsolomon_amd/membench: the pass kernels' loads, stores and
     chunk walk with an XOR in place of the products).  Each launch moves about
-    20 GiB; the best of three grid sizes is reported."""
+    20 GiB; the best of three grid sizes, each at full occupancy and at two
+    workgroups per CU (the occupancy of the coding passes), is reported."""
     path = os.path.join(os.path.dirname(os.path.abspath(rs.__file__)), "membench", "libmembench.so")
     lib = ctypes.CDLL(path)
     lib.membench_stream.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
-                                    ctypes.c_void_p, ctypes.c_uint, ctypes.c_void_p]
+                                    ctypes.c_void_p, ctypes.c_uint, ctypes.c_uint, ctypes.c_void_p]
     total = 20 << 30
     buf = torch.empty(total, dtype=torch.uint8, device="cuda")
     buf.random_(0, 256)
@@ -316,19 +317,21 @@ def ceiling(args, torch, rs):
         outs = (ctypes.c_void_p * 8)(*[base + (nin + i) * region for i in range(nout)])
         moved = (nin + nout) * region
         best = None
-        for grid in (1024, 2048, 4096):
-            def launch():
-                rc = lib.membench_stream(nin, nout, ins, outs, region, flag.data_ptr(), grid, stream)
-                if rc:
-                    raise RuntimeError(f"membench_stream error {rc}")
-            for _ in range(2):
-                launch()
-            t = event_time(torch, stream, launch, 5)
-            if best is None or t < best[0]:
-                best = (t, grid)
+        for grid in (2048, 4096, 8192):
+            for lds in (0, 64 << 10):
+                def launch():
+                    rc = lib.membench_stream(nin, nout, ins, outs, region, flag.data_ptr(), grid, lds, stream)
+                    if rc:
+                        raise RuntimeError(f"membench_stream error {rc}")
+                for _ in range(2):
+                    launch()
+                t = event_time(torch, stream, launch, 5)
+                if best is None or t < best[0]:
+                    best = (t, grid, lds)
         gbps = moved / best[0] / 1e9
         out[name] = {"gbps": round(gbps, 1), "pct_hbm_peak": round(100 * gbps / HBM_PEAK_GBPS, 2),
-                     "avg_launch_us": round(best[0] * 1e6, 1), "bytes_per_launch": moved, "grid": best[1]}
+                     "avg_launch_us": round(best[0] * 1e6, 1), "bytes_per_launch": moved, "grid": best[1],
+                     "workgroups_per_cu": 2 if best[2] else None}
         _log(f"bench: HBM ceiling {name}: {gbps:.1f} GB/s")
     del buf
     torch.cuda.empty_cache()

@@ -33,6 +33,9 @@ struct StreamArgs {
 
 template <int NIN, int NOUT>
 __global__ __launch_bounds__(kBlock) void hbm_stream(const StreamArgs a) {
+    // Dynamic LDS (unused beyond this word) only limits workgroups per CU.
+    extern __shared__ uint32_t lds_reserve[];
+    if (threadIdx.x == 0) lds_reserve[0] = 0;
     const int lane = threadIdx.x & 63;
     const uint64_t nchunk = a.bytes / (kR * 1024);
     const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (kBlock / 64);
@@ -67,19 +70,22 @@ __global__ __launch_bounds__(kBlock) void hbm_stream(const StreamArgs a) {
 extern "C" {
 
 // Launches one pass of the NIN-in / NOUT-out stream on `stream` with `grid`
-// workgroups of 256 threads.  Supported mixes: (1,1) copy, (4,0) read, (0,1)
-// write, (4,1), (10,4), (16,4).  Returns a hipError_t.
+// workgroups of 256 threads, each reserving `lds` bytes of LDS (0 or up to
+// 64 KiB: fewer workgroups per CU).  Supported mixes: (1,1) copy, (4,0)
+// read, (0,1) write, (4,1), (10,4), (16,4).  Returns a hipError_t.
 int membench_stream(int nin, int nout, const void *const *in, void *const *out, size_t bytes, uint32_t *flag,
-                    unsigned grid, void *stream) {
+                    unsigned grid, unsigned lds, void *stream) {
     StreamArgs a{};
-    if (nin < 0 || nin > 16 || nout < 0 || nout > 8 || bytes % (kR * 1024)) return hipErrorInvalidValue;
+    if (nin < 0 || nin > 16 || nout < 0 || nout > 8 || bytes % (kR * 1024) || lds > (64u << 10))
+        return hipErrorInvalidValue;
+    lds = lds < 4 ? 4 : lds;
     for (int j = 0; j < nin; ++j) a.in[j] = static_cast<const uint8_t *>(in[j]);
     for (int i = 0; i < nout; ++i) a.out[i] = static_cast<uint8_t *>(out[i]);
     a.bytes = bytes;
     a.flag = flag;
     hipStream_t s = static_cast<hipStream_t>(stream);
 #define RSX_MIX(I, O) \
-    if (nin == I && nout == O) { hipLaunchKernelGGL((hbm_stream<I, O>), dim3(grid), dim3(kBlock), 0, s, a); return hipGetLastError(); }
+    if (nin == I && nout == O) { hipLaunchKernelGGL((hbm_stream<I, O>), dim3(grid), dim3(kBlock), lds, s, a); return hipGetLastError(); }
     RSX_MIX(1, 1) RSX_MIX(4, 0) RSX_MIX(0, 1) RSX_MIX(4, 1) RSX_MIX(10, 4) RSX_MIX(16, 4)
 #undef RSX_MIX
     return hipErrorInvalidValue;
