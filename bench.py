@@ -787,17 +787,19 @@ def served(args, torch, rs):
         return span.value * 1e-9, lat
 
     replay(np.zeros(n))  # warm: decode matrices and error locators cached on the device
-    b_before = srv.stats()
+    b_before = srv.timing()
     t_sat, lat_sat = replay(np.zeros(n))
-    b_after = srv.stats()
+    b_after = srv.timing()
     rate = args.load * n / t_sat
     arrivals = np.cumsum(rng.exponential(1e9 / rate, n))
     t_poi, lat_poi = replay(arrivals)
-    b_end = srv.stats()
+    b_end = srv.timing()
     srv.close()
     leo_lat = lat_poi[is_leo]
     cls_lat = lat_poi[~is_leo]
-    _log(f"bench: served saturation {total / t_sat / 1e9:.1f} GB/s; Poisson {rate:.0f} req/s "
+    _log(f"bench: served saturation {total / t_sat / 1e9:.1f} GB/s (makespan {t_sat * 1e3:.2f} ms, host plan "
+         f"{(b_after[3] - b_before[3]) * 1e-6:.2f} ms, device {(b_after[4] - b_before[4]) * 1e-6:.2f} ms, "
+         f"{b_after[0] - b_before[0]} batches); Poisson {rate:.0f} req/s "
          f"{total / t_poi / 1e9:.1f} GB/s p50 {np.percentile(lat_poi, 50):.3f} ms "
          f"p99 {np.percentile(lat_poi, 99):.3f} ms")
     out = dict(
@@ -809,7 +811,9 @@ def served(args, torch, rs):
                     "p50_ms": round(float(np.percentile(lat_sat, 50)), 3),
                     "p99_ms": round(float(np.percentile(lat_sat, 99)), 3),
                     "batches": b_after[0] - b_before[0],
-                    "requests_batched": b_after[1] - b_before[1]},
+                    "requests_batched": b_after[1] - b_before[1],
+                    "host_plan_ms": round((b_after[3] - b_before[3]) * 1e-6, 3),
+                    "device_ms": round((b_after[4] - b_before[4]) * 1e-6, 3)},
         poisson={"load": args.load, "rate_req_per_s": round(rate, 1),
                  "gbps": round(total / t_poi / 1e9, 1), "makespan_ms": round(t_poi * 1e3, 2),
                  "p50_ms": round(float(np.percentile(lat_poi, 50)), 3),
@@ -819,7 +823,9 @@ def served(args, torch, rs):
                  "classic_p99_ms": round(float(np.percentile(cls_lat, 99)), 3),
                  "leopard_p50_ms": round(float(np.percentile(leo_lat, 50)), 3) if len(leo_lat) else None,
                  "leopard_p99_ms": round(float(np.percentile(leo_lat, 99)), 3) if len(leo_lat) else None,
-                 "batches": b_end[0] - b_after[0], "requests_batched": b_end[1] - b_after[1]},
+                 "batches": b_end[0] - b_after[0], "requests_batched": b_end[1] - b_after[1],
+                 "host_plan_ms": round((b_end[3] - b_after[3]) * 1e-6, 3),
+                 "device_ms": round((b_end[4] - b_after[4]) * 1e-6, 3)},
         max_inflight_bytes=b_end[2])
     del st
     torch.cuda.empty_cache()

@@ -269,9 +269,13 @@ int rsx_server_drain(rsx_server *s);
  * fills status / verify_ok / latency_ns per request and the makespan. */
 int rsx_server_run_trace(rsx_server *s, const rsx_request *reqs, const uint64_t *arrival_ns, int n,
                          int *status, int *verify_ok, uint64_t *latency_ns, uint64_t *makespan_ns);
-/* Counters: batched launches, requests in them, highest shard bytes in flight. */
+/* Counters: batched launches, requests in them, highest shard bytes in
+ * flight, host time spent planning and launching batches, and device time of
+ * the batches (sum over batches, first kernel start to last kernel end;
+ * batches that overlap count their overlap twice).  plan_ns / device_ns may
+ * be NULL. */
 int rsx_server_stats(const rsx_server *s, uint64_t *batches, uint64_t *requests,
-                     uint64_t *max_inflight_bytes);
+                     uint64_t *max_inflight_bytes, uint64_t *plan_ns, uint64_t *device_ns);
 
 /* ---- Streaming (streaming.go) ---------------------------------------------
  * StreamEncoder of NewStream (streaming.go:147): shards are read and written

@@ -663,7 +663,7 @@ lib.rsx_server_wait.argtypes = [_P, ctypes.c_uint64, _I, _I, _U64]
 lib.rsx_server_drain.argtypes = [_P]
 lib.rsx_server_run_trace.argtypes = [_P, ctypes.POINTER(_Request), _U64, ctypes.c_int, _I, _I, _U64,
                                      _U64]
-lib.rsx_server_stats.argtypes = [_P, _U64, _U64, _U64]
+lib.rsx_server_stats.argtypes = [_P, _U64, _U64, _U64, _U64, _U64]
 
 
 def _fill_outputs(op, k, shards, bufs):
@@ -741,9 +741,14 @@ class Server:
 
     def stats(self):
         """(batched launches, requests in them, max shard bytes in flight)."""
-        a, b, c = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
-        _check(lib.rsx_server_stats(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
-        return a.value, b.value, c.value
+        return self.timing()[:3]
+
+    def timing(self):
+        """(batched launches, requests in them, max shard bytes in flight,
+        host ns planning and launching batches, device ns of the batches)."""
+        v = [ctypes.c_uint64() for _ in range(5)]
+        _check(lib.rsx_server_stats(self._h, *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
 
 
 

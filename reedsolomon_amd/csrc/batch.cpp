@@ -25,6 +25,45 @@ int decode_plan(rsx_codec *c, const size_t *lens, bool data_only, const uint8_t 
 
 using namespace rsx;
 
+namespace {
+
+// Page-locked host staging for a plan's uploads: copies from it are
+// asynchronous, so planning does not wait for them.  Grows, never shrinks.
+struct PinnedBuffer {
+    uint8_t *ptr = nullptr;
+    size_t bytes = 0;
+    PinnedBuffer() = default;
+    PinnedBuffer(const PinnedBuffer &) = delete;
+    PinnedBuffer &operator=(const PinnedBuffer &) = delete;
+    ~PinnedBuffer() {
+        if (ptr) (void)hipHostFree(ptr);
+    }
+    int reserve(size_t n) {
+        if (n <= bytes) return RSX_OK;
+        n = std::max(n, bytes + bytes / 2);  // few regrowths (each frees: a device-wide wait)
+        if (ptr) (void)hipHostFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+        if (hipHostMalloc(reinterpret_cast<void **>(&ptr), n) != hipSuccess) {
+            (void)hipGetLastError();
+            ptr = nullptr;
+            return fail(RSX_ERR_NOMEM, "pinned staging of " + std::to_string(n) + " bytes failed");
+        }
+        bytes = n;
+        return RSX_OK;
+    }
+};
+
+size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
+// Grows a plan buffer by at least half its size, so plans of varying size
+// settle after a few regrowths (each one waits for the whole device).
+int reserve_grow(DeviceBuffer &b, size_t n) {
+    return b.reserve(n <= b.bytes ? n : std::max(n, b.bytes + b.bytes / 2));
+}
+
+}  // namespace
+
 struct rsx_batch {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -61,7 +100,14 @@ struct rsx_batch {
     };
     std::vector<Task> tasks[9];          // by output-count class 1..8
     std::vector<const uint8_t *> pool;   // pointer pool
-    DeviceBuffer d_tasks, d_ptrs, d_tables, d_tile_task;
+    // Everything a plan uploads goes through two pinned staging buffers and
+    // two copies on `stream`: d_blob (coefficient tables, the pointer pool,
+    // Leopard aux arrays and task lists) and d_launch (the task list and the
+    // tile -> task table of the chosen tile size).
+    DeviceBuffer d_blob, d_launch;
+    PinnedBuffer h_blob, h_launch;
+    const uint8_t *const *dev_ptrs = nullptr;  // the pool in d_blob
+    hipEvent_t uploaded = nullptr;             // after the plan's copies on `stream`
     std::vector<BatchLaunch> launches;   // one per non-empty class
     std::vector<int> classes;
     hipGraph_t graph = nullptr;
@@ -84,7 +130,7 @@ struct rsx_batch {
     // Leopard items by launch kind: GF(2^8), GF(2^16), GF(2^16) quarter-chunk tiles.
     std::vector<rsx::LeoItemPlan> leo[3];
     std::vector<rsx::LeoBatchTask> leo_tasks[3];
-    DeviceBuffer d_leo_tasks[3], d_leo_aux;
+    const rsx::LeoBatchTask *dev_leo_tasks[3] = {};  // in d_blob
     size_t leo_lds[3] = {0, 0, 0};
     size_t leo_deriv[3] = {0, 0, 0};  // largest register bytes (LeoItemPlan::regs) of the tasks
     uint32_t leo_slices[3] = {0, 0, 0};
@@ -93,7 +139,8 @@ struct rsx_batch {
     // per pass) over all such items, each with its two work arrays in d_mp_work.
     std::vector<rsx::LeoItemPlan> leo_mp;
     std::vector<rsx::LeoPassTask> mp_tasks;
-    DeviceBuffer d_mp_tasks, d_mp_work;
+    const rsx::LeoPassTask *dev_mp_tasks = nullptr;  // in d_blob
+    DeviceBuffer d_mp_work;
     size_t mp_work_bytes = 0;
     uint32_t mp_blocks[3] = {0, 0, 0};
     int mp_max_p = 0;
@@ -109,7 +156,7 @@ struct rsx_batch {
     std::vector<rsx::LeoItemPlan> leo_emp;
     std::vector<rsx::LeoEncPassTask> emp_tasks;
     std::vector<EncLaunch> emp_launches;
-    DeviceBuffer d_emp_tasks;
+    const rsx::LeoEncPassTask *dev_emp_tasks = nullptr;  // in d_blob
 };
 
 namespace {
@@ -177,22 +224,25 @@ int build_launches(rsx_batch *b, uint32_t tile) {
         b->launches.push_back(L);
         b->classes.push_back(mc);
     }
-    int err = b->d_tasks.reserve(std::max<size_t>(all.size(), 1) * sizeof(BatchTask));
+    // Task list then tile table, one pinned copy.  An earlier copy from
+    // h_launch has completed: plans and autotune trials wait for the stream
+    // before they rebuild.
+    const size_t task_bytes = align256(std::max<size_t>(all.size(), 1) * sizeof(BatchTask));
+    const size_t tt_bytes = use_table ? tile_task.size() * sizeof(uint32_t) : 0;
+    int err = reserve_grow(b->d_launch, task_bytes + std::max<size_t>(tt_bytes, 4));
+    if (!err) err = b->h_launch.reserve(task_bytes + std::max<size_t>(tt_bytes, 4));
     if (err) return err;
-    RSX_HIP(hipMemcpyAsync(b->d_tasks.ptr, all.data(), all.size() * sizeof(BatchTask),
-                           hipMemcpyHostToDevice, b->stream));
-    if (use_table) {
-        if ((err = b->d_tile_task.reserve(std::max<size_t>(tile_task.size(), 1) * sizeof(uint32_t)))) return err;
-        RSX_HIP(hipMemcpyAsync(b->d_tile_task.ptr, tile_task.data(), tile_task.size() * sizeof(uint32_t),
-                               hipMemcpyHostToDevice, b->stream));
-    }
-    RSX_HIP(hipStreamSynchronize(b->stream));
+    std::memcpy(b->h_launch.ptr, all.data(), all.size() * sizeof(BatchTask));
+    if (tt_bytes) std::memcpy(b->h_launch.ptr + task_bytes, tile_task.data(), tt_bytes);
+    RSX_HIP(hipMemcpyAsync(b->d_launch.ptr, b->h_launch.ptr, task_bytes + tt_bytes, hipMemcpyHostToDevice, b->stream));
+    const BatchTask *dev_tasks = b->d_launch.as<BatchTask>();
+    const uint32_t *dev_tt = reinterpret_cast<const uint32_t *>(b->d_launch.as<uint8_t>() + task_bytes);
     for (size_t i = 0; i < b->launches.size(); ++i) {
         BatchLaunch &L = b->launches[i];
-        L.tasks = b->d_tasks.as<BatchTask>() + reinterpret_cast<size_t>(L.tasks);
-        L.ptrs = b->d_ptrs.as<const uint8_t *>();
+        L.tasks = dev_tasks + reinterpret_cast<size_t>(L.tasks);
+        L.ptrs = b->dev_ptrs;
         L.flags = b->d_flags.as<uint32_t>();
-        L.tile_task = use_table ? b->d_tile_task.as<uint32_t>() + tt_off[i] : nullptr;
+        L.tile_task = use_table ? dev_tt + tt_off[i] : nullptr;
     }
     return RSX_OK;
 }
@@ -218,15 +268,15 @@ int enqueue(rsx_batch *b, hipStream_t s) {
     for (int fi = 0; fi < 3; ++fi) {
         if (b->leo_tasks[fi].empty()) continue;
         const unsigned g = std::min<unsigned>(static_cast<unsigned>(std::max(1, b->cus * 8)), b->leo_slices[fi]);
-        RSX_HIP(leo_batch_launch(*b->leo[fi][0].dev, b->d_leo_tasks[fi].as<LeoBatchTask>(),
+        RSX_HIP(leo_batch_launch(*b->leo[fi][0].dev, b->dev_leo_tasks[fi],
                                  static_cast<int>(b->leo_tasks[fi].size()), b->leo_slices[fi], b->leo_lds[fi], g,
                                  b->leo_deriv[fi], fi == 2, ls));
     }
     if (!b->mp_tasks.empty())
-        RSX_HIP(leo_decode_multipass_batch(*b->leo_mp[0].dev, b->d_mp_tasks.as<LeoPassTask>(),
+        RSX_HIP(leo_decode_multipass_batch(*b->leo_mp[0].dev, b->dev_mp_tasks,
                                            static_cast<int>(b->mp_tasks.size()), b->mp_blocks, b->mp_max_p, ls));
     for (const auto &L : b->emp_launches)
-        RSX_HIP(leo_encode_multipass_batch(*b->leo_emp[0].dev, b->d_emp_tasks.as<LeoEncPassTask>() + L.off, L.ntasks,
+        RSX_HIP(leo_encode_multipass_batch(*b->leo_emp[0].dev, b->dev_emp_tasks + L.off, L.ntasks,
                                            L.blocks, L.pass, L.max_p, ls));
     if (fork) RSX_HIP(hipEventRecord(b->join, b->side));
     int next_side = 0;
@@ -333,6 +383,7 @@ int rsx_batch_create(int device, rsx_batch **out) {
         hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&b->join, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->uploaded, hipEventDisableTiming) != hipSuccess ||
         hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) {
         rsx_batch_free(b);
         return fail(RSX_ERR_DEVICE, "stream creation failed");
@@ -356,6 +407,7 @@ void rsx_batch_free(rsx_batch *b) {
         if (b->cls[j]) (void)hipStreamSynchronize(b->cls[j]);
     destroy_graph(b);
     if (b->h_flags) (void)hipHostFree(b->h_flags);
+    if (b->uploaded) (void)hipEventDestroy(b->uploaded);
     if (b->fork) (void)hipEventDestroy(b->fork);
     if (b->join) (void)hipEventDestroy(b->join);
     if (b->side) (void)hipStreamDestroy(b->side);
@@ -604,7 +656,7 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
     }
     for (int it = 0; it < n; ++it) b->has_verify |= b->is_verify[it] == 1;
     if (b->has_verify) {
-        int e3 = b->d_flags.reserve(sizeof(uint32_t) * std::max(n, 1));
+        int e3 = reserve_grow(b->d_flags, sizeof(uint32_t) * std::max(n, 1));
         if (e3) return e3;
         if (b->h_flags_cap < static_cast<size_t>(n)) {
             if (b->h_flags) (void)hipHostFree(b->h_flags);
@@ -613,29 +665,56 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
             b->h_flags_cap = static_cast<size_t>(n);
         }
     }
-    // Batch-owned copy of every table the plan reads.
-    int err = b->d_tables.reserve(std::max<size_t>(htab.size(), 1) * 4);
-    if (err) return err;
-    if (!htab.empty())
-        RSX_HIP(hipMemcpyAsync(b->d_tables.ptr, htab.data(), htab.size() * 4, hipMemcpyHostToDevice, b->stream));
-    for (auto &p : pend) b->tasks[p.mc][p.idx].tables = b->d_tables.as<uint32_t>() + p.toff;
-    if ((err = b->d_ptrs.reserve(std::max<size_t>(b->pool.size(), 1) * sizeof(void *)))) return err;
-    RSX_HIP(hipMemcpyAsync(b->d_ptrs.ptr, b->pool.data(), b->pool.size() * sizeof(void *),
-                           hipMemcpyHostToDevice, b->stream));
-    // Leopard tasks: resolve pool / aux offsets to device addresses.
-    if (!laux.empty()) {
-        if ((err = b->d_leo_aux.reserve(laux.size()))) return err;
-        RSX_HIP(hipMemcpyAsync(b->d_leo_aux.ptr, laux.data(), laux.size(), hipMemcpyHostToDevice, b->stream));
+    // Layout of d_blob: tables, pointer pool, Leopard aux arrays, Leopard
+    // task lists (sizes are known here; device pointers inside the task
+    // lists are filled in against the blob's address below).
+    size_t emp_count = 0;
+    int max_groups = 0, max_p_emp = 0;
+    for (const LeoItemPlan &lp : b->leo_emp) {
+        emp_count += 2 * static_cast<size_t>(lp.groups) + 1;  // passes 1 and 2 per group, pass 3
+        max_groups = std::max(max_groups, lp.groups);
+        max_p_emp = std::max(max_p_emp, lp.emp.P);
     }
+    size_t off = 0;
+    const size_t o_tab = off;
+    off += align256(htab.size() * 4);
+    const size_t o_ptrs = off;
+    off += align256(b->pool.size() * sizeof(void *));
+    const size_t o_aux = off;
+    off += align256(laux.size());
+    size_t o_leo[3];
+    for (int fi = 0; fi < 3; ++fi) {
+        o_leo[fi] = off;
+        off += align256(b->leo[fi].size() * sizeof(LeoBatchTask));
+    }
+    const size_t o_emp = off;
+    off += align256(emp_count * sizeof(LeoEncPassTask));
+    const size_t o_mp = off;
+    off += align256(b->leo_mp.size() * sizeof(LeoPassTask));
+    const size_t blob_bytes = std::max<size_t>(off, 256);
+    int err = reserve_grow(b->d_blob, blob_bytes);
+    if (!err) err = b->h_blob.reserve(blob_bytes);
+    if (err) return err;
+    uint8_t *dev = b->d_blob.as<uint8_t>();
+    uint8_t *host = b->h_blob.ptr;
+    const uint32_t *dev_tables = reinterpret_cast<const uint32_t *>(dev + o_tab);
+    uint8_t *const *dev_pool = reinterpret_cast<uint8_t *const *>(dev + o_ptrs);
+    uint8_t *dev_aux = dev + o_aux;
+    b->dev_ptrs = reinterpret_cast<const uint8_t *const *>(dev_pool);
+    std::memcpy(host + o_tab, htab.data(), htab.size() * 4);
+    std::memcpy(host + o_ptrs, b->pool.data(), b->pool.size() * sizeof(void *));
+    if (!laux.empty()) std::memcpy(host + o_aux, laux.data(), laux.size());
+    for (auto &p : pend) b->tasks[p.mc][p.idx].tables = dev_tables + p.toff;
+    // Leopard tasks: resolve pool / aux offsets to device addresses.
     for (int fi = 0; fi < 3; ++fi) {
         uint32_t slice0 = 0;
         for (const LeoItemPlan &lp : b->leo[fi]) {
             LeoBatchTask t = lp.task;
             t.slice0 = slice0;
-            uint8_t *const *pp = b->d_ptrs.as<uint8_t *>() + lp.ptr_off;
+            uint8_t *const *pp = dev_pool + lp.ptr_off;
             if (t.op == 0) {
                 t.enc.data = pp;
-                t.enc.parity = b->d_ptrs.as<uint8_t *>() + lp.parity_pool;
+                t.enc.parity = dev_pool + lp.parity_pool;
                 if (lp.verify_item >= 0) {
                     t.enc.flag = b->d_flags.as<uint32_t>() + lp.verify_item;
                     t.enc.flag_value = 1;
@@ -643,10 +722,10 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
             } else {
                 t.dec.rows = pp;
                 t.dec.out = pp + t.dec.n;
-                t.dec.errlocs = reinterpret_cast<const uint16_t *>(b->d_leo_aux.as<uint8_t>() + lp.errl_off);
-                t.dec.out_rows = reinterpret_cast<const int *>(b->d_leo_aux.as<uint8_t>() + lp.rows_off);
-                t.dec.out_logs = reinterpret_cast<const uint16_t *>(b->d_leo_aux.as<uint8_t>() + lp.logs_off);
-                t.dec.need = reinterpret_cast<const uint32_t *>(b->d_leo_aux.as<uint8_t>() + lp.need_off);
+                t.dec.errlocs = reinterpret_cast<const uint16_t *>(dev_aux + lp.errl_off);
+                t.dec.out_rows = reinterpret_cast<const int *>(dev_aux + lp.rows_off);
+                t.dec.out_logs = reinterpret_cast<const uint16_t *>(dev_aux + lp.logs_off);
+                t.dec.need = reinterpret_cast<const uint32_t *>(dev_aux + lp.need_off);
             }
             if (static_cast<uint64_t>(slice0) + lp.slices > 0xFFFFFFFFull)
                 return fail(RSX_ERR_INVALID_INPUT, "batch too large for one plan");
@@ -656,30 +735,22 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
             b->leo_tasks[fi].push_back(t);
         }
         b->leo_slices[fi] = slice0;
-        if (b->leo_tasks[fi].empty()) continue;
-        const size_t bytes = b->leo_tasks[fi].size() * sizeof(LeoBatchTask);
-        if ((err = b->d_leo_tasks[fi].reserve(bytes))) return err;
-        RSX_HIP(hipMemcpyAsync(b->d_leo_tasks[fi].ptr, b->leo_tasks[fi].data(), bytes, hipMemcpyHostToDevice,
-                               b->stream));
+        b->dev_leo_tasks[fi] = reinterpret_cast<const LeoBatchTask *>(dev + o_leo[fi]);
+        if (!b->leo_tasks[fi].empty())
+            std::memcpy(host + o_leo[fi], b->leo_tasks[fi].data(), b->leo_tasks[fi].size() * sizeof(LeoBatchTask));
     }
     if (!b->leo_mp.empty() || !b->leo_emp.empty())
         if ((err = b->d_mp_work.reserve(b->mp_work_bytes))) return err;
     if (!b->leo_emp.empty()) {
-        int max_groups = 0, max_p = 0;
-        for (const LeoItemPlan &lp : b->leo_emp) {
-            max_groups = std::max(max_groups, lp.groups);
-            max_p = std::max(max_p, lp.emp.P);
-        }
         // Task arrays in launch order: (pass 1, pass 2) for g = 0, 1, ..., then pass 3.
         auto add_launch = [&](int pass, int g) -> int {
-            rsx_batch::EncLaunch L{b->emp_tasks.size(), 0, pass, max_p, 0};
+            rsx_batch::EncLaunch L{b->emp_tasks.size(), 0, pass, max_p_emp, 0};
             for (const LeoItemPlan &lp : b->leo_emp) {
                 if (pass != 3 && g >= lp.groups) continue;
                 LeoEncPassTask t{};
                 t.a = lp.emp;
-                uint8_t *const *pp = b->d_ptrs.as<uint8_t *>();
-                t.a.rows = pp + lp.ptr_off + static_cast<size_t>(g) * t.a.m;
-                t.a.parity = pp + lp.parity_pool;
+                t.a.rows = dev_pool + lp.ptr_off + static_cast<size_t>(g) * t.a.m;
+                t.a.parity = dev_pool + lp.parity_pool;
                 t.a.work = b->d_mp_work.as<uint8_t>() + reinterpret_cast<size_t>(lp.emp.work);
                 t.a.work2 = t.a.work + static_cast<size_t>(t.a.m) * static_cast<size_t>(t.a.slices) * 64;
                 t.a.col0 = 0;
@@ -704,22 +775,21 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
         for (int g = 0; g < max_groups; ++g)
             if (add_launch(1, g) || add_launch(2, g)) return fail(RSX_ERR_INVALID_INPUT, "batch too large for one plan");
         if (add_launch(3, 0)) return fail(RSX_ERR_INVALID_INPUT, "batch too large for one plan");
-        const size_t bytes = b->emp_tasks.size() * sizeof(LeoEncPassTask);
-        if ((err = b->d_emp_tasks.reserve(bytes))) return err;
-        RSX_HIP(hipMemcpyAsync(b->d_emp_tasks.ptr, b->emp_tasks.data(), bytes, hipMemcpyHostToDevice, b->stream));
+        if (b->emp_tasks.size() != emp_count) return fail(RSX_ERR_INVALID_INPUT, "encoder pass task count");
+        std::memcpy(host + o_emp, b->emp_tasks.data(), emp_count * sizeof(LeoEncPassTask));
     }
+    b->dev_emp_tasks = reinterpret_cast<const LeoEncPassTask *>(dev + o_emp);
     if (!b->leo_mp.empty()) {
-        uint8_t *aux = b->d_leo_aux.as<uint8_t>();
         for (const LeoItemPlan &lp : b->leo_mp) {
             LeoPassTask t{};
             t.a = lp.mp;
-            uint8_t *const *pp = b->d_ptrs.as<uint8_t *>() + lp.ptr_off;
+            uint8_t *const *pp = dev_pool + lp.ptr_off;
             t.a.rows = pp;
             t.a.out = pp + t.a.n;
-            t.a.errlocs = reinterpret_cast<const uint16_t *>(aux + lp.errl_off);
-            t.a.out_index = reinterpret_cast<const int *>(aux + lp.outidx_off);
-            t.a.out_logs = reinterpret_cast<const uint16_t *>(aux + lp.logs_off);
-            t.a.need = reinterpret_cast<const uint32_t *>(aux + lp.need_off);
+            t.a.errlocs = reinterpret_cast<const uint16_t *>(dev_aux + lp.errl_off);
+            t.a.out_index = reinterpret_cast<const int *>(dev_aux + lp.outidx_off);
+            t.a.out_logs = reinterpret_cast<const uint16_t *>(dev_aux + lp.logs_off);
+            t.a.need = reinterpret_cast<const uint32_t *>(dev_aux + lp.need_off);
             t.a.work = b->d_mp_work.as<uint8_t>() + reinterpret_cast<size_t>(lp.mp.work);
             t.a.work2 = t.a.work + static_cast<size_t>(t.a.n) * static_cast<size_t>(t.a.slices) * 64;
             uint32_t nb[3];
@@ -733,17 +803,20 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
             b->mp_max_p = std::max(b->mp_max_p, t.a.P);
             b->mp_tasks.push_back(t);
         }
-        const size_t bytes = b->mp_tasks.size() * sizeof(LeoPassTask);
-        if ((err = b->d_mp_tasks.reserve(bytes))) return err;
-        RSX_HIP(hipMemcpyAsync(b->d_mp_tasks.ptr, b->mp_tasks.data(), bytes, hipMemcpyHostToDevice, b->stream));
+        std::memcpy(host + o_mp, b->mp_tasks.data(), b->mp_tasks.size() * sizeof(LeoPassTask));
     }
-    RSX_HIP(hipStreamSynchronize(b->stream));
+    b->dev_mp_tasks = reinterpret_cast<const LeoPassTask *>(dev + o_mp);
+    if (off) RSX_HIP(hipMemcpyAsync(dev, host, off, hipMemcpyHostToDevice, b->stream));
     if (b->tuned) {
         if ((err = build_launches(b, b->tile))) return err;
     } else {
         if ((err = autotune(b))) return err;
     }
-    if (b->use_graph && (err = capture(b))) return err;
+    RSX_HIP(hipEventRecord(b->uploaded, b->stream));
+    if (b->use_graph) {
+        RSX_HIP(hipStreamSynchronize(b->stream));  // capture starts on an idle stream
+        if ((err = capture(b))) return err;
+    }
     b->planned = true;
     return RSX_OK;
 }
@@ -752,6 +825,8 @@ int rsx_batch_launch_on(rsx_batch *b, void *hip_stream) {
     if (!b->planned) return fail(RSX_ERR_INVALID_INPUT, "batch has no plan");
     (void)hipSetDevice(b->device);
     hipStream_t s = hip_stream ? static_cast<hipStream_t>(hip_stream) : b->stream;
+    // The plan's copies run on b->stream; another stream waits for them.
+    if (s != b->stream) RSX_HIP(hipStreamWaitEvent(s, b->uploaded, 0));
     if (b->use_graph) RSX_HIP(hipGraphLaunch(b->exec, s));
     else return enqueue(b, s);
     return RSX_OK;

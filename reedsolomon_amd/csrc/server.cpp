@@ -42,7 +42,8 @@ struct Req {
 
 struct Slot {
     rsx_batch *b = nullptr;
-    hipEvent_t ev = nullptr;
+    hipEvent_t ev = nullptr;     // after the batch's kernels (completion, timing)
+    hipEvent_t ev0 = nullptr;    // before them (timing)
     std::vector<std::pair<uint64_t, int>> tickets;  // (ticket, item index in the plan)
     bool busy = false;
 };
@@ -63,6 +64,7 @@ struct rsx_server {
     size_t inflight = 0, max_inflight = 0;
     size_t outstanding = 0;  // submitted and not completed
     uint64_t batches = 0, batched_requests = 0;
+    uint64_t plan_ns = 0, device_ns = 0;  // host planning + launch; device time of the batches
     Slot slots[kSlots];
     bool stop = false;
     std::thread sched, worker;
@@ -70,7 +72,8 @@ struct rsx_server {
 
 namespace {
 
-// Caller holds s->mu.
+// Caller holds s->mu and notifies done_cv / cv once it has completed
+// everything at hand (one wake-up per retired batch, not per request).
 void complete(rsx_server *s, uint64_t t, int status, int verify_ok) {
     Req &q = s->reqs[t];
     q.done = true;
@@ -80,6 +83,9 @@ void complete(rsx_server *s, uint64_t t, int status, int verify_ok) {
         std::chrono::duration_cast<std::chrono::nanoseconds>(Clock::now() - q.arrival).count());
     s->inflight -= q.bytes;
     --s->outstanding;
+}
+
+void notify_completed(rsx_server *s) {
     s->done_cv.notify_all();
     s->cv.notify_one();  // budget freed: the scheduler may admit more
 }
@@ -115,6 +121,7 @@ void worker_loop(rsx_server *s) {
         if (rc == RSX_OK) rc = rsx_synchronize(c);
         lock.lock();
         complete(s, t, rc, ok);
+        notify_completed(s);
     }
 }
 
@@ -125,6 +132,9 @@ void scheduler_loop(rsx_server *s) {
         // (1) retire finished slots
         for (Slot &sl : s->slots) {
             if (!sl.busy || hipEventQuery(sl.ev) != hipSuccess) continue;
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, sl.ev0, sl.ev) == hipSuccess) s->device_ns += static_cast<uint64_t>(ms * 1e6);
+            else (void)hipGetLastError();
             for (const auto &ti : sl.tickets) {
                 int ok = 0;
                 if (s->reqs[ti.first].r.op == RSX_OP_VERIFY) rsx_batch_verify_result(sl.b, ti.second, &ok);
@@ -132,6 +142,7 @@ void scheduler_loop(rsx_server *s) {
             }
             sl.tickets.clear();
             sl.busy = false;
+            notify_completed(s);
         }
         // (2) admit into a free slot
         Slot *free_slot = nullptr;
@@ -166,17 +177,26 @@ void scheduler_loop(rsx_server *s) {
             for (auto &it : items) nflags += static_cast<size_t>(std::max(it.nshards, 0));
             std::vector<uint8_t> filled(std::max<size_t>(nflags, 1));
             lock.unlock();
+            const Clock::time_point p0 = Clock::now();
             int rc = rsx_batch_plan(free_slot->b, items.data(), static_cast<int>(items.size()), status.data(),
                                     filled.data());
-            if (rc == RSX_OK) rc = rsx_batch_launch(free_slot->b);
+            hipStream_t bs = static_cast<hipStream_t>(rsx_batch_stream(free_slot->b));
             if (rc == RSX_OK) {
-                const hipError_t e =
-                    hipEventRecord(free_slot->ev, static_cast<hipStream_t>(rsx_batch_stream(free_slot->b)));
+                const hipError_t e = hipEventRecord(free_slot->ev0, bs);
                 if (e != hipSuccess) rc = rsx::hip_fail(e, "hipEventRecord");
             }
+            if (rc == RSX_OK) rc = rsx_batch_launch(free_slot->b);
+            if (rc == RSX_OK) {
+                const hipError_t e = hipEventRecord(free_slot->ev, bs);
+                if (e != hipSuccess) rc = rsx::hip_fail(e, "hipEventRecord");
+            }
+            const uint64_t plan_ns = static_cast<uint64_t>(
+                std::chrono::duration_cast<std::chrono::nanoseconds>(Clock::now() - p0).count());
             lock.lock();
+            s->plan_ns += plan_ns;
             ++s->batches;
             s->batched_requests += admitted.size();
+            bool failed = false;
             for (size_t i = 0; i < admitted.size(); ++i) {
                 if (rc == RSX_OK && status[i] == RSX_ERR_NOT_SUPPORTED && s->reqs[admitted[i]].leopard) {
                     // Leopard shapes / alignments the batch kernel does not
@@ -185,11 +205,13 @@ void scheduler_loop(rsx_server *s) {
                     s->work_cv.notify_one();
                 } else if (rc != RSX_OK || status[i] != RSX_OK) {
                     complete(s, admitted[i], rc != RSX_OK ? rc : status[i], 0);
+                    failed = true;
                 } else {
                     free_slot->tickets.emplace_back(admitted[i], static_cast<int>(i));
                 }
             }
             free_slot->busy = !free_slot->tickets.empty();
+            if (failed) notify_completed(s);
             continue;  // look for more work right away
         }
         bool busy = false;
@@ -240,6 +262,7 @@ int rsx_server_create(int device, size_t hbm_budget, int max_batch, rsx_server *
     auto release_slots = [&] {
         for (Slot &sl : s->slots) {
             if (sl.ev) (void)hipEventDestroy(sl.ev);
+            if (sl.ev0) (void)hipEventDestroy(sl.ev0);
             rsx_batch_free(sl.b);
         }
     };
@@ -251,8 +274,7 @@ int rsx_server_create(int device, size_t hbm_budget, int max_batch, rsx_server *
         }
         rsx_batch_set_graph(sl.b, 0);
         rsx_batch_set_tuning(sl.b, 16 << 10, 8);
-        if (hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming) != hipSuccess) {
-            sl.ev = nullptr;
+        if (hipEventCreate(&sl.ev) != hipSuccess || hipEventCreate(&sl.ev0) != hipSuccess) {
             (void)hipGetLastError();
             release_slots();
             return rsx::fail(RSX_ERR_DEVICE, "event creation failed");
@@ -284,6 +306,7 @@ void rsx_server_free(rsx_server *s) {
     s->worker.join();
     for (Slot &sl : s->slots) {
         if (sl.ev) (void)hipEventDestroy(sl.ev);
+        if (sl.ev0) (void)hipEventDestroy(sl.ev0);
         rsx_batch_free(sl.b);
     }
     delete s;
@@ -365,11 +388,14 @@ int rsx_server_run_trace(rsx_server *s, const rsx_request *reqs, const uint64_t 
     return rc;
 }
 
-int rsx_server_stats(const rsx_server *s, uint64_t *batches, uint64_t *requests, uint64_t *max_inflight_bytes) {
+int rsx_server_stats(const rsx_server *s, uint64_t *batches, uint64_t *requests, uint64_t *max_inflight_bytes,
+                     uint64_t *plan_ns, uint64_t *device_ns) {
     std::lock_guard<std::mutex> lock(s->mu);
     *batches = s->batches;
     *requests = s->batched_requests;
     *max_inflight_bytes = s->max_inflight;
+    if (plan_ns) *plan_ns = s->plan_ns;
+    if (device_ns) *device_ns = s->device_ns;
     return RSX_OK;
 }
 
