@@ -612,9 +612,24 @@ def mixed(args, torch, rs):
             ops.append(rs.OP_ENCODE)
             lens.append(st.full[i])
     bat = rs.Batch(0)
+    mixed_items = batch_items(rs, st, codecs, ops, lens)
     t_plan0 = time.perf_counter()
-    plan(rs, bat, batch_items(rs, st, codecs, ops, lens), nst)
+    plan(rs, bat, mixed_items, nst)
     t_plan = time.perf_counter() - t_plan0
+    # The same plan again on fresh batches: decode plans now cached, with
+    # autotuning, then with the tuning fixed (graph capture on and off).
+    plan_costs = {}
+    for name, tuned, graph in (("cached_patterns_autotune", False, True), ("tuned_graph", True, True),
+                               ("tuned_no_graph", True, False)):
+        b2 = rs.Batch(0)
+        if tuned:
+            b2.set_tuning(*bat.tuning())
+        rs.lib.rsx_batch_set_graph(b2._h, 1 if graph else 0)
+        t0 = time.perf_counter()
+        plan(rs, b2, mixed_items, nst)
+        plan_costs[name] = round((time.perf_counter() - t0) * 1e3, 2)
+        b2.synchronize()
+        del b2
     handles = [codecs[s].handle for s in shapes]
     filled = (ctypes.c_uint8 * 64)()
 
@@ -648,10 +663,14 @@ def mixed(args, torch, rs):
     t_bat = (time.perf_counter() - t0) / reps
     _log(f"bench: mixed plain {total / t_plain / 1e9:.1f} GB/s, batched {total / t_bat / 1e9:.1f} GB/s")
     tile, bpc = bat.tuning()
+    # replays after which plan + replays beats the same passes made per call
+    breakeven = {k: round(v / (t_plain - t_bat) / 1e3, 2) for k, v in
+                 [("first_plan", t_plan * 1e3)] + list(plan_costs.items())} if t_plain > t_bat else None
     out = dict(stripes=nst, bytes_per_pass=total, plain_gbps=round(total / t_plain / 1e9, 1),
                batched_gbps=round(total / t_bat / 1e9, 1), batched_over_plain=round(t_plain / t_bat, 2),
                ms_plain=round(t_plain * 1e3, 3), ms_batched=round(t_bat * 1e3, 3),
-               plan_ms=round(t_plan * 1e3, 1), tuning={"tile_bytes": tile, "blocks_per_cu": bpc},
+               plan_ms=round(t_plan * 1e3, 1), plan_ms_again=plan_costs, breakeven_replays=breakeven,
+               tuning={"tile_bytes": tile, "blocks_per_cu": bpc},
                reconstruct_fraction=round(ops.count(rs.OP_RECONSTRUCT) / nst, 3))
     del st, bat
     torch.cuda.empty_cache()
@@ -706,10 +725,24 @@ def leopard(args, torch, rs):
         _check(lib, lib.rsx_reconstruct(h, ptrs, lens, n, filled))
     _check(lib, lib.rsx_synchronize(h))
     t_rec = (time.perf_counter() - t0) / reps
+    # A new erasure pattern per call (cold): its error locators are computed
+    # and uploaded first (the timings above reuse one cached pattern).
+    cold = []
+    for trial in range(3):
+        lost2 = set(np.random.default_rng(2000 + trial).choice(n, m, replace=False).tolist())
+        lens2 = (ctypes.c_size_t * n)(*[0 if i in lost2 else S for i in range(n)])
+        _check(lib, lib.rsx_synchronize(h))
+        t0 = time.perf_counter()
+        _check(lib, lib.rsx_reconstruct(h, ptrs, lens2, n, filled))
+        _check(lib, lib.rsx_synchronize(h))
+        cold.append(time.perf_counter() - t0)
+    t_cold = float(np.mean(cold))
     b = n * S
-    _log(f"bench: leopard16 encode {b / t_enc / 1e9:.1f} GB/s, reconstruct {b / t_rec / 1e9:.1f} GB/s")
+    _log(f"bench: leopard16 encode {b / t_enc / 1e9:.1f} GB/s, reconstruct {b / t_rec / 1e9:.1f} GB/s "
+         f"(new erasure pattern: {t_cold * 1e3:.2f} ms)")
     out = dict(k=k, m=m, shard_bytes=S, erasures=m, encode_ms=round(t_enc * 1e3, 2),
-               reconstruct_ms=round(t_rec * 1e3, 2), encode_gbps=round(b / t_enc / 1e9, 1),
+               reconstruct_ms=round(t_rec * 1e3, 2), reconstruct_cold_ms=round(t_cold * 1e3, 2),
+               encode_gbps=round(b / t_enc / 1e9, 1),
                reconstruct_gbps=round(b / t_rec / 1e9, 1),
                encode_plus_reconstruct_gbps=round(2 * b / (t_enc + t_rec) / 1e9, 1),
                reconstruct_equals_original=rebuilt_ok,

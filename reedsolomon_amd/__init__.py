@@ -604,7 +604,10 @@ class Batch:
         _check(lib.rsx_batch_set_graph(self._h, 1 if enabled else 0))
 
     def plan(self, items):
-        """Returns the per-item status codes (0 = ok)."""
+        """Returns the per-item status codes (0 = ok).  An autotuning plan runs
+        the work a few times, so torch work queued on the items' shards is
+        waited for first."""
+        _torch_sync()
         n = len(items)
         arr = (_BatchItem * max(n, 1))()
         keep, fills = [], []

@@ -7,6 +7,7 @@
 // equals the error the plain call would return.
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <tuple>
@@ -169,8 +170,15 @@ void destroy_graph(rsx_batch *b) {
 }
 
 // Tiles above which a plan finds tasks by binary search instead of keeping
-// a tile -> task table (4 bytes per tile) on the device.
+// a tile -> task table (4 bytes per tile) on the device.  RSX_BATCH_TILE_TABLE_MAX
+// (read at each plan) lowers it, so tests can run the search on small plans.
 constexpr uint64_t kTileTableMax = uint64_t{1} << 24;
+
+uint64_t tile_table_max() {
+    const char *v = std::getenv("RSX_BATCH_TILE_TABLE_MAX");
+    if (!v || !*v) return kTileTableMax;
+    return std::min<uint64_t>(kTileTableMax, std::strtoull(v, nullptr, 10));
+}
 
 // Output-count classes of the batched GF(2^8) pass (one kernel each).  Plans
 // keep 8-output tasks: a 16-output class measured slower in mixed plans
@@ -185,7 +193,7 @@ int build_launches(rsx_batch *b, uint32_t tile) {
     uint64_t total_tiles = 0;
     for (int mc : kClasses)
         for (const auto &t : b->tasks[mc]) total_tiles += (t.len + tile - 1) / tile;
-    const bool use_table = total_tiles <= kTileTableMax;
+    const bool use_table = total_tiles <= tile_table_max();
     b->launches.clear();
     b->classes.clear();
     for (int mc : kClasses) {

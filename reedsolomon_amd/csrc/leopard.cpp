@@ -59,17 +59,21 @@ static int device_tables(int device, int bits, hipStream_t stream, const LeoDev 
     return RSX_OK;
 }
 
-// Error locators of one erasure pattern (host copy + HBM copy).
+// Error locators of one erasure pattern (host copy; batched plans copy it).
 struct LeoPattern {
     std::vector<uint16_t> errlocs;  // n entries
     int input_count = 0;
-    DeviceBuffer d_errlocs;
-    // Output side of a per-stripe reconstruct, uploaded on first use: work
-    // rows of the erased shards, their reveal logs, and row -> output index.
+    // What a per-stripe reconstruct reads, uploaded on its first use in one
+    // allocation and one copy: the error locators, the work rows of the
+    // erased shards, their reveal logs, row -> output index, FFT pruning bits.
     bool outs_ready = false;
     int nout = 0;
     uint64_t pass3_blocks = 0;  // three-pass reconstruct: row blocks with an output row
-    DeviceBuffer d_rows, d_logs, d_outidx, d_need;
+    std::vector<uint8_t> host_blob;  // source of the copy (kept while it may run)
+    DeviceBuffer d_blob;
+    const uint16_t *d_errlocs = nullptr, *d_logs = nullptr;
+    const int *d_rows = nullptr, *d_outidx = nullptr;
+    const uint32_t *d_need = nullptr;
 };
 
 struct LeopardState {
@@ -391,11 +395,6 @@ int pattern_for(rsx_codec *c, const uint8_t *erased, bool recover_all, std::shar
     }
     auto pat = std::make_shared<LeoPattern>();
     pat->errlocs = leo_error_locators(*s.field, k, p, erased, &pat->input_count);
-    int err = pat->d_errlocs.reserve(pat->errlocs.size() * 2);
-    if (err) return err;
-    RSX_HIP(hipMemcpyAsync(pat->d_errlocs.ptr, pat->errlocs.data(), pat->errlocs.size() * 2,
-                           hipMemcpyHostToDevice, c->stream));
-    RSX_HIP(hipStreamSynchronize(c->stream));
     s.patterns.emplace(std::move(key), pat);
     *out = std::move(pat);
     return RSX_OK;
@@ -435,19 +434,28 @@ int reconstruct_device(rsx_codec *c, const std::vector<uint8_t *> &shards, const
         }
         const int no = static_cast<int>(rows.size());
         const std::vector<uint32_t> need = leo_fft_need(n, rows);
-        if ((err = pat->d_need.reserve(std::max<size_t>(need.size(), 1) * sizeof(uint32_t)))) return err;
-        RSX_HIP(hipMemcpyAsync(pat->d_need.ptr, need.data(), need.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
-                               c->stream));
-        if ((err = pat->d_rows.reserve(std::max(no, 1) * sizeof(int)))) return err;
-        if ((err = pat->d_logs.reserve(std::max(no, 1) * sizeof(uint16_t)))) return err;
-        if ((err = pat->d_outidx.reserve(out_index.size() * sizeof(int)))) return err;
+        // One buffer: errlocs (n), logs (no), rows (no), out_index (n), need.
+        auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+        const size_t o_err = 0, o_logs = up(pat->errlocs.size() * 2), o_rows = o_logs + up(logs.size() * 2),
+                     o_idx = o_rows + up(rows.size() * 4), o_need = o_idx + up(out_index.size() * 4),
+                     total_bytes = o_need + up(std::max<size_t>(need.size(), 1) * 4);
+        pat->host_blob.assign(total_bytes, 0);
+        uint8_t *h = pat->host_blob.data();
+        std::memcpy(h + o_err, pat->errlocs.data(), pat->errlocs.size() * 2);
         if (no) {
-            RSX_HIP(hipMemcpyAsync(pat->d_rows.ptr, rows.data(), no * sizeof(int), hipMemcpyHostToDevice, c->stream));
-            RSX_HIP(hipMemcpyAsync(pat->d_logs.ptr, logs.data(), no * sizeof(uint16_t), hipMemcpyHostToDevice,
-                                   c->stream));
+            std::memcpy(h + o_logs, logs.data(), logs.size() * 2);
+            std::memcpy(h + o_rows, rows.data(), rows.size() * 4);
         }
-        RSX_HIP(hipMemcpyAsync(pat->d_outidx.ptr, out_index.data(), out_index.size() * sizeof(int),
-                               hipMemcpyHostToDevice, c->stream));
+        std::memcpy(h + o_idx, out_index.data(), out_index.size() * 4);
+        if (!need.empty()) std::memcpy(h + o_need, need.data(), need.size() * 4);
+        if ((err = pat->d_blob.reserve(total_bytes))) return err;
+        RSX_HIP(hipMemcpyAsync(pat->d_blob.ptr, h, total_bytes, hipMemcpyHostToDevice, c->stream));
+        uint8_t *d = pat->d_blob.as<uint8_t>();
+        pat->d_errlocs = reinterpret_cast<const uint16_t *>(d + o_err);
+        pat->d_logs = reinterpret_cast<const uint16_t *>(d + o_logs);
+        pat->d_rows = reinterpret_cast<const int *>(d + o_rows);
+        pat->d_outidx = reinterpret_cast<const int *>(d + o_idx);
+        pat->d_need = reinterpret_cast<const uint32_t *>(d + o_need);
         pat->nout = no;
         int pC = 0, pP = 0, pA = 0;
         if (leo_multipass_shape(n, &pC, &pP, &pA)) pat->pass3_blocks = leo_pass3_blocks(pP, rows);
@@ -465,8 +473,8 @@ int reconstruct_device(rsx_codec *c, const std::vector<uint8_t *> &shards, const
     // n = 4096 GF(2^16) transforms: one tile of quarter-chunk slices.
     if (!slice && s.fused && s.bits == 16 && kQuarterTiles) slice = leo_quarter_slice(n);
     if (slice) {
-        const LeoDecodeArgs a{lptr, pat->d_errlocs.as<uint16_t>(), pat->d_rows.as<int>(), pat->d_logs.as<uint16_t>(), optr,
-                              size, n, nout, ifft_trunc, m + k, slice, pat->d_need.as<uint32_t>()};
+        const LeoDecodeArgs a{lptr, pat->d_errlocs, pat->d_rows, pat->d_logs, optr,
+                              size, n, nout, ifft_trunc, m + k, slice, pat->d_need};
         if (nout) RSX_HIP(leo_decode_fused(*s.dev, a, c->stream));
         return RSX_OK;
     }
@@ -478,10 +486,10 @@ int reconstruct_device(rsx_codec *c, const std::vector<uint8_t *> &shards, const
         if ((err = s.work2.reserve(cw * n))) return err;
         for (size_t off = 0; off < size; off += cw) {
             const size_t cols = std::min(cw, size - off);
-            const LeoMultipassArgs a{lptr, pat->d_errlocs.as<uint16_t>(), pat->d_outidx.as<int>(), pat->d_logs.as<uint16_t>(),
+            const LeoMultipassArgs a{lptr, pat->d_errlocs, pat->d_outidx, pat->d_logs,
                                      optr, s.work.as<uint8_t>(), s.work2.as<uint8_t>(), off,
                                      static_cast<int>(cols / 64), n, ifft_trunc, m + k, mC, mP, mA,
-                                     pat->d_need.as<uint32_t>(), pat->pass3_blocks};
+                                     pat->d_need, pat->pass3_blocks};
             RSX_HIP(leo_decode_multipass(*s.dev, a, c->stream));
         }
         return RSX_OK;
@@ -494,7 +502,7 @@ int reconstruct_device(rsx_codec *c, const std::vector<uint8_t *> &shards, const
         const size_t cols = std::min(chunk, size - off);
         LeoWork w{s.work.as<uint8_t>(), chunk, cols};
         LeoWork w2{s.work2.as<uint8_t>(), chunk, cols};
-        RSX_HIP(leo_load_rows(f, lptr, pat->d_errlocs.as<uint16_t>(), off, n, w, c->stream));
+        RSX_HIP(leo_load_rows(f, lptr, pat->d_errlocs, off, n, w, c->stream));
         // leopard.go:650-693 ifftDITDecoder
         int dist = 1, dist4 = 4;
         while (dist4 <= n) {
@@ -506,7 +514,7 @@ int reconstruct_device(rsx_codec *c, const std::vector<uint8_t *> &shards, const
         RSX_HIP(leo_derivative(w, w2, n, c->stream));
         if ((err = fft(c, s, w2, m + k, n))) return err;
         if (nout)
-            RSX_HIP(leo_store_rows(f, w2, pat->d_rows.as<int>(), pat->d_logs.as<uint16_t>(), optr, off, nout, c->stream));
+            RSX_HIP(leo_store_rows(f, w2, pat->d_rows, pat->d_logs, optr, off, nout, c->stream));
     }
     return RSX_OK;
 }

@@ -74,29 +74,29 @@ uint16_t LeoField::mul_log(uint16_t a, uint16_t log_b) const {
 }
 
 void LeoField::fwht(uint16_t *data, int mtrunc) const {
-    const uint64_t mask = order - 1;
+    // 32-bit lanes (sums stay below 2^17), so the loop over `off` vectorizes;
+    // a - b wraps modulo 2^32 and folds to the same residue the reference's
+    // 64-bit uint arithmetic gives (leopard.go:1082-1091).
+    const uint32_t mask = order - 1;
+    const int sh = bits;
     int dist = 1, dist4 = 4;
     while (dist4 <= static_cast<int>(order)) {
-        for (int r = 0; r < mtrunc; r += dist4)
-            for (int off = r; off < r + dist; ++off) {
-                uint64_t t0 = data[off], t1 = data[off + dist], t2 = data[off + 2 * dist],
-                         t3 = data[off + 3 * dist];
-                auto fwht2 = [&](uint64_t &a, uint64_t &b) {
-                    uint64_t s = a + b, d = a - b;
-                    s = (s + (s >> bits)) & mask;
-                    d = (d + (d >> bits)) & mask;
-                    a = s;
-                    b = d;
-                };
-                fwht2(t0, t1);
-                fwht2(t2, t3);
-                fwht2(t0, t2);
-                fwht2(t1, t3);
-                data[off] = static_cast<uint16_t>(t0);
-                data[off + dist] = static_cast<uint16_t>(t1);
-                data[off + 2 * dist] = static_cast<uint16_t>(t2);
-                data[off + 3 * dist] = static_cast<uint16_t>(t3);
+        for (int r = 0; r < mtrunc; r += dist4) {
+            uint16_t *p0 = data + r, *p1 = p0 + dist, *p2 = p1 + dist, *p3 = p2 + dist;
+            for (int off = 0; off < dist; ++off) {
+                const uint32_t t0 = p0[off], t1 = p1[off], t2 = p2[off], t3 = p3[off];
+                uint32_t s01 = t0 + t1, d01 = t0 - t1, s23 = t2 + t3, d23 = t2 - t3;
+                s01 = (s01 + (s01 >> sh)) & mask;
+                d01 = (d01 + (d01 >> sh)) & mask;
+                s23 = (s23 + (s23 >> sh)) & mask;
+                d23 = (d23 + (d23 >> sh)) & mask;
+                uint32_t a = s01 + s23, c = s01 - s23, b = d01 + d23, d = d01 - d23;
+                p0[off] = static_cast<uint16_t>((a + (a >> sh)) & mask);
+                p2[off] = static_cast<uint16_t>((c + (c >> sh)) & mask);
+                p1[off] = static_cast<uint16_t>((b + (b >> sh)) & mask);
+                p3[off] = static_cast<uint16_t>((d + (d >> sh)) & mask);
             }
+        }
         dist = dist4;
         dist4 <<= 2;
     }
@@ -156,8 +156,16 @@ std::vector<uint16_t> leo_error_locators(const LeoField &f, int k, int p, const 
         else ic = m + i + 1;
     }
     f.fwht(e.data(), m + k);
-    for (unsigned i = 0; i < f.order; ++i)
-        e[i] = static_cast<uint16_t>((static_cast<uint64_t>(e[i]) * f.walsh[i]) % f.modulus);
+    // (e * walsh) % modulus (leopard.go:506-508), reduced by folding: 2^bits
+    // is 1 modulo 2^bits - 1, so no division per entry.
+    const uint32_t fmask = f.modulus;
+    for (unsigned i = 0; i < f.order; ++i) {
+        uint32_t x = static_cast<uint32_t>(e[i]) * f.walsh[i];
+        x = (x & fmask) + (x >> f.bits);
+        x = (x & fmask) + (x >> f.bits);
+        if (x >= f.modulus) x -= f.modulus;
+        e[i] = static_cast<uint16_t>(x);
+    }
     f.fwht(e.data(), static_cast<int>(f.order));
     if (input_count) *input_count = ic;
     e.resize(n);
