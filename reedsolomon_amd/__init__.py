@@ -274,6 +274,123 @@ def _zeros_like(x, n):
     return np.zeros(n, np.uint8)
 
 
+def split_shards(data, data_shards: int, total_shards: int, multiple: int = 1):
+    """Encoder.Split (reedsolomon.go:1803-1860; Leopard leopard.go:319-376,
+    leopard8.go:293-350, where `multiple` is 64): perShard =
+    ceil(len / data_shards) rounded up to `multiple`; shard i holds bytes
+    [i * perShard, (i + 1) * perShard) of `data` (views where the input has
+    them), zero-padded past its end, and the remaining shards, parity
+    included, are zero.  Go's reuse of spare slice capacity has no Python
+    counterpart; the bytes of every shard are the reference's."""
+    n = len(data)
+    if n == 0:
+        raise ErrShortData()
+    if total_shards == 1 and n % multiple == 0:
+        return [data]
+    per = (n + data_shards - 1) // data_shards
+    per = (per + multiple - 1) // multiple * multiple
+    full = n // per
+    out = [data[i * per:(i + 1) * per] for i in range(min(full, total_shards))]
+    rest = data[full * per:]
+    for _ in range(total_shards - len(out)):
+        pad = _zeros_like(data, per)
+        m = min(len(rest), per)
+        if m:
+            pad[:m] = rest[:m]
+            rest = rest[m:]
+        out.append(pad)
+    return out
+
+
+def join_shards(dst, shards, data_shards: int, out_size: int) -> None:
+    """Encoder.Join (reedsolomon.go:1875-1910, leopard.go:276-310): writes the
+    first out_size bytes of the data shards to dst; ErrTooFewShards with fewer
+    than data_shards entries, ErrReconstructRequired for a None data shard
+    before out_size is covered, ErrShortData when they hold fewer bytes."""
+    if len(shards) < data_shards:
+        raise ErrTooFewShards()
+    shards = shards[:data_shards]
+    size = 0
+    for sh in shards:
+        if sh is None:
+            raise ErrReconstructRequired()
+        size += len(sh)
+        if size >= out_size:
+            break
+    if size < out_size:
+        raise ErrShortData()
+    write = out_size
+    for sh in shards:
+        if write == 0:
+            return
+        b = _host_bytes(sh)
+        if write < len(b):
+            dst.write(b[:write])
+            return
+        dst.write(b)
+        write -= len(b)
+
+
+def write_shard_files(path: str, shards, out_dir=None):
+    """The reference's file layout for shards (examples/simple-encoder.go:91-101):
+    shard i of `path` is written to `<out_dir or dir of path>/<file>.<i>`.
+    Returns the file names."""
+    d, f = os.path.split(path)
+    d = out_dir if out_dir else d
+    names = []
+    for i, sh in enumerate(shards):
+        name = os.path.join(d, f"{f}.{i}")
+        with open(name, "wb") as fh:
+            fh.write(_host_bytes(sh))
+        names.append(name)
+    return names
+
+
+def read_shard_files(path: str, total_shards: int):
+    """examples/simple-decoder.go:76-85: reads `<path>.<i>` for every shard;
+    a file that cannot be read is None (a missing shard)."""
+    shards = []
+    for i in range(total_shards):
+        try:
+            with open(f"{path}.{i}", "rb") as fh:
+                shards.append(np.frombuffer(fh.read(), dtype=np.uint8).copy())
+        except OSError:
+            shards.append(None)
+    return shards
+
+
+def encode_file(enc, path: str, out_dir=None):
+    """examples/simple-encoder.go:77-101: Split the file, Encode, write
+    `<file>.<i>` shard files.  Returns the file names."""
+    with open(path, "rb") as fh:
+        data = np.frombuffer(fh.read(), dtype=np.uint8).copy()
+    shards = enc.Split(data)
+    enc.Encode(shards)
+    return write_shard_files(path, shards, out_dir)
+
+
+def decode_files(enc, path: str, out_path: str, out_size=None) -> bool:
+    """examples/simple-decoder.go:72-123: read `<path>.<i>`, Verify, Reconstruct
+    if needed (then Verify again), Join into out_path.  out_size defaults to
+    the example's len(shards[0]) * DataShards (the original size is not
+    stored).  Returns whether a reconstruction was needed."""
+    shards = read_shard_files(path, enc.TotalShards())
+    rebuilt = False
+    try:
+        ok = enc.Verify(shards)
+    except RSError:
+        ok = False  # a missing shard: Verify reports it as an error
+    if not ok:
+        enc.Reconstruct(shards)
+        rebuilt = True
+        if not enc.Verify(shards):
+            raise RSError("verification failed after reconstruction, data likely corrupted")
+    size = next(len(s) for s in shards if s is not None)
+    with open(out_path, "wb") as fh:
+        enc.Join(fh, shards, out_size if out_size is not None else size * enc.DataShards())
+    return rebuilt
+
+
 def _host_bytes(x) -> bytes:
     if _is_torch(x):
         return x.cpu().numpy().tobytes()
@@ -412,56 +529,16 @@ class Encoder:
                 shards[i] = bufs[i]
 
     # Split / Join (reedsolomon.go:1803-1910, leopard.go:276-314, leopard8.go:293-350)
-    # are data-format helpers with no device work: the shard layout is the
-    # reference's (per-shard size rounded up to ShardSizeMultiple, data
-    # shards cut from the input, zero padding, zeroed parity).
+    # are data-format helpers with no device work; see split_shards / join_shards.
     def Split(self, data):
         """Shards of `data` (numpy array or torch tensor): full shards are
         views of `data`, the rest new zero-padded buffers."""
-        n = len(data)
-        if n == 0:
-            raise ErrShortData()
-        total, k = self.TotalShards(), self._k
-        mult = self.ShardSizeMultiple()
-        if total == 1 and n % mult == 0:
-            return [data]
-        per = (n + k - 1) // k
-        per = (per + mult - 1) // mult * mult
-        full = n // per
-        out = [data[i * per:(i + 1) * per] for i in range(min(full, total))]
-        rest = data[full * per:]
-        for _ in range(total - len(out)):
-            pad = _zeros_like(data, per)
-            m = min(len(rest), per)
-            if m:
-                pad[:m] = rest[:m]
-                rest = rest[m:]
-            out.append(pad)
-        return out
+        return split_shards(data, self._k, self.TotalShards(), self.ShardSizeMultiple())
 
     def Join(self, dst, shards, outSize: int) -> None:
         """Writes the first outSize bytes of the data shards to dst (a file-like
         object with write())."""
-        if len(shards) < self._k:
-            raise ErrTooFewShards()
-        shards = shards[:self._k]
-        size = 0
-        for sh in shards:
-            if sh is None:
-                raise ErrReconstructRequired()
-            size += len(sh)
-            if size >= outSize:
-                break
-        if size < outSize:
-            raise ErrShortData()
-        write = outSize
-        for sh in shards:
-            b = _host_bytes(sh)
-            if write < len(b):
-                dst.write(b[:write])
-                return
-            dst.write(b)
-            write -= len(b)
+        join_shards(dst, shards, self._k, outSize)
 
     def Update(self, shards, newDatashards) -> None:
         """reedsolomon.go:762.  None = nil.  Parity is updated in place and each

@@ -53,3 +53,24 @@ def test_split_join_round_trip(mode, opt):
                 holes[0] = None
                 with pytest.raises(rs.ErrReconstructRequired):
                     enc.Join(io.BytesIO(), holes, size)
+
+
+@pytest.mark.parametrize("opt", range(3))
+def test_shard_files_encode_decode(tmp_path, opt):
+    # examples/simple-encoder.go and simple-decoder.go: file -> name.N shards
+    # -> two shard files lost -> Verify, Reconstruct, Join -> the file again
+    # (zero-padded to len(shards[0]) * DataShards, as the example writes it).
+    enc = rs.New(6, 3, *OPTS[opt])
+    raw = np.random.default_rng(40 + opt).integers(0, 256, 100_003, dtype=np.uint8)
+    src = tmp_path / "file.bin"
+    src.write_bytes(raw.tobytes())
+    names = rs.encode_file(enc, str(src))
+    assert len(names) == 9
+    out = tmp_path / "file.out"
+    assert not rs.decode_files(enc, str(src), str(out))
+    got = out.read_bytes()
+    assert got[:len(raw)] == raw.tobytes() and not any(got[len(raw):])
+    for i in (1, 7):
+        (tmp_path / f"file.bin.{i}").unlink()
+    assert rs.decode_files(enc, str(src), str(out), out_size=len(raw))
+    assert out.read_bytes() == raw.tobytes()
