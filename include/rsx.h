@@ -49,6 +49,8 @@ extern "C" {
 #define RSX_ERR_DEVICE -20              /* HIP runtime error (no reference counterpart) */
 #define RSX_ERR_NOMEM -21               /* device allocation failed (no reference counterpart) */
 #define RSX_ERR_MIXED_MEMORY -22        /* host and device shards in one call (no counterpart) */
+#define RSX_ERR_CANCELLED -23           /* request-loop request cancelled (no counterpart) */
+#define RSX_ERR_OVER_BUDGET -24         /* request-loop admission rejected (no counterpart) */
 
 /* Engine selection, options.go:283-305 (WithLeopardGF16 / WithLeopardGF). */
 #define RSX_LEOPARD_AS_NEEDED 0
@@ -252,9 +254,9 @@ typedef struct rsx_request {
 
 typedef struct rsx_server rsx_server;
 
-/* hbm_budget: bytes of shards in flight (a larger request is admitted alone;
- * one batch takes at most budget / 8); max_batch: requests per launch
- * (0 = 4096). */
+/* hbm_budget: bytes of shards in flight (0 = 4 GiB; a request larger than
+ * the budget is rejected at submit with RSX_ERR_OVER_BUDGET; one batch takes
+ * at most budget / 8); max_batch: requests per launch (0 = 4096). */
 int rsx_server_create(int device, size_t hbm_budget, int max_batch, rsx_server **out);
 void rsx_server_free(rsx_server *s); /* drains first */
 int rsx_server_submit(rsx_server *s, const rsx_request *req, uint64_t *ticket);
@@ -264,9 +266,28 @@ int rsx_server_poll(rsx_server *s, uint64_t ticket, int *done, int *status, int 
                     uint64_t *latency_ns);
 int rsx_server_wait(rsx_server *s, uint64_t ticket, int *status, int *verify_ok,
                     uint64_t *latency_ns);
+/* Waits until every submitted request has completed.  On a paused server
+ * with requests queued it returns RSX_ERR_INVALID_INPUT instead of waiting. */
 int rsx_server_drain(rsx_server *s);
+/* Cancels a request that has not been admitted yet: *cancelled = 1 and the
+ * ticket completes with status RSX_ERR_CANCELLED (its shards untouched; it
+ * is still reported once by poll / wait).  A request already admitted into a
+ * batch, or complete, runs to its normal result: *cancelled = 0. */
+int rsx_server_cancel(rsx_server *s, uint64_t ticket, int *cancelled);
+/* paused != 0: the scheduler admits nothing new (batches already planned
+ * finish); 0 resumes.  Lets a caller hold requests queued, e.g. to cancel
+ * them.  rsx_server_free resumes before draining. */
+int rsx_server_pause(rsx_server *s, int paused);
+/* Rejects, with RSX_ERR_OVER_BUDGET at submit, a request that would take the
+ * bytes queued and not yet admitted past max_queued_bytes (0 = no limit,
+ * the default). */
+int rsx_server_set_queue_limit(rsx_server *s, size_t max_queued_bytes);
+/* Requests rejected at submit, cancelled, and shard bytes queued now. */
+int rsx_server_admission(const rsx_server *s, uint64_t *rejected, uint64_t *cancelled,
+                         uint64_t *queued_bytes);
 /* Replays a trace: request i arrives arrival_ns[i] after the start (sorted);
- * fills status / verify_ok / latency_ns per request and the makespan. */
+ * fills status / verify_ok / latency_ns per request and the makespan.  A
+ * request rejected at submit gets that error as its status (latency 0). */
 int rsx_server_run_trace(rsx_server *s, const rsx_request *reqs, const uint64_t *arrival_ns, int n,
                          int *status, int *verify_ok, uint64_t *latency_ns, uint64_t *makespan_ns);
 /* Counters: batched launches, requests in them, highest shard bytes in

@@ -125,6 +125,8 @@ _ERRORS = {
     -20: _err("ErrDevice", -20, "HIP device error"),
     -21: _err("ErrNoMem", -21, "device allocation failed"),
     -22: _err("ErrMixedMemory", -22, "host and device shards mixed in one call"),
+    -23: _err("ErrCancelled", -23, "request cancelled before admission"),
+    -24: _err("ErrOverBudget", -24, "request rejected: larger than the HBM budget or the queue is full"),
 }
 
 
@@ -744,6 +746,10 @@ lib.rsx_server_drain.argtypes = [_P]
 lib.rsx_server_run_trace.argtypes = [_P, ctypes.POINTER(_Request), _U64, ctypes.c_int, _I, _I, _U64,
                                      _U64]
 lib.rsx_server_stats.argtypes = [_P, _U64, _U64, _U64, _U64, _U64]
+lib.rsx_server_cancel.argtypes = [_P, ctypes.c_uint64, _I]
+lib.rsx_server_pause.argtypes = [_P, ctypes.c_int]
+lib.rsx_server_set_queue_limit.argtypes = [_P, ctypes.c_size_t]
+lib.rsx_server_admission.argtypes = [_P, _U64, _U64, _U64]
 
 
 def _fill_outputs(op, k, shards, bufs):
@@ -815,6 +821,28 @@ class Server:
                 _fill_outputs(op, enc.DataShards(), shards, keep[i][1])
         return ([st[i] for i in range(n)], [bool(ok[i]) for i in range(n)],
                 [lat[i] for i in range(n)], span.value)
+
+    def cancel(self, ticket: int) -> bool:
+        """Cancels a request not yet admitted into a batch (True); its wait()
+        then reports ErrCancelled's code.  False: it runs to completion."""
+        c = ctypes.c_int()
+        _check(lib.rsx_server_cancel(self._h, ticket, ctypes.byref(c)))
+        return bool(c.value)
+
+    def pause(self, paused: bool = True) -> None:
+        """Holds (True) or resumes (False) admission of queued requests."""
+        _check(lib.rsx_server_pause(self._h, int(bool(paused))))
+
+    def set_queue_limit(self, max_queued_bytes: int) -> None:
+        """submit() raises ErrOverBudget past this many queued shard bytes
+        (0: no limit)."""
+        _check(lib.rsx_server_set_queue_limit(self._h, max_queued_bytes))
+
+    def admission(self):
+        """(requests rejected at submit, requests cancelled, bytes queued)."""
+        v = [ctypes.c_uint64() for _ in range(3)]
+        _check(lib.rsx_server_admission(self._h, *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
 
     def drain(self) -> None:
         _check(lib.rsx_server_drain(self._h))

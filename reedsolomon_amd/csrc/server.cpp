@@ -1,11 +1,12 @@
 // server.cpp — request loop with continuous batching (include/rsx.h,
 // "Serving").
 //
-// Submit copies the request and appends it to a FIFO.  The scheduler thread
-// loops: (1) retire launch slots whose completion event has fired, (2) when a
+// Submit copies the request and appends it to a FIFO; a request larger than
+// the HBM budget, or one that would take the queued bytes past the queue
+// limit, is rejected there.  A queued request can be cancelled until the
+// scheduler admits it.  The scheduler thread loops: (1) retire launch slots whose completion event has fired, (2) when a
 // slot is free, admit queued requests in order while the shard bytes in
-// flight stay within the HBM budget (a request larger than the budget is
-// admitted alone), plan the admitted requests (classic and Leopard) into that
+// flight stay within the HBM budget (unless paused), plan the admitted requests (classic and Leopard) into that
 // slot's batch and launch it on the slot's stream, (3) hand the Leopard
 // requests the batch kernel does not take (unaligned shards, rows too tall
 // for one LDS tile) to a worker thread that runs the per-stripe call.
@@ -34,6 +35,7 @@ struct Req {
     size_t bytes = 0;
     Clock::time_point arrival;
     bool leopard = false;
+    bool admitted = false;  // taken off the queue by the scheduler
     bool done = false;
     int status = 0;
     int verify_ok = 0;
@@ -62,7 +64,10 @@ struct rsx_server {
     std::deque<uint64_t> queue, work;
     uint64_t next = 1;
     size_t inflight = 0, max_inflight = 0;
+    size_t queued_bytes = 0, queue_limit = 0;  // 0: no limit
     size_t outstanding = 0;  // submitted and not completed
+    uint64_t rejected = 0, cancelled = 0;
+    bool paused = false;
     uint64_t batches = 0, batched_requests = 0;
     uint64_t plan_ns = 0, device_ns = 0;  // host planning + launch; device time of the batches
     Slot slots[kSlots];
@@ -75,13 +80,13 @@ namespace {
 // Caller holds s->mu and notifies done_cv / cv once it has completed
 // everything at hand (one wake-up per retired batch, not per request).
 void complete(rsx_server *s, uint64_t t, int status, int verify_ok) {
-    Req &q = s->reqs[t];
+    Req &q = s->reqs.at(t);
     q.done = true;
     q.status = status;
     q.verify_ok = verify_ok;
     q.latency_ns = static_cast<uint64_t>(
         std::chrono::duration_cast<std::chrono::nanoseconds>(Clock::now() - q.arrival).count());
-    s->inflight -= q.bytes;
+    if (q.admitted) s->inflight -= q.bytes;
     --s->outstanding;
 }
 
@@ -101,7 +106,7 @@ void worker_loop(rsx_server *s) {
         }
         const uint64_t t = s->work.front();
         s->work.pop_front();
-        Req &q = s->reqs[t];
+        Req &q = s->reqs.at(t);
         rsx_codec *c = q.r.codec;
         std::vector<uint8_t *> sh = q.shards;
         std::vector<size_t> ln = q.lens;
@@ -137,7 +142,7 @@ void scheduler_loop(rsx_server *s) {
             else (void)hipGetLastError();
             for (const auto &ti : sl.tickets) {
                 int ok = 0;
-                if (s->reqs[ti.first].r.op == RSX_OP_VERIFY) rsx_batch_verify_result(sl.b, ti.second, &ok);
+                if (s->reqs.at(ti.first).r.op == RSX_OP_VERIFY) rsx_batch_verify_result(sl.b, ti.second, &ok);
                 complete(s, ti.first, RSX_OK, ok);
             }
             sl.tickets.clear();
@@ -153,15 +158,17 @@ void scheduler_loop(rsx_server *s) {
             }
         std::vector<uint64_t> admitted;
         size_t batch_bytes = 0;
-        while (free_slot && !s->queue.empty() && static_cast<int>(admitted.size()) < s->max_batch) {
+        while (free_slot && !s->paused && !s->queue.empty() && static_cast<int>(admitted.size()) < s->max_batch) {
             const uint64_t t = s->queue.front();
-            Req &q = s->reqs[t];
+            Req &q = s->reqs.at(t);
             if (s->inflight > 0 && s->inflight + q.bytes > s->budget) break;  // FIFO: wait for room
             // A batch takes at most an eighth of the budget, so under load the
             // next batch is planned while earlier ones run.
             if (!admitted.empty() && batch_bytes + q.bytes > s->budget / 8) break;
             batch_bytes += q.bytes;
             s->queue.pop_front();
+            q.admitted = true;
+            s->queued_bytes -= q.bytes;
             s->inflight += q.bytes;
             s->max_inflight = std::max(s->max_inflight, s->inflight);
             admitted.push_back(t);
@@ -169,7 +176,7 @@ void scheduler_loop(rsx_server *s) {
         if (!admitted.empty()) {
             std::vector<rsx_batch_item> items(admitted.size());
             for (size_t i = 0; i < admitted.size(); ++i) {
-                const Req &q = s->reqs[admitted[i]];
+                const Req &q = s->reqs.at(admitted[i]);
                 items[i] = {q.r.codec, q.r.op, q.r.nshards, q.shards.data(), q.lens.data()};
             }
             std::vector<int> status(admitted.size());
@@ -198,7 +205,7 @@ void scheduler_loop(rsx_server *s) {
             s->batched_requests += admitted.size();
             bool failed = false;
             for (size_t i = 0; i < admitted.size(); ++i) {
-                if (rc == RSX_OK && status[i] == RSX_ERR_NOT_SUPPORTED && s->reqs[admitted[i]].leopard) {
+                if (rc == RSX_OK && status[i] == RSX_ERR_NOT_SUPPORTED && s->reqs.at(admitted[i]).leopard) {
                     // Leopard shapes / alignments the batch kernel does not
                     // take run through the per-stripe call on the worker.
                     s->work.push_back(admitted[i]);
@@ -240,6 +247,15 @@ int submit_at(rsx_server *s, const rsx_request *req, Clock::time_point arrival, 
     q.arrival = arrival;
     q.leopard = req->codec->engine != 0;
     std::lock_guard<std::mutex> lock(s->mu);
+    if (q.bytes > s->budget) {
+        ++s->rejected;
+        return rsx::fail(RSX_ERR_OVER_BUDGET, "request larger than the server's HBM budget");
+    }
+    if (s->queue_limit && s->queued_bytes + q.bytes > s->queue_limit) {
+        ++s->rejected;
+        return rsx::fail(RSX_ERR_OVER_BUDGET, "server queue full");
+    }
+    s->queued_bytes += q.bytes;
     const uint64_t t = s->next++;
     s->reqs.emplace(t, std::move(q));
     s->queue.push_back(t);
@@ -289,12 +305,15 @@ int rsx_server_create(int device, size_t hbm_budget, int max_batch, rsx_server *
 
 int rsx_server_drain(rsx_server *s) {
     std::unique_lock<std::mutex> lock(s->mu);
-    s->done_cv.wait(lock, [&] { return s->outstanding == 0; });
+    // A paused server admits nothing, so queued requests would never finish.
+    s->done_cv.wait(lock, [&] { return s->outstanding == 0 || (s->paused && !s->queue.empty()); });
+    if (s->outstanding) return rsx::fail(RSX_ERR_INVALID_INPUT, "server paused with queued requests");
     return RSX_OK;
 }
 
 void rsx_server_free(rsx_server *s) {
     if (!s) return;
+    rsx_server_pause(s, 0);
     rsx_server_drain(s);
     {
         std::lock_guard<std::mutex> lock(s->mu);
@@ -314,6 +333,46 @@ void rsx_server_free(rsx_server *s) {
 
 int rsx_server_submit(rsx_server *s, const rsx_request *req, uint64_t *ticket) {
     return submit_at(s, req, Clock::now(), ticket);
+}
+
+int rsx_server_cancel(rsx_server *s, uint64_t ticket, int *cancelled) {
+    *cancelled = 0;
+    std::lock_guard<std::mutex> lock(s->mu);
+    const auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) return rsx::fail(RSX_ERR_INVALID_INPUT, "unknown ticket");
+    Req &q = it->second;
+    if (q.done || q.admitted) return RSX_OK;  // finished, or its batch is already planned
+    s->queue.erase(std::find(s->queue.begin(), s->queue.end(), ticket));
+    s->queued_bytes -= q.bytes;
+    ++s->cancelled;
+    complete(s, ticket, RSX_ERR_CANCELLED, 0);
+    *cancelled = 1;
+    notify_completed(s);
+    return RSX_OK;
+}
+
+int rsx_server_pause(rsx_server *s, int paused) {
+    {
+        std::lock_guard<std::mutex> lock(s->mu);
+        s->paused = paused != 0;
+    }
+    s->cv.notify_all();
+    s->done_cv.notify_all();  // a drain in progress re-checks its condition
+    return RSX_OK;
+}
+
+int rsx_server_set_queue_limit(rsx_server *s, size_t max_queued_bytes) {
+    std::lock_guard<std::mutex> lock(s->mu);
+    s->queue_limit = max_queued_bytes;
+    return RSX_OK;
+}
+
+int rsx_server_admission(const rsx_server *s, uint64_t *rejected, uint64_t *cancelled, uint64_t *queued_bytes) {
+    std::lock_guard<std::mutex> lock(s->mu);
+    *rejected = s->rejected;
+    *cancelled = s->cancelled;
+    *queued_bytes = s->queued_bytes;
+    return RSX_OK;
 }
 
 int rsx_server_poll(rsx_server *s, uint64_t ticket, int *done, int *status, int *verify_ok,
@@ -359,15 +418,21 @@ int rsx_server_run_trace(rsx_server *s, const rsx_request *reqs, const uint64_t 
             const auto left = at - Clock::now();
             if (left > std::chrono::microseconds(200)) std::this_thread::sleep_for(left / 2);
         }
-        const int rc = submit_at(s, &reqs[i], at, &tickets[i]);
-        if (rc) return rc;
+        // A rejected request keeps its error as its status and no ticket.
+        status[i] = submit_at(s, &reqs[i], at, &tickets[i]);
+        if (status[i] != RSX_OK) tickets[i] = 0;
     }
-    rsx_server_drain(s);
+    int rc = rsx_server_drain(s);
+    if (rc) return rc;
     Clock::time_point last = t0;
-    int rc = RSX_OK;
     {
         std::lock_guard<std::mutex> lock(s->mu);
         for (int i = 0; i < n; ++i) {
+            if (!tickets[i]) {
+                verify_ok[i] = 0;
+                latency_ns[i] = 0;
+                continue;
+            }
             const auto it = s->reqs.find(tickets[i]);
             if (it == s->reqs.end()) {  // already reported by a concurrent poll / wait
                 status[i] = RSX_ERR_INVALID_INPUT;

@@ -176,7 +176,7 @@ def test_server_submit_wait_budget_and_errors():
     reqs = _make_requests(rng, 40)
     solo_ok = [_run_solo(enc, op, solo) for enc, op, solo, _, _ in reqs]
     biggest = max(r[4][2] * (r[4][0] + r[4][1]) for r in reqs)
-    budget = 2 << 20
+    budget = biggest  # a request larger than the budget is rejected at submit
     srv = rs.Server(0, hbm_budget=budget, max_batch=8)
     tickets = [srv.submit(e, op, mix) for e, op, _, mix, _ in reqs]
     enc = rs.New(5, 2)
@@ -190,7 +190,70 @@ def test_server_submit_wait_budget_and_errors():
     assert srv.wait(t_bad)[0] == rs.ErrShardSize.code
     srv.drain()
     _, _, max_inflight = srv.stats()
-    assert 0 < max_inflight <= max(budget, biggest)
+    assert 0 < max_inflight <= budget
     with pytest.raises(rs.RSError):
         srv.submit(enc, rs.OP_ENCODE, bad[:6])
+    srv.close()
+
+
+def _encode_request(rng, k, m, size):
+    """An Encode request with stale parity: (encoder, shards, codeword)."""
+    ref = _codeword(rsref.Classic(k, m), k, m, size, rng)
+    host = ref[:k] + [rng.integers(0, 256, size, dtype=np.uint8) for _ in range(m)]
+    return rs.New(k, m), [_dev(h) for h in host], ref, host
+
+
+def test_server_cancel_pause_and_admission_rejection():
+    rng = np.random.default_rng(2323)
+    srv = rs.Server(0, hbm_budget=8 << 20, max_batch=8)
+    srv.pause()
+    reqs = [_encode_request(rng, 10, 4, 65536) for _ in range(6)]
+    tickets = [srv.submit(enc, rs.OP_ENCODE, sh) for enc, sh, _, _ in reqs]
+    assert srv.cancel(tickets[1]) and srv.cancel(tickets[4])
+    assert not srv.cancel(tickets[4])  # already complete (cancelled), not yet reported
+    per = 14 * 65536
+    assert srv.admission() == (0, 2, 4 * per)
+    # a request larger than the whole budget is rejected at submit
+    big_enc, big, _, _ = _encode_request(rng, 10, 4, 1 << 20)
+    with pytest.raises(rs.ErrOverBudget):
+        srv.submit(big_enc, rs.OP_ENCODE, big)
+    # the queue limit rejects what would take the queued bytes past it
+    srv.set_queue_limit(5 * per)
+    extra = _encode_request(rng, 10, 4, 65536)
+    t_extra = srv.submit(extra[0], rs.OP_ENCODE, extra[1])
+    with pytest.raises(rs.ErrOverBudget):
+        srv.submit(extra[0], rs.OP_ENCODE, _encode_request(rng, 10, 4, 65536)[1])
+    assert srv.admission() == (2, 2, 5 * per)
+    with pytest.raises(rs.ErrInvalidInput):
+        srv.drain()  # paused with requests queued: would never finish
+    srv.set_queue_limit(0)
+    srv.pause(False)
+    for i, (t, (enc, sh, ref, host)) in enumerate(zip(tickets, reqs)):
+        st, _, _ = srv.wait(t)
+        if i in (1, 4):
+            assert st == rs.ErrCancelled.code
+            for j in range(14):  # a cancelled request never ran: stale parity stays
+                assert np.array_equal(sh[j].cpu().numpy(), host[j])
+        else:
+            assert st == 0
+            for j in range(14):
+                assert np.array_equal(sh[j].cpu().numpy(), ref[j]), (i, j)
+    assert srv.wait(t_extra)[0] == 0
+    # a finished request cannot be cancelled; a reported ticket is unknown
+    enc, sh, ref, _ = _encode_request(rng, 6, 3, 4096)
+    t = srv.submit(enc, rs.OP_ENCODE, sh)
+    srv.drain()
+    assert not srv.cancel(t)
+    assert srv.wait(t)[0] == 0
+    with pytest.raises(rs.ErrInvalidInput):
+        srv.cancel(t)
+    # a trace with an oversized request: its status is the rejection
+    trace = [_encode_request(rng, 4, 2, 4096) for _ in range(3)]
+    trace.insert(1, _encode_request(rng, 10, 4, 1 << 20))
+    st, _, lat, _ = srv.run_trace([(e, rs.OP_ENCODE, sh) for e, sh, _, _ in trace], [0, 1000, 2000, 3000])
+    assert st == [0, rs.ErrOverBudget.code, 0, 0] and lat[1] == 0
+    for i in (0, 2, 3):
+        e, sh, ref, _ = trace[i]
+        for j in range(6):
+            assert np.array_equal(sh[j].cpu().numpy(), ref[j])
     srv.close()
