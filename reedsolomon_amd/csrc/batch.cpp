@@ -240,7 +240,7 @@ int build_launches(rsx_batch *b, uint32_t tile) {
     int err = reserve_grow(b->d_launch, task_bytes + std::max<size_t>(tt_bytes, 4));
     if (!err) err = b->h_launch.reserve(task_bytes + std::max<size_t>(tt_bytes, 4));
     if (err) return err;
-    std::memcpy(b->h_launch.ptr, all.data(), all.size() * sizeof(BatchTask));
+    if (!all.empty()) std::memcpy(b->h_launch.ptr, all.data(), all.size() * sizeof(BatchTask));
     if (tt_bytes) std::memcpy(b->h_launch.ptr + task_bytes, tile_task.data(), tt_bytes);
     RSX_HIP(hipMemcpyAsync(b->d_launch.ptr, b->h_launch.ptr, task_bytes + tt_bytes, hipMemcpyHostToDevice, b->stream));
     const BatchTask *dev_tasks = b->d_launch.as<BatchTask>();
@@ -709,8 +709,8 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
     uint8_t *const *dev_pool = reinterpret_cast<uint8_t *const *>(dev + o_ptrs);
     uint8_t *dev_aux = dev + o_aux;
     b->dev_ptrs = reinterpret_cast<const uint8_t *const *>(dev_pool);
-    std::memcpy(host + o_tab, htab.data(), htab.size() * 4);
-    std::memcpy(host + o_ptrs, b->pool.data(), b->pool.size() * sizeof(void *));
+    if (!htab.empty()) std::memcpy(host + o_tab, htab.data(), htab.size() * 4);
+    if (!b->pool.empty()) std::memcpy(host + o_ptrs, b->pool.data(), b->pool.size() * sizeof(void *));
     if (!laux.empty()) std::memcpy(host + o_aux, laux.data(), laux.size());
     for (auto &p : pend) b->tasks[p.mc][p.idx].tables = dev_tables + p.toff;
     // Leopard tasks: resolve pool / aux offsets to device addresses.

@@ -66,7 +66,7 @@ bool invert_in_place(uint8_t *aug, int n);
 int build_coding_matrix(int k, int m, int kind, bool fast_one_parity, const uint8_t *custom,
                         int custom_rows, int custom_cols, Matrix &out);
 
-// Split-nibble product tables for the device kernels.  A byte x is split into
+// Bit-group product tables for the device kernels.  A byte x is split into
 // bit groups [0:3), [3:6), [6:8); c*x is the XOR of three table lookups
 // c*(x & 7), c*(x & 0x38), c*(x & 0xC0), each a byte select of v_perm_b32.
 // Per coefficient: 5 dwords {lo0..3, lo4..7, mid0..3, mid4..7, hi0..3}.

@@ -142,22 +142,6 @@ unsigned grid_for(size_t size, int slice) {
 
 }  // namespace
 
-int leo_fused_slice(int bits, int rows, int regions, int step_rows, size_t size, size_t lds_budget) {
-    const int unit = bits == 16 ? 32 : 16;
-    auto bytes = [&](int slice) { return static_cast<size_t>(regions) * slice * (rows + 1); };
-    // Enough column units that one radix-4 step gives every lane an item
-    // while the shard still spans 512 slices (two per CU), at least 128 bytes
-    // per row read (64 where nothing wider fits) ...
-    int slice = 128;
-    while ((step_rows / 4) * (slice / unit) < kFB && slice < 4096 && size / (2 * slice) >= 512) slice *= 2;
-    // ... within the LDS budget.
-    while (slice > 64 && bytes(slice) > lds_budget) slice /= 2;
-    // GF(2^16) tiles of whole chunks that leave room for fewer than four
-    // workgroups per CU take one 32-byte unit per workgroup instead.
-    if (bits == 16 && slice >= 64 && bytes(64) > (40u << 10) && bytes(32) <= lds_budget) slice = 32;
-    return bytes(slice) <= lds_budget ? slice : 0;
-}
-
 template <class K>
 static hipError_t allow_lds(K kernel, size_t lds) {
     if (lds <= (64u << 10)) return hipSuccess;
@@ -165,39 +149,12 @@ static hipError_t allow_lds(K kernel, size_t lds) {
                                static_cast<int>(lds));
 }
 
-// 512 threads when LDS leaves room for one workgroup per CU and one radix-4
-// step over `rows` rows has that many items; otherwise 256 and several
-// workgroups per CU.
-static unsigned threads_for(int bits, int rows, int slice, size_t lds) {
-    const bool alone = lds > (80u << 10);
-    const int units = std::max(1, slice / (bits == 16 ? 32 : 16));
-    return alone && (rows / 4) * units >= kFBMax ? kFBMax : kFB;
-}
-
-// Only while two workgroups fit per CU: at n = 8192 (one 512-thread
-// workgroup per CU) the three-pass schedule is faster (1.79 vs 1.87 ms for
-// k=4096 m=1024 64 KiB with the passes in 8 column chunks, 1.55 ms in one);
-// at n = 4096 the quarter tile is 3-6% faster.
-int leo_quarter_slice(int n) {
-    const size_t lds = static_cast<size_t>(16) * (n + 1);
-    const unsigned threads = threads_for(16, n, 16, lds);
-    const size_t deriv = static_cast<size_t>(n) * 16;
-    return lds <= (80u << 10) && deriv <= static_cast<size_t>(kDerivMax) * 16 * threads ? 16 : 0;
-}
-
-int leo_quarter_encode_slice(int m, int groups) {
-    const size_t lds = static_cast<size_t>(16) * (m + 1);
-    const unsigned threads = threads_for(16, m, 16, lds);
-    const size_t acc = groups > 1 ? static_cast<size_t>(m) * 16 : 0;
-    return lds <= kLeoFusedEncodeLds && acc <= static_cast<size_t>(kDerivMax) * 16 * threads ? 16 : 0;
-}
-
 hipError_t leo_encode_fused(const LeoDev &f, const LeoEncodeArgs &a, hipStream_t s) {
     // Several groups accumulate m * slice bytes in registers (kDerivMax
     // 16-byte slots per thread).
     const size_t acc = a.groups > 1 ? static_cast<size_t>(a.m) * a.slice : 0;
     const size_t lds = static_cast<size_t>(a.slice) * (a.m + 1);
-    const unsigned threads = threads_for(f.bits, a.m, a.slice, lds);
+    const unsigned threads = leo_tile_threads(f.bits, a.m, a.slice, lds);
     if (acc > static_cast<size_t>(kDerivMax) * 16 * threads) return hipErrorInvalidValue;
     if (a.slice == 16 && (f.bits != 16 || lds > kLeoFusedEncodeLds)) return hipErrorInvalidValue;
     const bool small = acc <= kLeoDerivSmall;
@@ -250,7 +207,7 @@ hipError_t leo_decode_fused(const LeoDev &f, const LeoDecodeArgs &a, hipStream_t
     // The derivative holds n * slice / 16 slots in registers, kDerivMax per thread.
     const size_t deriv = static_cast<size_t>(a.n) * a.slice;
     const size_t lds = static_cast<size_t>(a.slice) * (a.n + 1);
-    const unsigned threads = threads_for(f.bits, a.n, a.slice, lds);
+    const unsigned threads = leo_tile_threads(f.bits, a.n, a.slice, lds);
     if (deriv > static_cast<size_t>(kDerivMax) * 16 * threads) return hipErrorInvalidValue;
     const bool small = deriv <= kLeoDerivSmall;
     hipError_t e;

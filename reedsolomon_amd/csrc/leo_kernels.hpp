@@ -78,6 +78,13 @@ constexpr size_t kLeoFusedDecodeLds = 64u << 10;
 // Reconstructs with n * slice within this many bytes run a kernel variant
 // that keeps half the derivative registers (more workgroups per CU).
 constexpr size_t kLeoDerivSmall = 32u << 10;
+// Threads per workgroup of the LDS-resident kernels (512 where one radix-4
+// step over the tile has that many items and LDS leaves room for one
+// workgroup per CU), and the uint4 slots per thread holding the derivative.
+constexpr int kLeoTileThreads = 256;
+constexpr int kLeoTileThreadsMax = 512;
+constexpr int kLeoDerivMax = static_cast<int>(kLeoFusedDecodeLds / 16 / kLeoTileThreads);
+unsigned leo_tile_threads(int bits, int rows, int slice, size_t lds);
 
 struct LeoEncodeArgs {
     const uint8_t *const *data;  // groups*m row pointers, group-major, null past row k
@@ -166,6 +173,8 @@ uint64_t leo_pass3_blocks(int P, const std::vector<int> &out_rows);
 // Row blocks of pass 3 to launch: one past the highest set bit.
 int leo_pass3_grid(uint64_t blocks);
 
+// Bytes per row per workgroup of the three-pass kernels: one 64-byte chunk.
+constexpr int kLeoPassSlice = 64;
 // Tile shapes for n rows; false when n is out of the multipass range.
 bool leo_multipass_shape(int n, int *C, int *P, int *A);
 // Column chunk (bytes) whose two work arrays stay within 512 MiB each.

@@ -41,7 +41,7 @@ constexpr int kMB = 256;     // threads per workgroup of pass 3 (z = u + L(v) in
 // keep 16 waves per CU busy (5% faster than 256 at n = 8192).
 constexpr int kMB1 = 512;    // threads per workgroup of pass 1
 constexpr int kMB2 = 512;    // threads per workgroup of pass 2
-constexpr int kSlice = 64;   // bytes per row per workgroup: one 64-byte chunk
+constexpr int kSlice = kLeoPassSlice;
 
 __device__ __forceinline__ uint8_t *wrow(uint8_t *work, int n, int s, int r) {
     return work + (static_cast<size_t>(s) * n + r) * kSlice;
@@ -497,18 +497,6 @@ __global__ __launch_bounds__(kMB) void k_enc_pass3_batch(const LeoDev f, const L
 
 }  // namespace
 
-void leo_pass_blocks(const LeoMultipassArgs &a, uint32_t blocks[3]) {
-    const uint32_t sl = static_cast<uint32_t>(a.slices);
-    blocks[0] = sl * static_cast<uint32_t>(a.n / a.C);
-    blocks[1] = sl * static_cast<uint32_t>(a.P / a.A);
-    blocks[2] = sl * static_cast<uint32_t>(a.n / a.P);
-}
-
-uint32_t leo_enc_pass_blocks(const LeoEncPassArgs &a, int pass) {
-    const uint32_t sl = static_cast<uint32_t>(a.slices);
-    return sl * static_cast<uint32_t>(pass == 1 ? a.m / a.C : pass == 2 ? a.P / a.A : a.m / a.P);
-}
-
 hipError_t leo_encode_multipass_batch(const LeoDev &f, const LeoEncPassTask *tasks, int ntasks, uint32_t blocks,
                                       int pass, int max_p, hipStream_t s) {
     if (f.bits != 16 || ntasks <= 0 || blocks == 0 || pass < 1 || pass > 3) return hipErrorInvalidValue;
@@ -526,41 +514,6 @@ hipError_t leo_encode_multipass_batch(const LeoDev &f, const LeoEncPassTask *tas
     else if (pass == 2) hipLaunchKernelGGL(k_enc_pass2_batch<F16>, dim3(blocks), dim3(kMB2), lds, s, f, tasks, ntasks);
     else hipLaunchKernelGGL(k_enc_pass3_batch<F16>, dim3(blocks), dim3(kMB), lds, s, f, tasks, ntasks);
     return hipGetLastError();
-}
-
-bool leo_multipass_shape(int n, int *C, int *P, int *A) {
-    if (n < 2048) return false;
-    int p = n >> 2;
-    while (p > 1024) p >>= 2;
-    const int J = n / p;
-    if (J > 64) return false;  // pass 3's row blocks fit one 64-bit mask (never for n <= 65536)
-    *C = 1024;
-    *P = p;
-    *A = 512 / J;
-    return true;
-}
-
-uint64_t leo_pass3_blocks(int P, const std::vector<int> &out_rows) {
-    uint64_t b = 0;
-    for (int r : out_rows) b |= uint64_t{1} << (r / P);  // n / P <= 64 (leo_multipass_shape)
-    return b;
-}
-
-int leo_pass3_grid(uint64_t blocks) {
-    int g = 0;
-    while (g < 64 && (blocks >> g) != 0) ++g;
-    return g;
-}
-
-// Column chunk of one pass sequence: each work array is at most 512 MiB, so
-// k=4096 m=1024 shards of 64 KiB run as one chunk (three launches per call).
-// Fewer, larger launches beat Infinity-Cache-sized chunks: 8 chunks of 8 KiB
-// (64 MiB arrays) took 1.69 ms for that reconstruct, one chunk 1.52 ms.
-size_t leo_multipass_chunk(int n, size_t size) {
-    size_t cw = (size_t{512} << 20) / static_cast<size_t>(n);
-    cw -= cw % kSlice;
-    cw = std::max<size_t>(cw, kSlice);
-    return std::min(cw, size);
 }
 
 hipError_t leo_decode_multipass(const LeoDev &f, const LeoMultipassArgs &a, hipStream_t s) {

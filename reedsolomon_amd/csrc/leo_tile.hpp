@@ -14,9 +14,9 @@ namespace leo_tile {
 
 using namespace leo;
 
-constexpr int kFB = 256;        // threads per workgroup (512 where a step has that many items)
-constexpr int kFBMax = 512;
-constexpr int kDerivMax = kLeoFusedDecodeLds / 16 / kFB;  // uint4 per thread in the derivative
+constexpr int kFB = kLeoTileThreads;      // threads per workgroup (512 where a step has that many items)
+constexpr int kFBMax = kLeoTileThreadsMax;
+constexpr int kDerivMax = kLeoDerivMax;    // uint4 per thread in the derivative
 
 // Work rows of one slice in LDS, slot-major: 16-byte slot s of row r sits at
 // base + (s * r1 + pos(r)) * 16 with r1 = rows + 1.  A GF(2^8) column unit
