@@ -73,7 +73,7 @@ def _parse():
     ap.add_argument("--hbm-budget-gib", type=float, default=16.0,
                     help="served stream: shard bytes admitted in flight")
     ap.add_argument("--only", choices=["rs10_4", "rs10_3", "mixed", "leopard16", "served", "stream", "reference",
-                                       "ceiling"],
+                                       "ceiling", "butterfly"],
                     help="run one secondary section alone and print its JSON (for profiling)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="with --encode-only: also copy 1 GiB HBM->HBM 5 times (counter calibration)")
@@ -335,6 +335,49 @@ def ceiling(args, torch, rs):
         _log(f"bench: HBM ceiling {name}: {gbps:.1f} GB/s")
     del buf
     torch.cuda.empty_cache()
+    return out
+
+
+def butterfly(args, torch, rs):
+    """GF(2^16) butterfly issue rates with operands in registers
+    (reedsolomon_amd/membench/butterfly_rates.hip): the library's VALU
+    formulation (v_perm_b32 byte selects, the Leopard kernels' own mul_add)
+    against bit-sliced MFMA formulations (int8 32x32x32, FP4 32x32x64, and
+    int8 with the least repacking between layers).  Element-butterflies per
+    second, best of three grids; the MFMA figures are upper bounds (no skew
+    matrices, no data exchange, no LDS)."""
+    path = os.path.join(os.path.dirname(os.path.abspath(rs.__file__)), "membench", "libmembench.so")
+    lib = ctypes.CDLL(path)
+    lib.butterfly_rates.argtypes = [ctypes.c_int, ctypes.c_uint, ctypes.c_uint, ctypes.c_void_p, ctypes.c_void_p]
+    side = torch.cuda.Stream()
+    stream = side.cuda_stream
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    # (name, iterations, element-butterflies per wave per iteration, MFMA ops per wave per iteration)
+    kinds = [("valu_perm", 2000, 64 * 4 * 16, 0), ("mfma_i8", 20000, 4 * 32, 4 * 32 * 32 * 32 * 2),
+             ("mfma_fp4", 20000, 4 * 64, 4 * 32 * 32 * 64 * 2), ("mfma_i8_repack", 5000, 4 * 32, 4 * 32 * 32 * 32 * 2)]
+    out = {}
+    for kind, (name, iters, bfly, ops) in enumerate(kinds):
+        best = None
+        for grid in (cus * 4, cus * 8, cus * 16):
+            buf = torch.empty(grid * 256, dtype=torch.int32, device="cuda")
+
+            def launch():
+                rc = lib.butterfly_rates(kind, grid, iters, buf.data_ptr(), stream)
+                if rc:
+                    raise RuntimeError(f"butterfly_rates error {rc}")
+            launch()
+            t = event_time(torch, stream, launch, 3)
+            waves = grid * 4
+            rate = waves * iters * bfly / t
+            if best is None or rate > best[0]:
+                best = (rate, grid, t, waves * iters * ops / t)
+        out[name] = {"element_butterflies_per_s": float(f"{best[0]:.4g}"), "grid": best[1],
+                     "avg_launch_us": round(best[2] * 1e6, 1)}
+        if ops:
+            out[name]["mfma_tops"] = round(best[3] / 1e12, 1)
+        _log(f"bench: butterfly {name}: {best[0]:.3g} element-butterflies/s")
+    out["mfma_over_valu"] = {k: round(out[k]["element_butterflies_per_s"] / out["valu_perm"]["element_butterflies_per_s"], 3)
+                             for k in ("mfma_i8", "mfma_fp4", "mfma_i8_repack")}
     return out
 
 
@@ -984,7 +1027,7 @@ def main() -> int:
     args = _parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                          "(multi-GPU sharding is described in DESIGN.md, not launched)"}))
+                          "(multi-GPU sharding is designed, not launched)"}))
         return 2
     import torch
     import reedsolomon_amd as rs
@@ -992,7 +1035,7 @@ def main() -> int:
     torch.cuda.set_device(0)
     if args.only:
         fn = {"rs10_4": rs10_4, "rs10_3": config0, "mixed": mixed, "leopard16": leopard, "served": served,
-              "stream": stream, "reference": reference_shapes, "ceiling": ceiling}[args.only]
+              "stream": stream, "reference": reference_shapes, "ceiling": ceiling, "butterfly": butterfly}[args.only]
         sec = fn(args, torch, rs)
         sec.pop("sample", None)
         print(json.dumps({"section": args.only, **sec}), flush=True)
@@ -1002,9 +1045,10 @@ def main() -> int:
         head.pop("sample", None)
         print(json.dumps({"kernel": "batch_pass<4>", **head}), flush=True)
         return 0
-    mix = leo = srv = r104 = c0 = strm = refs = ceil = None
+    mix = leo = srv = r104 = c0 = strm = refs = ceil = bfly = None
     if not args.quick:
         ceil = ceiling(args, torch, rs)
+        bfly = butterfly(args, torch, rs)
         r104 = rs10_4(args, torch, rs)
         c0 = config0(args, torch, rs)
         refs = reference_shapes(args, torch, rs)
@@ -1055,6 +1099,7 @@ def main() -> int:
         "reference_shapes": refs,
         "mixed_workload": mix,
         "leopard16": leo,
+        "gf16_butterfly_rates": bfly,
         "served": srv,
         "stream": strm,
         "cpu_baseline": cpu,

@@ -20,7 +20,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "librsx.so")
-MEMBENCH_SRC = os.path.join(HERE, "membench", "membench.hip")
+MEMBENCH_SRC = [os.path.join(HERE, "membench", f) for f in ("membench.hip", "butterfly_rates.hip")]
 MEMBENCH_LIB = os.path.join(HERE, "membench", "libmembench.so")
 ARCH = os.environ.get("RSX_OFFLOAD_ARCH", "gfx950")
 
@@ -60,7 +60,7 @@ def build(verbose: bool = False) -> str:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
     tmp = MEMBENCH_LIB + ".tmp"
-    cmd = [HIPCC, *COMMON, "-x", "hip", "-shared", MEMBENCH_SRC, "-o", tmp]
+    cmd = [HIPCC, *COMMON, "-x", "hip", "-shared", *MEMBENCH_SRC, "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {MEMBENCH_SRC}:\n{r.stderr}")
