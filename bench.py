@@ -552,6 +552,21 @@ def reference_shapes(args, torch, rs):
         _check(lib, lib.rsx_encode(enc.handle, ptrs, full, n))
         return buf, ptrs, full
 
+    mlib = _membench(rs)
+
+    def c_abi(enc, ptrs, full, n, size, calls=2000):
+        """Encode called `calls` times back to back from C (no interpreter in
+        the loop), wall clock to the last result: the per-call cost a C or
+        cgo caller sees."""
+        fn = ctypes.cast(lib.rsx_encode, ctypes.c_void_p)
+        _check(lib, mlib.membench_call_loop(fn, enc.handle, ptrs, full, n, 50))
+        _check(lib, lib.rsx_synchronize(enc.handle))
+        t0 = time.perf_counter()
+        _check(lib, mlib.membench_call_loop(fn, enc.handle, ptrs, full, n, calls))
+        _check(lib, lib.rsx_synchronize(enc.handle))
+        t = (time.perf_counter() - t0) / calls
+        return {"us_per_call": round(t * 1e6, 2), "gbps": round(n * size / t / 1e9, 2)}
+
     def timed(enc, calls):
         stream = lib.rsx_get_stream(enc.handle)
         for c in calls[:2]:
@@ -582,9 +597,11 @@ def reference_shapes(args, torch, rs):
             bop, blens = rs.OP_RECONSTRUCT, (lambda i, pats=pats: pats[i % len(pats)])
         t = timed(enc, calls)
         gbps = n * size / t / 1e9
+        cab = c_abi(enc, ptrs, full, n, size) if op == "encode" else None
         del buf
         bg = batched(k, m, size, enc, bop, blens)
         classic.append({"op": op, "k": k, "m": m, "shard_bytes": size, "gbps": round(gbps, 1),
+                        "c_abi_loop": cab,
                         "batched_gbps": bg, "reference_mbps": ref, "ratio": round(gbps * 1000 / ref, 1),
                         "batched_ratio": round(bg * 1000 / ref, 1) if bg else None})
     leo = []
@@ -607,6 +624,7 @@ def reference_shapes(args, torch, rs):
             t = timed(enc, [fn])
             row[name] = {"gbps": round(n * size / t / 1e9, 2), "reference_mbps": ref,
                          "ratio": round(n * size / t / 1e6 / ref, 2)}
+        row["encode"]["c_abi_loop"] = c_abi(enc, ptrs, full, n, size)
         del buf
         for name, fn, ref, bop, blens in cases:
             bg = batched(k, m, size, enc, bop, lambda i, blens=blens: blens)
@@ -908,25 +926,79 @@ def served(args, torch, rs):
     return out
 
 
+class _MemStreams(ctypes.Structure):
+    """reedsolomon_amd/membench/hostloops.hip MemStreams."""
+    _fields_ = [("buf", ctypes.POINTER(ctypes.c_void_p)), ("len", ctypes.POINTER(ctypes.c_size_t)),
+                ("pos", ctypes.POINTER(ctypes.c_size_t)), ("cap", ctypes.POINTER(ctypes.c_size_t))]
+
+
+def _membench(rs):
+    path = os.path.join(os.path.dirname(os.path.abspath(rs.__file__)), "membench", "libmembench.so")
+    mlib = ctypes.CDLL(path)
+    mlib.membench_call_loop.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_int, ctypes.c_int]
+    return mlib
+
+
 def stream(args, torch, rs):
-    """StreamEncoder RS(10,4): 10 x 64 MiB data streams from in-memory file
-    objects through the pinned-host / HBM pipeline (4 MiB blocks); the
-    figure includes the Python read/write callbacks and PCIe transfers."""
+    """StreamEncoder RS(10,4): 10 x 64 MiB data streams from host memory
+    through the pinned-host / HBM pipeline, parity back to host memory.  Two
+    sources: native memcpy callbacks (reedsolomon_amd/membench/hostloops.hip,
+    the pipeline's own rate) at 4 and 16 MiB blocks, and Python file objects
+    (io.BytesIO, what a Python caller of the shim sees).  All figures include
+    the PCIe transfers both ways."""
     import io
     k, m, size = 10, 4, 64 << 20
     rng = np.random.default_rng(SEED + 5)
-    data = [rng.integers(0, 256, size, dtype=np.uint8).tobytes() for _ in range(k)]
+    data = [rng.integers(0, 256, size, dtype=np.uint8) for _ in range(k)]
+    b = (k + m) * size
+    mlib = _membench(rs)
+    rd = rs._READ_FN(ctypes.cast(mlib.membench_mem_read, ctypes.c_void_p).value)
+    wr = rs._WRITE_FN(ctypes.cast(mlib.membench_mem_write, ctypes.c_void_p).value)
+    out = {"k": k, "m": m, "shard_bytes": size}
+    parity = None
+    for block in (4 << 20, 16 << 20):
+        enc = rs.NewStream(k, m, block_size=block)
+        outs = [np.empty(size, np.uint8) for _ in range(m)]
+
+        def native():
+            rbuf = (ctypes.c_void_p * k)(*[a.ctypes.data for a in data])
+            rlen = (ctypes.c_size_t * k)(*([size] * k))
+            rpos = (ctypes.c_size_t * k)()
+            wbuf = (ctypes.c_void_p * m)(*[a.ctypes.data for a in outs])
+            wcap = (ctypes.c_size_t * m)(*([size] * m))
+            wpos = (ctypes.c_size_t * m)()
+            rctx = _MemStreams(rbuf, rlen, rpos, None)
+            wctx = _MemStreams(wbuf, None, wpos, wcap)
+            es = ctypes.c_int(-1)
+            _check(rs.lib, rs.lib.rsx_stream_encode(enc._h, rd, ctypes.byref(rctx), wr, ctypes.byref(wctx),
+                                                    ctypes.byref(es)))
+            assert all(wpos[i] == size for i in range(m))
+        native()  # warm
+        reps = 3
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            native()
+        t = (time.perf_counter() - t0) / reps
+        out[f"native_{block >> 20}mib_blocks"] = {"encode_ms": round(t * 1e3, 1), "encode_gbps": round(b / t / 1e9, 2)}
+        if parity is None:
+            parity = [o.copy() for o in outs]
+        else:
+            assert all(np.array_equal(p, o) for p, o in zip(parity, outs)), "block sizes disagree"
+        _log(f"bench: stream encode {b / t / 1e9:.1f} GB/s (native callbacks, {block >> 20} MiB blocks)")
     enc = rs.NewStream(k, m)
-    enc.Encode([io.BytesIO(d) for d in data], [io.BytesIO() for _ in range(m)])  # warm
+    raw = [d.tobytes() for d in data]
+    sinks = [io.BytesIO() for _ in range(m)]
+    enc.Encode([io.BytesIO(d) for d in raw], sinks)  # warm, and the parity check
+    assert all(s.getvalue() == p.tobytes() for s, p in zip(sinks, parity)), "python and native parity differ"
     reps = 3
     t0 = time.perf_counter()
     for _ in range(reps):
-        enc.Encode([io.BytesIO(d) for d in data], [io.BytesIO() for _ in range(m)])
-    t_enc = (time.perf_counter() - t0) / reps
-    b = (k + m) * size
-    _log(f"bench: stream encode {b / t_enc / 1e9:.1f} GB/s (host streams)")
-    return dict(k=k, m=m, shard_bytes=size, block_bytes=4 << 20, encode_ms=round(t_enc * 1e3, 1),
-                encode_gbps=round(b / t_enc / 1e9, 2), note="host memory streams via Python file objects")
+        enc.Encode([io.BytesIO(d) for d in raw], [io.BytesIO() for _ in range(m)])
+    t_py = (time.perf_counter() - t0) / reps
+    out["python_bytesio"] = {"encode_ms": round(t_py * 1e3, 1), "encode_gbps": round(b / t_py / 1e9, 2)}
+    _log(f"bench: stream encode {b / t_py / 1e9:.1f} GB/s (Python file objects)")
+    return out
 
 
 def _cpu_model():

@@ -20,7 +20,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "librsx.so")
-MEMBENCH_SRC = [os.path.join(HERE, "membench", f) for f in ("membench.hip", "butterfly_rates.hip")]
+MEMBENCH_SRC = [os.path.join(HERE, "membench", f) for f in ("membench.hip", "butterfly_rates.hip", "hostloops.hip")]
 MEMBENCH_LIB = os.path.join(HERE, "membench", "libmembench.so")
 ARCH = os.environ.get("RSX_OFFLOAD_ARCH", "gfx950")
 
