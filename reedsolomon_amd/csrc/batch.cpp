@@ -80,7 +80,7 @@ struct rsx_batch {
     // Work arrays of a plan's three-pass Leopard items (two of n x shard
     // bytes each); items past the cap get ErrNotSupported.
     static constexpr size_t kPassWorkCap = size_t{4} << 30;
-    static constexpr bool kSpreadClasses = true;
+    bool spread_classes = true;  // off: every class in order on `stream`
     hipStream_t cls[kClassStreams] = {};
     hipEvent_t cls_join[kClassStreams] = {};
     int cus = 0;
@@ -263,7 +263,7 @@ int enqueue(rsx_batch *b, hipStream_t s) {
     const bool leo = !b->leo_tasks[0].empty() || !b->leo_tasks[1].empty() || !b->leo_tasks[2].empty() ||
                      !b->mp_tasks.empty() || !b->emp_launches.empty();
     const bool fork = leo && !b->launches.empty();
-    const int spread = rsx_batch::kSpreadClasses
+    const int spread = b->spread_classes
                            ? std::min<int>(static_cast<int>(b->launches.size()) - 1, rsx_batch::kClassStreams)
                            : 0;
     if (fork || spread > 0) RSX_HIP(hipEventRecord(b->fork, s));
@@ -374,6 +374,30 @@ int autotune(rsx_batch *b) {
 
 }  // namespace
 
+namespace rsx {
+
+int batch_reserve(rsx_batch *b, size_t bytes, int max_items) {
+    (void)hipSetDevice(b->device);
+    RSX_HIP(hipStreamSynchronize(b->stream));
+    int err = reserve_grow(b->d_blob, bytes);
+    if (!err) err = reserve_grow(b->d_launch, bytes);
+    if (!err) err = b->h_blob.reserve(bytes);
+    if (!err) err = b->h_launch.reserve(bytes);
+    if (!err) err = reserve_grow(b->d_flags, sizeof(uint32_t) * std::max(max_items, 1));
+    if (!err && b->h_flags_cap < static_cast<size_t>(max_items)) {
+        if (b->h_flags) (void)hipHostFree(b->h_flags);
+        b->h_flags = nullptr;
+        b->h_flags_cap = 0;
+        RSX_HIP(hipHostMalloc(reinterpret_cast<void **>(&b->h_flags), sizeof(uint32_t) * std::max(max_items, 1)));
+        b->h_flags_cap = static_cast<size_t>(max_items);
+    }
+    return err;
+}
+
+void batch_set_class_streams(rsx_batch *b, bool on) { b->spread_classes = on; }
+
+}  // namespace rsx
+
 extern "C" {
 
 int rsx_batch_create(int device, rsx_batch **out) {
@@ -474,6 +498,7 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
     // seen and uploaded once; keyed by (host table, codec, decode-cache
     // epoch) so a table freed by a cache flush is never matched again.
     std::map<std::tuple<const void *, const rsx_codec *, uint64_t>, size_t> table_off;
+    const std::pair<const std::tuple<const void *, const rsx_codec *, uint64_t>, size_t> *last_table = nullptr;
     std::vector<std::shared_ptr<const DecodePlan>> held_plans;  // decode plans read by this plan
     std::vector<uint32_t> htab;
     size_t fill_off = 0;
@@ -485,6 +510,8 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
     };
     std::vector<Pending> pend;
     DeviceRanges ranges;
+    std::vector<const uint8_t *> in;  // one item's inputs / outputs (reused)
+    std::vector<uint8_t *> out;
 
     for (int it = 0; it < n; ++it) {
         const rsx_batch_item &item = items[it];
@@ -506,7 +533,8 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
                 continue;
             }
             LeoItemPlan lp;
-            const int e = leopard_plan_item(c, item.op, item.shards, item.lens, item.nshards, b->pool, laux, lp, fl);
+            const int e = leopard_plan_item(c, item.op, item.shards, item.lens, item.nshards, b->pool, laux, lp, fl,
+                                            ranges);
             if (e) {
                 if (fl) std::memset(fl, 0, c->total);
                 status[it] = e;
@@ -537,8 +565,8 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
         }
         const int k = c->k, total = c->total;
         const size_t *lens = item.lens;
-        std::vector<const uint8_t *> in;
-        std::vector<uint8_t *> out;
+        in.clear();
+        out.clear();
         const uint32_t *tables = nullptr;  // host copy of the item's tables
         uint64_t epoch = ~uint64_t{0};
         int mtot = 0;
@@ -609,16 +637,14 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
         {
             // Batched items run on device memory only; a missing or host
             // buffer becomes this item's status, never a kernel access.
-            std::vector<const void *> ptrs(in.begin(), in.end());
             bool missing = false;
             for (const uint8_t *p : in) missing |= p == nullptr;
-            for (size_t i = 0; i < out.size(); ++i) {
-                missing |= out[i] == nullptr;
-                ptrs.push_back(out[i]);
-            }
+            for (const uint8_t *p : out) missing |= p == nullptr;
             bool host = false;
-            if (!missing)
-                for (const void *p : ptrs) host |= !ranges.is_device(p);
+            if (!missing) {
+                for (const uint8_t *p : in) host |= !ranges.is_device(p);
+                for (const uint8_t *p : out) host |= !ranges.is_device(p);
+            }
             if (missing) err_item = fail(RSX_ERR_INVALID_INPUT, "no buffer for a shard of a batched item");
             else if (host) err_item = fail(RSX_ERR_MIXED_MEMORY, "batched items need shards in device memory");
             if (err_item) {
@@ -633,12 +659,15 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
         for (auto *p : in) bits |= reinterpret_cast<uintptr_t>(p);
         for (auto *p : out) bits |= reinterpret_cast<uintptr_t>(p);
         const auto tkey = std::make_tuple(static_cast<const void *>(tables), static_cast<const rsx_codec *>(c), epoch);
-        auto tf = table_off.find(tkey);
-        if (tf == table_off.end()) {
-            tf = table_off.emplace(tkey, htab.size()).first;
-            htab.insert(htab.end(), tables, tables + static_cast<size_t>(k) * mtot * 5);
+        if (!last_table || tkey != last_table->first) {  // consecutive items often share tables
+            auto tf = table_off.find(tkey);
+            if (tf == table_off.end()) {
+                tf = table_off.emplace(tkey, htab.size()).first;
+                htab.insert(htab.end(), tables, tables + static_cast<size_t>(k) * mtot * 5);
+            }
+            last_table = &*tf;
         }
-        const size_t toff = tf->second;
+        const size_t toff = last_table->second;
         const uint32_t in_off = static_cast<uint32_t>(b->pool.size());
         b->pool.insert(b->pool.end(), in.begin(), in.end());
         for (int o0 = 0; o0 < mtot; o0 += 8) {

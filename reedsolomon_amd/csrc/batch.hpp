@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "../../include/rsx.h"
+
 #include <cstddef>
 #include <cstdint>
 
@@ -49,5 +51,15 @@ hipError_t launch_batch_compare(const CompareEntry *entries, int n, uint32_t *fl
 
 // Launches the persistent pass for tasks of exactly m_class outputs.
 hipError_t launch_batch_pass(const BatchLaunch &L, int m_class, unsigned grid, hipStream_t s);
+
+// Request-loop settings of a batch (no C-ABI counterpart).  batch_reserve
+// sizes the plan's staging and device buffers to `bytes` each and its Verify
+// flags to max_items, so later plans up to that size never regrow them (a
+// regrowth frees memory, which waits for the whole device).
+// batch_set_class_streams(false) runs every class in order on the batch's
+// stream: no fork / join events per launch, for callers that already run
+// several batches side by side.
+int batch_reserve(rsx_batch *b, size_t bytes, int max_items);
+void batch_set_class_streams(rsx_batch *b, bool on);
 
 }  // namespace rsx

@@ -701,7 +701,7 @@ size_t aux_append(std::vector<uint8_t> &aux, const void *src, size_t bytes) {
 
 int leopard_plan_item(rsx_codec *c, int op, uint8_t *const *shards, const size_t *lens, int nshards,
                       std::vector<const uint8_t *> &pool, std::vector<uint8_t> &aux, LeoItemPlan &out,
-                      uint8_t *filled) {
+                      uint8_t *filled, DeviceRanges &ranges) {
     const int k = c->k, p = c->m, total = c->total;
     LeopardState &s = *c->leo;
     out = LeoItemPlan{};
@@ -735,7 +735,6 @@ int leopard_plan_item(rsx_codec *c, int op, uint8_t *const *shards, const size_t
     const bool recover_all = op != RSX_OP_RECONSTRUCT_DATA;
     const int end = recover_all ? total : k;
     // Device memory, 16-byte aligned, a buffer for every output.
-    DeviceRanges ranges;
     uintptr_t bits = 0;
     for (int i = 0; i < total; ++i) {
         const bool used = !erased[i] || (rec && i < end) || op == RSX_OP_ENCODE;

@@ -56,9 +56,10 @@ struct LeoItemPlan {
 // Returns the item's status; RSX_ERR_NOT_SUPPORTED where the batched form
 // does not apply (shards not 16-byte aligned).  GF(2^16) reconstructs with
 // 8192 or more work rows and encodes above the tiles are planned as three-pass
-// items (kinds 3 and 4).
+// items (kinds 3 and 4).  `ranges` caches the device allocations the plan
+// has already classified.
 int leopard_plan_item(rsx_codec *c, int op, uint8_t *const *shards, const size_t *lens, int nshards,
                       std::vector<const uint8_t *> &pool, std::vector<uint8_t> &aux, LeoItemPlan &out,
-                      uint8_t *filled);
+                      uint8_t *filled, DeviceRanges &ranges);
 
 }  // namespace rsx
