@@ -3,8 +3,9 @@
 // run their device semantics on the host, byte by byte, reading the plan's
 // task lists, tile table and pointer arrays exactly as the kernels do, so
 // ASan checks every range the planners hand to a launch and the results can
-// be checked.  The Leopard launchers only check their arguments: the host
-// build covers the Leopard planners, not the transforms.
+// be checked.  The Leopard error locators are computed with the host field's
+// transforms; the other Leopard launchers only check their arguments: the
+// host build covers the Leopard planners, not the transforms.
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -12,6 +13,9 @@
 #include "../batch.hpp"
 #include "../kernels.hpp"
 #include "../leo_kernels.hpp"
+#include "../leopard_field.hpp"
+
+#include <vector>
 
 namespace rsx {
 
@@ -111,6 +115,29 @@ hipError_t leo_store_rows(const LeoDev &, const LeoWork &, const int *rows, cons
                           size_t, int count, hipStream_t) {
     return (rows && dst) || count == 0 ? hipSuccess : hipErrorInvalidValue;
 }
+hipError_t leo_error_locators_device(const LeoDev &f, const uint8_t *ind, int mtrunc, int n, const int *rows,
+                                     int nrows, uint16_t *errlocs, uint16_t *logs, hipStream_t) {
+    // The host field's transforms on the same input (the device kernel's
+    // arithmetic is LeoField::fwht's).
+    if (f.bits != 16 || !f.walsh || mtrunc <= 0 || mtrunc > 65536 || n <= 0 || n > 65536 || nrows < 0)
+        return hipErrorInvalidValue;
+    const LeoField &lf = leo_field(16);
+    std::vector<uint16_t> e(65536, 0);
+    for (int i = 0; i < mtrunc; ++i) e[i] = ind[i];
+    lf.fwht(e.data(), mtrunc);
+    for (int i = 0; i < 65536; ++i) {
+        uint32_t x = static_cast<uint32_t>(e[i]) * f.walsh[i];
+        x = (x & 0xFFFFu) + (x >> 16);
+        x = (x & 0xFFFFu) + (x >> 16);
+        if (x >= 0xFFFFu) x -= 0xFFFFu;
+        e[i] = static_cast<uint16_t>(x);
+    }
+    lf.fwht(e.data(), 65536);
+    for (int i = 0; i < n; ++i) errlocs[i] = e[i];
+    for (int q = 0; q < nrows; ++q) logs[q] = static_cast<uint16_t>(0xFFFFu - e[rows[q]]);
+    return hipSuccess;
+}
+
 hipError_t leo_gf16_mul_xor(const LeoDev &, const Gf16MulXor &a, hipStream_t) {
     return a.in && a.nout >= 0 && a.nout <= 8 ? hipSuccess : hipErrorInvalidValue;
 }

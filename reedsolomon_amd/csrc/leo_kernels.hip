@@ -158,6 +158,74 @@ dim3 grid2(size_t units, int rows) {
 
 }  // namespace
 
+namespace {
+
+constexpr int kErrThreads = 1024;
+constexpr unsigned kOrder16 = 65536;
+
+// In-place Walsh-Hadamard transform mod 65535 over e[0, 65536), radix-4 steps
+// (dist 1, 4, 16, ...), only blocks that start below mtrunc: the arithmetic of
+// LeoField::fwht (32-bit sums folded to 16 bits), so the results are equal.
+__device__ void fwht_lds(uint16_t *e, int mtrunc) {
+    constexpr uint32_t mask = 0xFFFFu;
+    for (int dist = 1; dist * 4 <= static_cast<int>(kOrder16); dist *= 4) {
+        const int dist4 = dist * 4;
+        const int groups = (mtrunc + dist4 - 1) / dist4 * dist;
+        for (int g = threadIdx.x; g < groups; g += kErrThreads) {
+            const int r = g / dist * dist4 + g % dist;
+            const uint32_t t0 = e[r], t1 = e[r + dist], t2 = e[r + 2 * dist], t3 = e[r + 3 * dist];
+            uint32_t s01 = t0 + t1, d01 = t0 - t1, s23 = t2 + t3, d23 = t2 - t3;
+            s01 = (s01 + (s01 >> 16)) & mask;
+            d01 = (d01 + (d01 >> 16)) & mask;
+            s23 = (s23 + (s23 >> 16)) & mask;
+            d23 = (d23 + (d23 >> 16)) & mask;
+            const uint32_t a = s01 + s23, c = s01 - s23, b = d01 + d23, d = d01 - d23;
+            e[r] = static_cast<uint16_t>((a + (a >> 16)) & mask);
+            e[r + 2 * dist] = static_cast<uint16_t>((c + (c >> 16)) & mask);
+            e[r + dist] = static_cast<uint16_t>((b + (b >> 16)) & mask);
+            e[r + 3 * dist] = static_cast<uint16_t>((d + (d >> 16)) & mask);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kErrThreads) void k_error_locators(const uint16_t *walsh, const uint8_t *ind,
+                                                                int mtrunc, int n, const int *rows, int nrows,
+                                                                uint16_t *errlocs, uint16_t *logs) {
+    extern __shared__ uint16_t e[];  // kOrder16 entries
+    for (int i = threadIdx.x; i < static_cast<int>(kOrder16); i += kErrThreads) e[i] = i < mtrunc ? ind[i] : 0;
+    __syncthreads();
+    fwht_lds(e, mtrunc);
+    // (e * logWalsh) mod 65535 by folding: 2^16 is 1 modulo 65535.
+    for (int i = threadIdx.x; i < static_cast<int>(kOrder16); i += kErrThreads) {
+        uint32_t x = static_cast<uint32_t>(e[i]) * walsh[i];
+        x = (x & 0xFFFFu) + (x >> 16);
+        x = (x & 0xFFFFu) + (x >> 16);
+        if (x >= 0xFFFFu) x -= 0xFFFFu;
+        e[i] = static_cast<uint16_t>(x);
+    }
+    __syncthreads();
+    fwht_lds(e, static_cast<int>(kOrder16));
+    for (int i = threadIdx.x; i < n; i += kErrThreads) errlocs[i] = e[i];
+    for (int q = threadIdx.x; q < nrows; q += kErrThreads) logs[q] = static_cast<uint16_t>(0xFFFFu - e[rows[q]]);
+}
+
+}  // namespace
+
+hipError_t leo_error_locators_device(const LeoDev &f, const uint8_t *ind, int mtrunc, int n, const int *rows,
+                                     int nrows, uint16_t *errlocs, uint16_t *logs, hipStream_t s) {
+    if (f.bits != 16 || !f.walsh || mtrunc <= 0 || mtrunc > static_cast<int>(kOrder16) || n <= 0 ||
+        n > static_cast<int>(kOrder16) || nrows < 0)
+        return hipErrorInvalidValue;
+    const size_t lds = kOrder16 * sizeof(uint16_t);
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_error_locators),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_error_locators, dim3(1), dim3(kErrThreads), lds, s, f.walsh, ind, mtrunc, n, rows, nrows,
+                       errlocs, logs);
+    return hipGetLastError();
+}
+
 hipError_t leo_gf16_mul_xor(const LeoDev &f, const Gf16MulXor &a, hipStream_t s) {
     const size_t units = a.len / F16::kUnitBytes;
     if (units == 0 || a.nout <= 0) return hipSuccess;

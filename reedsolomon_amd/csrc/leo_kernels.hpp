@@ -19,6 +19,7 @@ struct LeoDev {
     unsigned modulus = 0;
     const uint32_t *tables = nullptr; // (modulus+1) multiply records
     const uint16_t *skew = nullptr;   // fftSkew, `modulus` entries
+    const uint16_t *walsh = nullptr;  // logWalsh, modulus + 1 entries
 };
 
 struct LeoWork {
@@ -67,6 +68,15 @@ struct Gf16MulXor {
     size_t len;
 };
 hipError_t leo_gf16_mul_xor(const LeoDev &f, const Gf16MulXor &a, hipStream_t s);
+
+// Error locators of one erasure pattern on the device (leopard.go:496-510,
+// GF(2^16)): ind holds the mtrunc = m + k 0/1 erasure indicators in work-row
+// order (leo_erasure_indicator); the kernel runs the truncated Walsh-Hadamard
+// transform, the product with logWalsh and the full transform in LDS (one
+// workgroup, 128 KiB), then writes errlocs[0, n) and, for the nrows output
+// rows, the reveal logs logs[q] = modulus - errlocs[rows[q]] (leopard.go:640-646).
+hipError_t leo_error_locators_device(const LeoDev &f, const uint8_t *ind, int mtrunc, int n, const int *rows,
+                                     int nrows, uint16_t *errlocs, uint16_t *logs, hipStream_t s);
 
 // ---- LDS-resident transforms (leo_fused.hip) ----
 // Encode keeps one group's m rows of a column slice in LDS and, with several

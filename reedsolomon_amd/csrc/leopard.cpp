@@ -27,7 +27,7 @@ namespace rsx {
 
 // Field tables resident on one device (shared by every codec on it).
 struct LeoDeviceTables {
-    DeviceBuffer tables, skew;
+    DeviceBuffer tables, skew, walsh;
     LeoDev dev;
 };
 
@@ -46,22 +46,29 @@ static int device_tables(int device, int bits, hipStream_t stream, const LeoDev 
         if (err) return err;
         err = t->skew.reserve(f.skew.size() * 2 + 64);
         if (err) return err;
+        err = t->walsh.reserve(f.walsh.size() * 2);
+        if (err) return err;
         RSX_HIP(hipMemcpyAsync(t->tables.ptr, mt.data(), mt.size() * 4, hipMemcpyHostToDevice, stream));
         RSX_HIP(hipMemcpyAsync(t->skew.ptr, f.skew.data(), f.skew.size() * 2, hipMemcpyHostToDevice, stream));
+        RSX_HIP(hipMemcpyAsync(t->walsh.ptr, f.walsh.data(), f.walsh.size() * 2, hipMemcpyHostToDevice, stream));
         RSX_HIP(hipStreamSynchronize(stream));
         t->dev.bits = bits;
         t->dev.modulus = f.modulus;
         t->dev.tables = t->tables.as<uint32_t>();
         t->dev.skew = t->skew.as<uint16_t>();
+        t->dev.walsh = t->walsh.as<uint16_t>();
         it = g_leo_tables.emplace(key, std::move(t)).first;
     }
     *out = &it->second->dev;
     return RSX_OK;
 }
 
-// Error locators of one erasure pattern (host copy; batched plans copy it).
+// Error locators of one erasure pattern.  A GF(2^16) pattern first met by a
+// per-stripe reconstruct gets them from the device (leo_error_locators_device,
+// straight into d_blob); the host copy, which batched plans copy into their
+// own uploads, is computed when a plan first asks for it.
 struct LeoPattern {
-    std::vector<uint16_t> errlocs;  // n entries
+    std::vector<uint16_t> errlocs;  // n entries, or empty until a host copy is needed
     int input_count = 0;
     // What a per-stripe reconstruct reads, uploaded on its first use in one
     // allocation and one copy: the error locators, the work rows of the
@@ -377,7 +384,9 @@ int encode_device(rsx_codec *c, const std::vector<uint8_t *> &data, const std::v
     return RSX_OK;
 }
 
-int pattern_for(rsx_codec *c, const uint8_t *erased, bool recover_all, std::shared_ptr<LeoPattern> *out) {
+// Caller holds c->mu.  host_errlocs: the caller reads pat->errlocs.
+int pattern_for(rsx_codec *c, const uint8_t *erased, bool recover_all, std::shared_ptr<LeoPattern> *out,
+                bool host_errlocs = true) {
     LeopardState &s = *c->leo;
     const int k = c->k, p = c->m, total = c->total;
     std::string key(1 + (total + 7) / 8, '\0');
@@ -385,18 +394,21 @@ int pattern_for(rsx_codec *c, const uint8_t *erased, bool recover_all, std::shar
     for (int i = 0; i < total; ++i)
         if (erased[i]) key[1 + i / 8] |= static_cast<char>(1 << (i % 8));
     auto it = s.patterns.find(key);
-    if (it != s.patterns.end()) {
-        *out = it->second;
-        return RSX_OK;
+    if (it == s.patterns.end()) {
+        if (s.patterns.size() >= 4096) {
+            RSX_HIP(hipStreamSynchronize(c->stream));
+            s.patterns.clear();
+        }
+        auto pat = std::make_shared<LeoPattern>();
+        std::vector<uint8_t> ind;
+        pat->input_count = leo_erasure_indicator(k, p, erased, ind);
+        it = s.patterns.emplace(std::move(key), std::move(pat)).first;
     }
-    if (s.patterns.size() >= 4096) {
-        RSX_HIP(hipStreamSynchronize(c->stream));
-        s.patterns.clear();
-    }
-    auto pat = std::make_shared<LeoPattern>();
-    pat->errlocs = leo_error_locators(*s.field, k, p, erased, &pat->input_count);
-    s.patterns.emplace(std::move(key), pat);
-    *out = std::move(pat);
+    LeoPattern &pat = *it->second;
+    // Filled once, under c->mu, and never changed after: readers that got
+    // the pattern earlier never see it change.
+    if (host_errlocs && pat.errlocs.empty()) pat.errlocs = leo_error_locators(*s.field, k, p, erased, nullptr);
+    *out = it->second;
     return RSX_OK;
 }
 
@@ -408,7 +420,8 @@ int reconstruct_device(rsx_codec *c, const std::vector<uint8_t *> &shards, const
     const int k = c->k, p = c->m, total = c->total;
     const int m = ceil_pow2(p), n = ceil_pow2(m + k);
     std::shared_ptr<LeoPattern> pat;
-    int err = pattern_for(c, erased, recover_all, &pat);
+    // GF(2^16) error locators of a new pattern come from the device below.
+    int err = pattern_for(c, erased, recover_all, &pat, s.bits != 16);
     if (err) return err;
     // Row table for the scaled load: parity rows, zero rows, data rows, zero rows.
     std::vector<uint8_t *> tab(static_cast<size_t>(n), nullptr);
@@ -424,33 +437,44 @@ int reconstruct_device(rsx_codec *c, const std::vector<uint8_t *> &shards, const
         // Output rows, reveal logs (leopard.go:640-646) and row -> output
         // index depend only on the pattern: uploaded once.
         std::vector<int> rows, out_index(static_cast<size_t>(n), -1);
-        std::vector<uint16_t> logs;
         for (int i = 0; i < end; ++i) {
             if (!erased[i]) continue;
             const int pos = i >= k ? i - k : i + m;
             out_index[pos] = static_cast<int>(rows.size());
             rows.push_back(pos);
-            logs.push_back(static_cast<uint16_t>(s.field->modulus - pat->errlocs[pos]));
         }
         const int no = static_cast<int>(rows.size());
+        // Without a host copy, the error locators and reveal logs are
+        // computed into the blob by the device from the erasure indicator
+        // (one workgroup, ~0.35 ms of host transforms saved at n = 8192).
+        const bool on_device = pat->errlocs.empty();
+        std::vector<uint8_t> ind;
+        if (on_device) leo_erasure_indicator(k, p, erased, ind);
         const std::vector<uint32_t> need = leo_fft_need(n, rows);
-        // One buffer: errlocs (n), logs (no), rows (no), out_index (n), need.
+        // One buffer: errlocs (n), logs (no), rows (no), out_index (n), need, indicator.
         auto up = [](size_t b) { return (b + 255) / 256 * 256; };
-        const size_t o_err = 0, o_logs = up(pat->errlocs.size() * 2), o_rows = o_logs + up(logs.size() * 2),
+        const size_t o_err = 0, o_logs = up(static_cast<size_t>(n) * 2), o_rows = o_logs + up(rows.size() * 2),
                      o_idx = o_rows + up(rows.size() * 4), o_need = o_idx + up(out_index.size() * 4),
-                     total_bytes = o_need + up(std::max<size_t>(need.size(), 1) * 4);
+                     o_ind = o_need + up(std::max<size_t>(need.size(), 1) * 4), total_bytes = o_ind + up(ind.size());
         pat->host_blob.assign(total_bytes, 0);
         uint8_t *h = pat->host_blob.data();
-        std::memcpy(h + o_err, pat->errlocs.data(), pat->errlocs.size() * 2);
-        if (no) {
-            std::memcpy(h + o_logs, logs.data(), logs.size() * 2);
-            std::memcpy(h + o_rows, rows.data(), rows.size() * 4);
+        if (!on_device) {
+            std::memcpy(h + o_err, pat->errlocs.data(), static_cast<size_t>(n) * 2);
+            uint16_t *hl = reinterpret_cast<uint16_t *>(h + o_logs);
+            for (int q = 0; q < no; ++q) hl[q] = static_cast<uint16_t>(s.field->modulus - pat->errlocs[rows[q]]);
         }
+        if (no) std::memcpy(h + o_rows, rows.data(), rows.size() * 4);
         std::memcpy(h + o_idx, out_index.data(), out_index.size() * 4);
         if (!need.empty()) std::memcpy(h + o_need, need.data(), need.size() * 4);
+        if (!ind.empty()) std::memcpy(h + o_ind, ind.data(), ind.size());
         if ((err = pat->d_blob.reserve(total_bytes))) return err;
         RSX_HIP(hipMemcpyAsync(pat->d_blob.ptr, h, total_bytes, hipMemcpyHostToDevice, c->stream));
         uint8_t *d = pat->d_blob.as<uint8_t>();
+        if (on_device)
+            RSX_HIP(leo_error_locators_device(*s.dev, d + o_ind, static_cast<int>(ind.size()), n,
+                                              reinterpret_cast<const int *>(d + o_rows), no,
+                                              reinterpret_cast<uint16_t *>(d + o_err),
+                                              reinterpret_cast<uint16_t *>(d + o_logs), c->stream));
         pat->d_errlocs = reinterpret_cast<const uint16_t *>(d + o_err);
         pat->d_logs = reinterpret_cast<const uint16_t *>(d + o_logs);
         pat->d_rows = reinterpret_cast<const int *>(d + o_rows);

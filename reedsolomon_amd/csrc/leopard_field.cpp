@@ -142,19 +142,28 @@ const LeoField &leo_field(int bits) {
     return bits == 16 ? f16 : f8;
 }
 
+int leo_erasure_indicator(int k, int p, const uint8_t *erased, std::vector<uint8_t> &ind) {
+    const int m = ceil_pow2(p);
+    ind.assign(static_cast<size_t>(m + k), 0);
+    int ic = p;
+    for (int i = 0; i < p; ++i)
+        if (erased[k + i]) ind[i] = 1;
+    for (int i = p; i < m; ++i) ind[i] = 1;
+    for (int i = 0; i < k; ++i) {
+        if (erased[i]) ind[i + m] = 1;
+        else ic = m + i + 1;
+    }
+    return ic;
+}
+
 std::vector<uint16_t> leo_error_locators(const LeoField &f, int k, int p, const uint8_t *erased,
                                          int *input_count) {
     const int m = ceil_pow2(p);
     const int n = ceil_pow2(m + k);
+    std::vector<uint8_t> ind;
+    const int ic = leo_erasure_indicator(k, p, erased, ind);
     std::vector<uint16_t> e(f.order, 0);
-    int ic = p;
-    for (int i = 0; i < p; ++i)
-        if (erased[k + i]) e[i] = 1;
-    for (int i = p; i < m; ++i) e[i] = 1;
-    for (int i = 0; i < k; ++i) {
-        if (erased[i]) e[i + m] = 1;
-        else ic = m + i + 1;
-    }
+    for (size_t i = 0; i < ind.size(); ++i) e[i] = ind[i];
     f.fwht(e.data(), m + k);
     // (e * walsh) % modulus (leopard.go:506-508), reduced by folding: 2^bits
     // is 1 modulo 2^bits - 1, so no division per entry.

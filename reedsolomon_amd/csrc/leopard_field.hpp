@@ -42,6 +42,12 @@ const LeoField &leo_field(int bits);
 std::vector<uint16_t> leo_error_locators(const LeoField &f, int k, int p, const uint8_t *erased,
                                          int *input_count);
 
+// The first transform's input of leo_error_locators: ind[i] (m + k entries,
+// m = ceilPow2(p)) is 1 for an erased parity row, a padding row [p, m) or an
+// erased data row (at m + i).  Returns the IFFT's input count (one past the
+// last present data row, or p).
+int leo_erasure_indicator(int k, int p, const uint8_t *erased, std::vector<uint8_t> &ind);
+
 int ceil_pow2(int n);
 
 }  // namespace rsx

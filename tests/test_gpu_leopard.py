@@ -408,3 +408,35 @@ def test_one_leopard_codec_shared_by_threads(bits, k, m):
     for th in threads:
         th.join()
     assert not errors, errors[:5]
+
+
+@pytest.mark.parametrize("k,m,size", [(300, 100, 256), (2000, 50, 128), (4096, 1024, 64)])
+def test_new_patterns_error_locators_on_device(k, m, size):
+    # A GF(2^16) pattern first met by a per-call reconstruct gets its error
+    # locators and reveal logs from the device transform; a batch plan of the
+    # same pattern on another codec computes them on the host.  Both must
+    # rebuild the oracle's codeword exactly, for patterns from one erasure to
+    # the maximum (every data shard or every parity shard among them).
+    rng = np.random.default_rng(k + m)
+    n = k + m
+    data = [rng.integers(0, 256, size, dtype=np.uint8) for _ in range(k)]
+    _, ref = oracle_encode(16, k, m, data)
+    pats = [[0], list(range(m)), list(range(k, k + m)), sorted(rng.choice(n, m, replace=False).tolist()),
+            sorted(rng.choice(n, int(rng.integers(2, m)), replace=False).tolist())]
+    per_call = rs.New(k, m, opt(16))
+    batched = rs.New(k, m, opt(16))
+    for lost in pats:
+        shards = [None if i in lost else dev(ref[i]) for i in range(n)]
+        per_call.Reconstruct(shards)
+        for i in range(n):
+            assert np.array_equal(host(shards[i]), ref[i]), (lost[:4], i)
+        work = [None if i in lost else dev(ref[i]) for i in range(n)]
+        b = rs.Batch(0)
+        assert b.run([(batched, rs.OP_RECONSTRUCT, work)]) == [0]
+        for i in range(n):
+            assert np.array_equal(host(work[i]), ref[i]), ("batched", lost[:4], i)
+    # the same patterns again: the cached device blobs serve them
+    for lost in pats[:2]:
+        shards = [None if i in lost else dev(ref[i]) for i in range(n)]
+        per_call.Reconstruct(shards)
+        assert all(np.array_equal(host(shards[i]), ref[i]) for i in range(n))
