@@ -23,6 +23,7 @@ LIB = os.path.join(HERE, "librsx.so")
 MEMBENCH_SRC = [os.path.join(HERE, "membench", f) for f in ("membench.hip", "butterfly_rates.hip", "hostloops.hip")]
 MEMBENCH_LIB = os.path.join(HERE, "membench", "libmembench.so")
 ARCH = os.environ.get("RSX_OFFLOAD_ARCH", "gfx950")
+ROCM_LIB = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib")
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 
@@ -54,7 +55,8 @@ def build(verbose: bool = False) -> str:
     with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs,
+           f"-L{ROCM_LIB}", "-lrocprofiler-sdk-roctx"]  # trace ranges (trace.hpp)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")

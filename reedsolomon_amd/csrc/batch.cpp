@@ -16,6 +16,7 @@
 #include "codec.hpp"
 #include "kernels.hpp"
 #include "leopard.hpp"
+#include "trace.hpp"
 
 namespace rsx {
 int check_shards(const size_t *lens, int n, bool nilok);
@@ -469,6 +470,7 @@ int rsx_batch_get_tuning(const rsx_batch *b, int *tile_bytes, int *blocks_per_cu
 void *rsx_batch_stream(rsx_batch *b) { return b->stream; }
 
 int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status, uint8_t *filled) {
+    RSX_TRACE("rsx_batch_plan");
     (void)hipSetDevice(b->device);
     RSX_HIP(hipStreamSynchronize(b->stream));
     destroy_graph(b);
@@ -859,6 +861,7 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
 }
 
 int rsx_batch_launch_on(rsx_batch *b, void *hip_stream) {
+    RSX_TRACE("rsx_batch_launch");
     if (!b->planned) return fail(RSX_ERR_INVALID_INPUT, "batch has no plan");
     (void)hipSetDevice(b->device);
     hipStream_t s = hip_stream ? static_cast<hipStream_t>(hip_stream) : b->stream;

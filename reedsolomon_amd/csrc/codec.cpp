@@ -11,6 +11,7 @@
 #include "batch.hpp"
 #include "kernels.hpp"
 #include "leopard.hpp"
+#include "trace.hpp"
 
 namespace rsx {
 
@@ -350,6 +351,7 @@ int rsx_decode_cache_size(const rsx_codec *c) {
 
 // reedsolomon.go:686-702 Encode.
 int rsx_encode(rsx_codec *c, uint8_t *const *shards, const size_t *lens, int nshards) {
+    RSX_TRACE("rsx_encode");
     if (c->engine) return leopard_encode(c, shards, lens, nshards);
     if (nshards != c->total) return fail(RSX_ERR_TOO_FEW_SHARDS, "too few shards given");
     int err = check_shards(lens, nshards, false);
@@ -386,6 +388,7 @@ int rsx_encode(rsx_codec *c, uint8_t *const *shards, const size_t *lens, int nsh
 // reedsolomon.go:708-757 EncodeIdx: parity ^= column idx of the parity rows * data.
 int rsx_encode_idx(rsx_codec *c, const uint8_t *data, size_t len, int idx, uint8_t *const *parity,
                    const size_t *parity_lens, int nparity) {
+    RSX_TRACE("rsx_encode_idx");
     if (c->engine) return fail(RSX_ERR_NOT_SUPPORTED, "operation not supported");
     if (nparity != c->m) return fail(RSX_ERR_TOO_FEW_SHARDS, "too few shards given");
     if (nparity == 0) return RSX_OK;
@@ -423,6 +426,7 @@ int rsx_encode_idx(rsx_codec *c, const uint8_t *data, size_t len, int idx, uint8
 
 // reedsolomon.go:857-871 Verify + 1373-1387 checkSomeShards.
 int rsx_verify(rsx_codec *c, uint8_t *const *shards, const size_t *lens, int nshards, int *ok) {
+    RSX_TRACE("rsx_verify");
     *ok = 0;
     if (c->engine) return leopard_verify(c, shards, lens, nshards, ok);
     if (nshards != c->total) return fail(RSX_ERR_TOO_FEW_SHARDS, "too few shards given");
@@ -594,6 +598,7 @@ extern "C" {
 
 int rsx_reconstruct(rsx_codec *c, uint8_t *const *shards, const size_t *lens, int nshards,
                     uint8_t *filled) {
+    RSX_TRACE("rsx_reconstruct");
     std::memset(filled, 0, static_cast<size_t>(std::max(nshards, 0)));
     if (c->engine) return leopard_reconstruct(c, shards, lens, nshards, true, filled);
     return reconstruct_classic(c, shards, lens, nshards, false, nullptr, 0, filled);
@@ -601,6 +606,7 @@ int rsx_reconstruct(rsx_codec *c, uint8_t *const *shards, const size_t *lens, in
 
 int rsx_reconstruct_data(rsx_codec *c, uint8_t *const *shards, const size_t *lens, int nshards,
                          uint8_t *filled) {
+    RSX_TRACE("rsx_reconstruct_data");
     std::memset(filled, 0, static_cast<size_t>(std::max(nshards, 0)));
     if (c->engine) return leopard_reconstruct(c, shards, lens, nshards, false, filled);
     return reconstruct_classic(c, shards, lens, nshards, true, nullptr, 0, filled);
@@ -609,6 +615,7 @@ int rsx_reconstruct_data(rsx_codec *c, uint8_t *const *shards, const size_t *len
 // reedsolomon.go:1655-1660 ReconstructSome; leopard.go:378-383.
 int rsx_reconstruct_some(rsx_codec *c, uint8_t *const *shards, const size_t *lens, int nshards,
                          const uint8_t *required, int nrequired, uint8_t *filled) {
+    RSX_TRACE("rsx_reconstruct_some");
     std::memset(filled, 0, static_cast<size_t>(std::max(nshards, 0)));
     if (c->engine)
         return leopard_reconstruct(c, shards, lens, nshards, nrequired == c->total, filled);

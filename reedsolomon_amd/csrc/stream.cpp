@@ -14,6 +14,7 @@
 #include <cstring>
 
 #include "codec.hpp"
+#include "trace.hpp"
 
 using namespace rsx;
 
@@ -137,6 +138,7 @@ rsx_codec *rsx_stream_codec(rsx_stream *s) { return s->c; }
 // streaming.go:199-210 Encode.
 int rsx_stream_encode(rsx_stream *s, rsx_stream_read_fn rd, void *rctx, rsx_stream_write_fn wr, void *wctx,
                       int *err_stream) {
+    RSX_TRACE("rsx_stream_encode");
     *err_stream = -1;
     rsx_codec *c = s->c;
     const int k = c->k, m = c->m, total = c->total;
@@ -177,6 +179,7 @@ int rsx_stream_encode(rsx_stream *s, rsx_stream_read_fn rd, void *rctx, rsx_stre
 
 // streaming.go:430-455 Verify.
 int rsx_stream_verify(rsx_stream *s, rsx_stream_read_fn rd, void *rctx, int *ok, int *err_stream) {
+    RSX_TRACE("rsx_stream_verify");
     *ok = 0;
     *err_stream = -1;
     rsx_codec *c = s->c;
@@ -216,6 +219,7 @@ int rsx_stream_verify(rsx_stream *s, rsx_stream_read_fn rd, void *rctx, int *ok,
 // streaming.go:477-527 Reconstruct.
 int rsx_stream_reconstruct(rsx_stream *s, const uint8_t *valid, const uint8_t *fill, rsx_stream_read_fn rd,
                            void *rctx, rsx_stream_write_fn wr, void *wctx, int *err_stream) {
+    RSX_TRACE("rsx_stream_reconstruct");
     *err_stream = -1;
     rsx_codec *c = s->c;
     const int k = c->k, total = c->total;

@@ -12,6 +12,7 @@
 
 #include "codec.hpp"
 #include "kernels.hpp"
+#include "trace.hpp"
 
 using namespace rsx;
 
@@ -70,6 +71,7 @@ extern "C" {
 // reedsolomon.go:762-797 Update + 799-818 updateParityShards.
 int rsx_update(rsx_codec *c, uint8_t *const *shards, const size_t *lens, int nshards, uint8_t *const *new_data,
                const size_t *new_lens, int nnew) {
+    RSX_TRACE("rsx_update");
     if (c->engine) return fail(RSX_ERR_NOT_SUPPORTED, "operation not supported");
     const int k = c->k, m = c->m;
     if (nshards != c->total) return fail(RSX_ERR_TOO_FEW_SHARDS, "too few shards given");
@@ -134,6 +136,7 @@ int rsx_update(rsx_codec *c, uint8_t *const *shards, const size_t *lens, int nsh
 // reedsolomon.go:1464-1617 DecodeIdx.
 int rsx_decode_idx(rsx_codec *c, uint8_t *const *dst, const size_t *dst_lens, int ndst, const uint8_t *expect,
                    int nexpect, uint8_t *const *input, const size_t *input_lens, int ninput) {
+    RSX_TRACE("rsx_decode_idx");
     if (c->engine) return fail(RSX_ERR_NOT_SUPPORTED, "operation not supported");
     const int k = c->k, total = c->total;
     (void)hipSetDevice(c->device);
