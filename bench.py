@@ -944,7 +944,8 @@ def stream(args, torch, rs):
     """StreamEncoder RS(10,4): 10 x 64 MiB data streams from host memory
     through the pinned-host / HBM pipeline, parity back to host memory.  Two
     sources: native memcpy callbacks (reedsolomon_amd/membench/hostloops.hip,
-    the pipeline's own rate) at 4 and 16 MiB blocks, and Python file objects
+    the pipeline's own rate) at 4 and 16 MiB blocks, one shard at a time
+    and all shards at once (NewStreamC), and Python file objects
     (io.BytesIO, what a Python caller of the shim sees).  All figures include
     the PCIe transfers both ways."""
     import io
@@ -957,8 +958,8 @@ def stream(args, torch, rs):
     wr = rs._WRITE_FN(ctypes.cast(mlib.membench_mem_write, ctypes.c_void_p).value)
     out = {"k": k, "m": m, "shard_bytes": size}
     parity = None
-    for block in (4 << 20, 16 << 20):
-        enc = rs.NewStream(k, m, block_size=block)
+    for block, conc in ((4 << 20, False), (16 << 20, False), (4 << 20, True), (16 << 20, True)):
+        enc = rs.NewStreamC(k, m, conc, conc, block_size=block)
         outs = [np.empty(size, np.uint8) for _ in range(m)]
 
         def native():
@@ -980,12 +981,14 @@ def stream(args, torch, rs):
         for _ in range(reps):
             native()
         t = (time.perf_counter() - t0) / reps
-        out[f"native_{block >> 20}mib_blocks"] = {"encode_ms": round(t * 1e3, 1), "encode_gbps": round(b / t / 1e9, 2)}
+        key = f"native_{block >> 20}mib_blocks" + ("_concurrent" if conc else "")
+        out[key] = {"encode_ms": round(t * 1e3, 1), "encode_gbps": round(b / t / 1e9, 2)}
         if parity is None:
             parity = [o.copy() for o in outs]
         else:
             assert all(np.array_equal(p, o) for p, o in zip(parity, outs)), "block sizes disagree"
-        _log(f"bench: stream encode {b / t / 1e9:.1f} GB/s (native callbacks, {block >> 20} MiB blocks)")
+        _log(f"bench: stream encode {b / t / 1e9:.1f} GB/s (native callbacks, {block >> 20} MiB blocks"
+             f"{', concurrent' if conc else ''})")
     enc = rs.NewStream(k, m)
     raw = [d.tobytes() for d in data]
     sinks = [io.BytesIO() for _ in range(m)]

@@ -315,6 +315,12 @@ int rsx_stream_create(int data_shards, int parity_shards, const rsx_options *opt
                       rsx_stream **out);
 void rsx_stream_free(rsx_stream *s);
 rsx_codec *rsx_stream_codec(rsx_stream *s);
+/* WithConcurrentStreamReads / WithConcurrentStreamWrites (options.go:122-146,
+ * NewStreamC streaming.go:190): the read (write) callbacks of a block run for
+ * all shards at once on a pool of threads, so they must be safe to call for
+ * different shards concurrently.  Of several failing streams the lowest index
+ * is reported.  Off by default, as in the reference. */
+int rsx_stream_set_concurrency(rsx_stream *s, int reads, int writes);
 /* rsStream.Encode, streaming.go:199: reads shards 0..k-1, writes parity 0..m-1. */
 int rsx_stream_encode(rsx_stream *s, rsx_stream_read_fn rd, void *rctx, rsx_stream_write_fn wr, void *wctx,
                       int *err_stream);

@@ -88,6 +88,7 @@ lib.rsx_stream_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, c
 lib.rsx_stream_free.argtypes = [_P]
 lib.rsx_stream_free.restype = None
 lib.rsx_stream_encode.argtypes = [_P, _READ_FN, _P, _WRITE_FN, _P, ctypes.POINTER(ctypes.c_int)]
+lib.rsx_stream_set_concurrency.argtypes = [_P, ctypes.c_int, ctypes.c_int]
 lib.rsx_stream_verify.argtypes = [_P, _READ_FN, _P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
 lib.rsx_stream_reconstruct.argtypes = [_P, _U8, _U8, _READ_FN, _P, _WRITE_FN, _P, ctypes.POINTER(ctypes.c_int)]
 lib.rsx_set_stream.argtypes = [_P, _P]
@@ -201,10 +202,42 @@ def _noop(*_a, **_k):
 
 
 # CPU-tuning options of the reference have no meaning on the GPU.
-WithMaxGoroutines = WithMinSplitSize = WithAutoGoroutines = _noop
+WithMaxGoroutines = WithMinSplitSize = _noop
 WithSSSE3 = WithAVX2 = WithSSE2 = WithAVX512 = WithGFNI = WithAVXGFNI = _noop
-WithConcurrentStreams = WithConcurrentStreamReads = WithConcurrentStreamWrites = _noop
-WithStreamBlockSize = WithWorkAllocator = _noop
+WithWorkAllocator = _noop
+
+
+# Stream options (options.go:98-146, streaming.go:157-163, 178-183): kept on
+# the option struct's Python side and applied by NewStream.
+def WithAutoGoroutines(shard_size: int):
+    """Also the stream block size when WithStreamBlockSize is not given."""
+    def f(o):
+        o._shard_size = shard_size
+    return f
+
+
+def WithStreamBlockSize(n: int):
+    def f(o):
+        o._stream_bs = n
+    return f
+
+
+def WithConcurrentStreams(enabled: bool = True):
+    def f(o):
+        o._conc_reads = o._conc_writes = bool(enabled)
+    return f
+
+
+def WithConcurrentStreamReads(enabled: bool = True):
+    def f(o):
+        o._conc_reads = bool(enabled)
+    return f
+
+
+def WithConcurrentStreamWrites(enabled: bool = True):
+    def f(o):
+        o._conc_writes = bool(enabled)
+    return f
 
 
 def device_count() -> int:
@@ -1083,17 +1116,34 @@ class StreamEncoder:
 
 
 def NewStream(data_shards: int, parity_shards: int, *opts, block_size: int = 0) -> StreamEncoder:
-    """streaming.go:147 NewStream (classic engine; block_size 0 = 4 MiB)."""
+    """streaming.go:147 NewStream (classic engine).  The block size is
+    block_size, else WithStreamBlockSize, else a positive WithAutoGoroutines
+    shard size (streaming.go:157-163), else 4 MiB."""
     o = _Options()
     lib.rsx_options_default(ctypes.byref(o))
     for f in opts:
         f(o)
+    bs = block_size or getattr(o, "_stream_bs", 0)
+    if bs <= 0 and getattr(o, "_shard_size", 0) > 0:
+        bs = o._shard_size
     h = ctypes.c_void_p()
-    _check(lib.rsx_stream_create(data_shards, parity_shards, ctypes.byref(o), block_size, ctypes.byref(h)))
-    return StreamEncoder(h.value, data_shards, parity_shards)
+    _check(lib.rsx_stream_create(data_shards, parity_shards, ctypes.byref(o), max(bs, 0), ctypes.byref(h)))
+    st = StreamEncoder(h.value, data_shards, parity_shards)
+    reads, writes = getattr(o, "_conc_reads", False), getattr(o, "_conc_writes", False)
+    if reads or writes:
+        _check(lib.rsx_stream_set_concurrency(st._h, int(reads), int(writes)))
+    return st
+
+
+def NewStreamC(data_shards: int, parity_shards: int, conc_reads: bool, conc_writes: bool, *opts,
+               block_size: int = 0) -> StreamEncoder:
+    """streaming.go:190 NewStreamC: NewStream with concurrent reads / writes."""
+    return NewStream(data_shards, parity_shards, *opts, WithConcurrentStreamReads(conc_reads),
+                     WithConcurrentStreamWrites(conc_writes), block_size=block_size)
 
 
 __all__ = [n for n in dir() if n.startswith(("With", "Err", "New", "Encoder", "RSError"))] + [
-    "device_count", "lib", "LIB_PATH", "Batch", "Server", "LowLevel", "Inv", "NewStream", "StreamEncoder",
+    "device_count", "lib", "LIB_PATH", "Batch", "Server", "LowLevel", "Inv", "NewStream", "NewStreamC",
+    "StreamEncoder",
     "StreamReadError", "StreamWriteError", "OP_ENCODE", "OP_RECONSTRUCT",
     "OP_RECONSTRUCT_DATA", "OP_VERIFY"]

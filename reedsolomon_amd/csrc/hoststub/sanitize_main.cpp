@@ -10,7 +10,7 @@
 //     thread that pauses, resumes, cancels and reads the counters meanwhile,
 //     Leopard requests on the worker thread, and a trace with a rejected
 //     request;
-//   - the stream encoder over in-memory callbacks.
+//   - the stream encoder over in-memory callbacks, sequential and concurrent.
 // GF(2^8) results are checked (the host launchers compute them); Leopard
 // items are checked for status only.  Exit status 0 = every check passed.
 #include <hip/hip_runtime.h>
@@ -496,6 +496,18 @@ void streams(std::mt19937 &rng) {
     uint8_t valid[6] = {1, 0, 1, 1, 1, 1}, fill[6] = {0, 1, 0, 0, 0, 0};
     CHECK(rsx_stream_reconstruct(s, valid, fill, mem_read, &r, mem_write, &r, &es) == RSX_OK);
     CHECK(r.out[1] == m.in[1]);
+    // concurrent reads and writes: callbacks for all shards at once
+    CHECK(rsx_stream_set_concurrency(s, 1, 1) == RSX_OK);
+    MemStreams c;
+    c.in = m.in;
+    c.out.resize(6);
+    c.pos.assign(6, 0);
+    CHECK(rsx_stream_encode(s, mem_read, &c, mem_write, &c, &es) == RSX_OK);
+    CHECK(c.out[0] == m.out[0] && c.out[1] == m.out[1]);
+    MemStreams cv;
+    cv.in = v.in;
+    cv.pos.assign(6, 0);
+    CHECK(rsx_stream_verify(s, mem_read, &cv, &ok, &es) == RSX_OK && ok == 1);
     rsx_stream_free(s);
 }
 

@@ -4,6 +4,13 @@
 // the GPU encodes / verifies / reconstructs block b (H2D, kernel, D2H on the
 // codec's stream), the host reads block b+1 into the other pinned block.
 //
+// With concurrent reads or writes (WithConcurrentStreamReads / Writes,
+// options.go:122-146; streaming.go:334-420 cReadShards / cWriteShards) the
+// callbacks of one block run on a pool of threads, one shard per task, so a
+// caller's callbacks must then be safe to call for different shards at once;
+// of several failing streams the lowest index is reported (the reference
+// reports whichever goroutine it collects first).
+//
 // Block semantics follow streaming.go:211-330: every reader is read with
 // ReadFull semantics; readers that end early must agree on their length
 // (ErrShardSize otherwise); a common length of 0 is the end of the stream;
@@ -11,12 +18,79 @@
 // the stream index.  Output of block b is written before an error found
 // while reading block b+1 is reported, as the sequential reference does.
 #include <algorithm>
+#include <condition_variable>
 #include <cstring>
+#include <functional>
+#include <memory>
+#include <mutex>
+#include <thread>
 
 #include "codec.hpp"
 #include "trace.hpp"
 
 using namespace rsx;
+
+namespace {
+
+// Runs fn(0 .. n-1) on a fixed set of threads plus the caller; returns when
+// every task has finished.
+class ShardPool {
+  public:
+    explicit ShardPool(int threads) {
+        for (int t = 0; t < threads; ++t) th_.emplace_back([this] { loop(); });
+    }
+    ~ShardPool() {
+        {
+            std::lock_guard<std::mutex> lock(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto &t : th_) t.join();
+    }
+    void run(int n, const std::function<void(int)> &fn) {
+        std::unique_lock<std::mutex> lock(mu_);
+        fn_ = &fn;
+        next_ = 0;
+        n_ = n;
+        left_ = n;
+        ++gen_;
+        cv_.notify_all();
+        work(lock);
+        done_cv_.wait(lock, [&] { return left_ == 0; });
+        fn_ = nullptr;
+    }
+
+  private:
+    void work(std::unique_lock<std::mutex> &lock) {
+        while (fn_ && next_ < n_) {
+            const int i = next_++;
+            const std::function<void(int)> *fn = fn_;
+            lock.unlock();
+            (*fn)(i);
+            lock.lock();
+            if (--left_ == 0) done_cv_.notify_all();
+        }
+    }
+    void loop() {
+        std::unique_lock<std::mutex> lock(mu_);
+        uint64_t seen = 0;
+        for (;;) {
+            cv_.wait(lock, [&] { return stop_ || gen_ != seen; });
+            if (stop_) return;
+            seen = gen_;
+            work(lock);
+        }
+    }
+    std::vector<std::thread> th_;
+    std::mutex mu_;
+    std::condition_variable cv_, done_cv_;
+    const std::function<void(int)> *fn_ = nullptr;
+    int next_ = 0, n_ = 0, left_ = 0;
+    uint64_t gen_ = 0;
+    bool stop_ = false;
+};
+
+}  // namespace
 
 struct rsx_stream {
     rsx_codec *c = nullptr;
@@ -24,6 +98,8 @@ struct rsx_stream {
     uint8_t *host[2] = {nullptr, nullptr};  // pinned, total * bs each
     DeviceBuffer dev[2];                    // total * bs each
     hipEvent_t done[2] = {nullptr, nullptr};
+    bool conc_reads = false, conc_writes = false;
+    std::unique_ptr<ShardPool> pool;  // while either is set
 };
 
 namespace {
@@ -38,11 +114,20 @@ struct Block {
 int read_block(rsx_stream *s, int slot, const uint8_t *use, int nuse_first, int n, rsx_stream_read_fn rd, void *ctx,
                Block &b, int *err_stream) {
     b.lens.assign(static_cast<size_t>(n), 0);
+    std::vector<long long> got_all;
+    if (s->conc_reads) {
+        // streaming.go:334-380 cReadShards: every reader at once.
+        got_all.assign(static_cast<size_t>(n), 0);
+        s->pool->run(n, [&](int i) {
+            if (use && !use[i]) return;
+            got_all[i] = rd(ctx, i, s->host[slot] + static_cast<size_t>(i + nuse_first) * s->bs, s->bs);
+        });
+    }
     long long common = -1;
     for (int i = 0; i < n; ++i) {
         if (use && !use[i]) continue;
         uint8_t *buf = s->host[slot] + static_cast<size_t>(i + nuse_first) * s->bs;
-        const long long got = rd(ctx, i, buf, s->bs);
+        const long long got = s->conc_reads ? got_all[i] : rd(ctx, i, buf, s->bs);
         if (got < 0) {
             *err_stream = i;
             return fail(RSX_ERR_STREAM_READ, "error reading stream");
@@ -65,10 +150,19 @@ int read_block(rsx_stream *s, int slot, const uint8_t *use, int nuse_first, int 
 // streaming.go:248-267 writeShards.
 int write_shards(rsx_stream *s, int slot, const uint8_t *which, int first, int n, size_t len,
                  rsx_stream_write_fn wr, void *ctx, int *err_stream) {
+    std::vector<long long> put_all;
+    if (s->conc_writes) {
+        // streaming.go:382-420 cWriteShards: every writer at once.
+        put_all.assign(static_cast<size_t>(n), 0);
+        s->pool->run(n, [&](int i) {
+            if (which && !which[i]) return;
+            put_all[i] = wr(ctx, i, s->host[slot] + static_cast<size_t>(first + i) * s->bs, len);
+        });
+    }
     for (int i = 0; i < n; ++i) {
         if (which && !which[i]) continue;
         const uint8_t *buf = s->host[slot] + static_cast<size_t>(first + i) * s->bs;
-        const long long put = wr(ctx, i, buf, len);
+        const long long put = s->conc_writes ? put_all[i] : wr(ctx, i, buf, len);
         if (put < 0 || static_cast<size_t>(put) != len) {
             *err_stream = i;
             return fail(RSX_ERR_STREAM_WRITE, put < 0 ? "error writing stream" : "short write");
@@ -134,6 +228,14 @@ void rsx_stream_free(rsx_stream *s) {
 }
 
 rsx_codec *rsx_stream_codec(rsx_stream *s) { return s->c; }
+
+int rsx_stream_set_concurrency(rsx_stream *s, int reads, int writes) {
+    s->conc_reads = reads != 0;
+    s->conc_writes = writes != 0;
+    if ((s->conc_reads || s->conc_writes) && !s->pool)
+        s->pool = std::make_unique<ShardPool>(std::min(s->c->total, 16) - 1);
+    return RSX_OK;
+}
 
 // streaming.go:199-210 Encode.
 int rsx_stream_encode(rsx_stream *s, rsx_stream_read_fn rd, void *rctx, rsx_stream_write_fn wr, void *wctx,

@@ -111,3 +111,51 @@ def test_stream_split_join(size):
         enc.Join(io.BytesIO(), [io.BytesIO(s.tobytes()) for s in shards], per * k + 1)
     with pytest.raises(rs.ErrShortData):
         enc.Split(io.BytesIO(raw), [io.BytesIO() for _ in range(k)], 0)
+
+
+@pytest.mark.parametrize("reads,writes", [(True, True), (True, False), (False, True)])
+def test_concurrent_streams_match_sequential(reads, writes):
+    # streaming.go:334-420 cReadShards / cWriteShards (NewStreamC): the same
+    # bytes as the sequential streams; a failing reader or writer is a
+    # StreamReadError / StreamWriteError naming a failing stream (here the
+    # lowest index of the failing ones).
+    k, m, size = 12, 4, 300001
+    rng = np.random.default_rng(77)
+    data = [rng.integers(0, 256, size, dtype=np.uint8) for _ in range(k)]
+    ref = _encode_ref(k, m, data)
+    enc = rs.NewStreamC(k, m, reads, writes, rs.WithStreamBlockSize(65536))
+    par = [io.BytesIO() for _ in range(m)]
+    enc.Encode([io.BytesIO(d.tobytes()) for d in data], par)
+    assert all(par[i].getvalue() == ref[k + i].tobytes() for i in range(m))
+    assert enc.Verify([io.BytesIO(x.tobytes()) for x in ref])
+    valid = [None if i in (2, k + 1) else io.BytesIO(ref[i].tobytes()) for i in range(k + m)]
+    fill = [io.BytesIO() if i in (2, k + 1) else None for i in range(k + m)]
+    enc.Reconstruct(valid, fill)
+    assert fill[2].getvalue() == ref[2].tobytes() and fill[k + 1].getvalue() == ref[k + 1].tobytes()
+    if reads:
+        bad = [io.BytesIO(d.tobytes()) for d in data]
+        bad[7], bad[3] = _Fail(), _Fail()
+        with pytest.raises(rs.StreamReadError) as e:
+            enc.Encode(bad, [io.BytesIO() for _ in range(m)])
+        assert e.value.stream == 3
+    if writes:
+        outs = [io.BytesIO() for _ in range(m)]
+        outs[2] = _Fail()
+        with pytest.raises(rs.StreamWriteError) as e:
+            enc.Encode([io.BytesIO(d.tobytes()) for d in data], outs)
+        assert e.value.stream == 2
+
+
+def test_stream_block_size_options():
+    # WithStreamBlockSize, else a positive WithAutoGoroutines shard size, sets
+    # the block (streaming.go:157-163); any block size gives the same parity.
+    k, m, size = 6, 3, 100000
+    rng = np.random.default_rng(5)
+    data = [rng.integers(0, 256, size, dtype=np.uint8) for _ in range(k)]
+    ref = _encode_ref(k, m, data)
+    for opts in ((rs.WithStreamBlockSize(4096),), (rs.WithAutoGoroutines(10000),),
+                 (rs.WithAutoGoroutines(0), rs.WithStreamBlockSize(777))):
+        enc = rs.NewStream(k, m, *opts)
+        par = [io.BytesIO() for _ in range(m)]
+        enc.Encode([io.BytesIO(d.tobytes()) for d in data], par)
+        assert all(par[i].getvalue() == ref[k + i].tobytes() for i in range(m)), opts
