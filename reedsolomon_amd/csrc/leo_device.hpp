@@ -11,6 +11,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <type_traits>
 
 #include "leo_kernels.hpp"
 
@@ -181,9 +182,14 @@ struct F8 {
     }
 };
 
+// GF(2^16) records are read from global memory (the constant address space:
+// wave-uniform reads become scalar loads).  GF(2^8)'s 256 records (5 KiB) are
+// staged in LDS by the LDS-resident kernels (lds_tables), so their pointer is
+// kept generic.
 template <class F>
-__device__ __forceinline__ ConstU32 *table(const LeoDev &f, unsigned log_m) {
-    return as_const(f.tables) + static_cast<size_t>(log_m) * F::kTW;
+__device__ __forceinline__ auto table(const LeoDev &f, unsigned log_m) {
+    if constexpr (std::is_same<F, F8>::value) return f.tables + static_cast<size_t>(log_m) * F::kTW;
+    else return as_const(f.tables) + static_cast<size_t>(log_m) * F::kTW;
 }
 
 // galois_noasm.go:94-113: fftDIT2 = mulAdd then xor, ifftDIT2 = xor then

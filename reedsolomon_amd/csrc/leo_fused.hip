@@ -85,6 +85,23 @@ __device__ __forceinline__ void decode_slice(const LeoDev &f, const LeoDecodeArg
     __syncthreads();
 }
 
+// GF(2^8): the workgroup's copy of the 256 multiply records in LDS, read by
+// the per-lane products instead of global memory; GF(2^16) keeps its 5 MB of
+// records in global memory.
+template <class F>
+__device__ __forceinline__ LeoDev lds_tables(const LeoDev &f) {
+    if constexpr (std::is_same<F, F8>::value) {
+        __shared__ uint32_t t8[256 * F8::kTW];
+        for (int i = threadIdx.x; i < 256 * F8::kTW; i += blockDim.x) t8[i] = f.tables[i];
+        __syncthreads();
+        LeoDev l = f;
+        l.tables = t8;
+        return l;
+    } else {
+        return f;
+    }
+}
+
 // Slice index of workgroup-loop index g: the two unit slices of a chunk go
 // to one XCD.
 __device__ __forceinline__ size_t slice_start(uint32_t g, uint32_t nslices, int slice) {
@@ -93,16 +110,18 @@ __device__ __forceinline__ size_t slice_start(uint32_t g, uint32_t nslices, int 
 }
 
 template <class F, int MAXD>
-__global__ __launch_bounds__(kFBMax) void k_encode_fused(const LeoDev f, const LeoEncodeArgs a) {
+__global__ __launch_bounds__(kFBMax) void k_encode_fused(const LeoDev f0, const LeoEncodeArgs a) {
     extern __shared__ uint4 lds_raw[];
+    const LeoDev f = lds_tables<F>(f0);
     const uint32_t nslices = static_cast<uint32_t>((a.size + a.slice - 1) / a.slice);
     for (uint32_t g = blockIdx.x; g < nslices; g += gridDim.x)
         encode_slice<F, MAXD>(f, a, reinterpret_cast<uint8_t *>(lds_raw), slice_start(g, nslices, a.slice));
 }
 
 template <class F, int MAXD>
-__global__ __launch_bounds__(kFBMax) void k_decode_fused(const LeoDev f, const LeoDecodeArgs a) {
+__global__ __launch_bounds__(kFBMax) void k_decode_fused(const LeoDev f0, const LeoDecodeArgs a) {
     extern __shared__ uint4 lds_raw[];
+    const LeoDev f = lds_tables<F>(f0);
     const uint32_t nslices = static_cast<uint32_t>((a.size + a.slice - 1) / a.slice);
     for (uint32_t g = blockIdx.x; g < nslices; g += gridDim.x)
         decode_slice<F, MAXD>(f, a, reinterpret_cast<uint8_t *>(lds_raw), slice_start(g, nslices, a.slice));
@@ -112,9 +131,10 @@ __global__ __launch_bounds__(kFBMax) void k_decode_fused(const LeoDev f, const L
 // their task by binary search over the first-slice offsets, and run that
 // task's slice.  The task is uniform per block.
 template <class F, int MAXD>
-__global__ __launch_bounds__(kFB) void k_leo_batch(const LeoDev f, const LeoBatchTask *tasks, int ntasks,
+__global__ __launch_bounds__(kFB) void k_leo_batch(const LeoDev f0, const LeoBatchTask *tasks, int ntasks,
                                                    uint32_t nslices) {
     extern __shared__ uint4 lds_raw[];
+    const LeoDev f = lds_tables<F>(f0);
     uint8_t *lds = reinterpret_cast<uint8_t *>(lds_raw);
     for (uint32_t g = blockIdx.x; g < nslices; g += gridDim.x) {
         int lo = 0, hi = ntasks - 1;

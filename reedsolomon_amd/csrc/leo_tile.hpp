@@ -142,7 +142,7 @@ __device__ __forceinline__ unsigned uniform_u16(const uint16_t *p, int i) {
 template <class F>
 __device__ __forceinline__ void uniform_records(const LeoDev &f, unsigned lm01, unsigned lm02, unsigned lm23,
                                                 uint32_t (&rec)[3][F::kTW]) {
-    ConstU32 *t0 = table<F>(f, lm01), *t1 = table<F>(f, lm02), *t2 = table<F>(f, lm23);
+    const auto t0 = table<F>(f, lm01), t1 = table<F>(f, lm02), t2 = table<F>(f, lm23);
 #pragma unroll
     for (int o = 0; o < F::kTW; ++o) {
         rec[0][o] = t0[o];
@@ -170,7 +170,7 @@ __device__ __forceinline__ void group_records(const LeoDev &f, unsigned lm01, un
 #pragma unroll
     for (int o = 0; o < F::kTW; ++o) v[o] = 0;
     if (r < 3) {
-        ConstU32 *t = table<F>(f, r == 0 ? lm01 : r == 1 ? lm02 : lm23);
+        const auto t = table<F>(f, r == 0 ? lm01 : r == 1 ? lm02 : lm23);
 #pragma unroll
         for (int o = 0; o < F::kTW; ++o) v[o] = t[o];
     }
