@@ -85,17 +85,21 @@ __device__ __forceinline__ void decode_slice(const LeoDev &f, const LeoDecodeArg
     __syncthreads();
 }
 
-// GF(2^8): the workgroup's copy of the 256 multiply records in LDS, read by
-// the per-lane products instead of global memory; GF(2^16) keeps its 5 MB of
-// records in global memory.
+// GF(2^8): the workgroup's copy of the 256 multiply records and the 255
+// fftSkew entries in LDS, read by the butterflies instead of global memory;
+// GF(2^16) keeps its 5 MB of records and its skews in global memory.
 template <class F>
 __device__ __forceinline__ LeoDev lds_tables(const LeoDev &f) {
     if constexpr (std::is_same<F, F8>::value) {
+        constexpr int kSkew = 255, kSkewPad = 288;  // the pad reads as zero
         __shared__ uint32_t t8[256 * F8::kTW];
+        __shared__ uint16_t s8[kSkewPad];
         for (int i = threadIdx.x; i < 256 * F8::kTW; i += blockDim.x) t8[i] = f.tables[i];
+        for (int i = threadIdx.x; i < kSkewPad; i += blockDim.x) s8[i] = i < kSkew ? f.skew[i] : 0;
         __syncthreads();
         LeoDev l = f;
         l.tables = t8;
+        l.skew = s8;
         return l;
     } else {
         return f;

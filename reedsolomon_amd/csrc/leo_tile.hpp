@@ -6,6 +6,8 @@
 // leopard8.go likewise); only where the work rows live differs.
 #pragma once
 
+#include <type_traits>
+
 #include "leo_device.hpp"
 #include "leo_kernels.hpp"
 
@@ -135,6 +137,14 @@ __device__ __forceinline__ unsigned uniform_u16(const uint16_t *p, int i) {
     return (d >> ((a & 2) << 3)) & 0xFFFFu;
 }
 
+// A wave-uniform skew: GF(2^16) by a scalar load from global memory, GF(2^8)
+// from the workgroup's LDS copy (lds_tables).
+template <class F>
+__device__ __forceinline__ unsigned uniform_skew(const uint16_t *p, int i) {
+    if constexpr (std::is_same<F, F8>::value) return p[__builtin_amdgcn_readfirstlane(i)];
+    else return uniform_u16(p, i);
+}
+
 // The three multiply records of a wave-uniform butterfly group, read
 // together (scalar loads, one wait) before any product uses them.  A skew
 // equal to the modulus reads a record that is never used (that product is
@@ -209,8 +219,8 @@ __device__ __forceinline__ void step4_groups(const Tile &w, int base, int dshift
         typename F::E x0 = Lds<F>::load(w, i, u), x1 = Lds<F>::load(w, i + dist, u),
                       x2 = Lds<F>::load(w, i + 2 * dist, u), x3 = Lds<F>::load(w, i + 3 * dist, u);
         if constexpr (UNI) {
-            const unsigned lm01 = uniform_u16(skew, iend + bias), lm02 = uniform_u16(skew, iend + dist + bias),
-                           lm23 = uniform_u16(skew, iend + 2 * dist + bias);
+            const unsigned lm01 = uniform_skew<F>(skew, iend + bias), lm02 = uniform_skew<F>(skew, iend + dist + bias),
+                           lm23 = uniform_skew<F>(skew, iend + 2 * dist + bias);
             uint32_t rec[3][F::kTW];
             uniform_records<F>(f, lm01, lm02, lm23, rec);
             if (INV) {
