@@ -186,10 +186,12 @@ struct F8 {
 // wave-uniform reads become scalar loads).  GF(2^8)'s 256 records (5 KiB) are
 // staged in LDS by the LDS-resident kernels (lds_tables), so their pointer is
 // kept generic.
+// Logs are below 2^16, so the record offset is a (full-rate) 24-bit multiply.
 template <class F>
 __device__ __forceinline__ auto table(const LeoDev &f, unsigned log_m) {
-    if constexpr (std::is_same<F, F8>::value) return f.tables + static_cast<size_t>(log_m) * F::kTW;
-    else return as_const(f.tables) + static_cast<size_t>(log_m) * F::kTW;
+    const unsigned off = __umul24(log_m, static_cast<unsigned>(F::kTW));
+    if constexpr (std::is_same<F, F8>::value) return f.tables + off;
+    else return as_const(f.tables) + off;
 }
 
 // galois_noasm.go:94-113: fftDIT2 = mulAdd then xor, ifftDIT2 = xor then

@@ -93,6 +93,9 @@ __device__ __forceinline__ void strided_quads(const Tile &w, const Tile *w2, int
                                               int djs, int nq, const uint16_t *sk, const LeoDev &f) {
     const int A = 1 << ashift, dj = 1 << djs;
     const int items = (A << log2i(nq)) << w.ushift;
+    // Rows l0 + k*ld == l0 ^ (k*ld): l0 has zero bits at ld and 2*ld.
+    const int ld = dj << ashift;
+    const int d1 = pos_off(ld), d2 = pos_off(2 * ld), d3 = pos_off(3 * ld);
     const int umask = (1 << w.ushift) - 1;
     // With w2 the two tiles are separate items (the second tile's after the
     // first's): a step over both keeps twice the lanes busy.  items is a
@@ -107,10 +110,12 @@ __device__ __forceinline__ void strided_quads(const Tile &w, const Tile *w2, int
         const int rj = (q >> djs) << (djs + 2);
         const int j = rj + (q & (dj - 1));
         const int iend = rj * P + dist;  // global r + dist
-        const int l0 = (j << ashift) + a, ld = dj << ashift;
+        const int l0 = (j << ashift) + a;
         const Tile &x = t ? *w2 : w;
-        typename F::E x0 = Lds<F>::load(x, l0, u), x1 = Lds<F>::load(x, l0 + ld, u),
-                      x2 = Lds<F>::load(x, l0 + 2 * ld, u), x3 = Lds<F>::load(x, l0 + 3 * ld, u);
+        uint8_t *c = Lds<F>::col(x, u);
+        const int o0 = pos_off(l0), o1 = o0 ^ d1, o2 = o0 ^ d2, o3 = o0 ^ d3;
+        typename F::E x0 = Lds<F>::at(x, c, o0), x1 = Lds<F>::at(x, c, o1), x2 = Lds<F>::at(x, c, o2),
+                      x3 = Lds<F>::at(x, c, o3);
         if constexpr (UNI) {
             const unsigned lm01 = uniform_u16(sk, iend), lm02 = uniform_u16(sk, iend + dist),
                            lm23 = uniform_u16(sk, iend + 2 * dist);
@@ -141,10 +146,10 @@ __device__ __forceinline__ void strided_quads(const Tile &w, const Tile *w2, int
                 fft2<F>(x2, x3, lm23, f);
             }
         }
-        Lds<F>::store(x, l0, u, x0);
-        Lds<F>::store(x, l0 + ld, u, x1);
-        Lds<F>::store(x, l0 + 2 * ld, u, x2);
-        Lds<F>::store(x, l0 + 3 * ld, u, x3);
+        Lds<F>::put(x, c, o0, x0);
+        Lds<F>::put(x, c, o1, x1);
+        Lds<F>::put(x, c, o2, x2);
+        Lds<F>::put(x, c, o3, x3);
     }
 }
 
@@ -224,11 +229,13 @@ __device__ __forceinline__ void pass2(const LeoDev &f, const LeoMultipassArgs &a
         const int rshift = ashift + jshift;
         for (int it = threadIdx.x; it < items; it += blockDim.x) {
             const int l = it & (R - 1), sl = it >> rshift;
-            const int j = l >> ashift, lo = l & (a.A - 1);
-            uint4 acc = *slot(in, l, sl);
+            const int j = l >> ashift;
+            uint8_t *c = column(in, sl);
+            const int o = pos_off(l);
+            uint4 acc = *cell(c, o);
             for (int b = 1; b < J; b <<= 1)
-                if (!(j & b)) acc = xor4(acc, *slot(in, ((j | b) << ashift) + lo, sl));
-            *slot(y, l, sl) = acc;
+                if (!(j & b)) acc = xor4(acc, *cell(c, o ^ pos_off(b << ashift)));  // row l | (b << ashift)
+            *cell(column(y, sl), o) = acc;
         }
         __syncthreads();
     }
@@ -262,8 +269,10 @@ __device__ __forceinline__ void pass3(const LeoDev &f, const LeoMultipassArgs &a
             if (it < items) {
                 const int t = it & (a.P - 1), sl = it >> ps;
                 uint4 acc = gload4(wrow(a.work, a.n, s, base + t) + sl * 16);
+                uint8_t *c = column(w, sl);
+                const int o = pos_off(t);
                 for (int b = 1; b < a.P; b <<= 1)
-                    if (!(t & b)) acc = xor4(acc, *slot(w, t | b, sl));
+                    if (!(t & b)) acc = xor4(acc, *cell(c, o ^ pos_off(b)));  // row t | b
                 res[jj] = acc;
             }
         }
@@ -358,8 +367,9 @@ __device__ __forceinline__ void enc_pass2(const LeoDev &f, const LeoEncPassArgs 
         __syncthreads();
     }
     if (!a.first) {
+        const int rshift = log2i(R);  // R = A * J, a power of two
         for (int it = threadIdx.x; it < R * (kSlice / 16); it += blockDim.x) {
-            const int l = it % R, sl = it / R;
+            const int l = it & (R - 1), sl = it >> rshift;
             *slot(in, l, sl) = xor4(*slot(in, l, sl), *slot(acc, l, sl));
         }
         __syncthreads();

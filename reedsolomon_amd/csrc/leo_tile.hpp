@@ -47,7 +47,11 @@ __device__ __forceinline__ int log2i(int x) {
     return s;
 }
 
-__device__ __forceinline__ int pos(int r) { return r ^ (((r >> 4) & 1) * 5) ^ (((r >> 5) & 1) * 10); }
+// (b4 * 5) ^ (b5 * 10) == ((r >> 4) & 3) * 5: the two terms share no bits.
+// pos is linear over XOR (pos(r ^ d) == pos(r) ^ pos(d)), so rows r and
+// r + d with no common bits, as a butterfly's partners and the derivative's
+// r | b are, sit at pos(r) ^ pos(d), one XOR from the first row's position.
+__device__ __forceinline__ int pos(int r) { return r ^ (((r >> 4) & 3) * 5); }
 
 // Workgroup index g -> unit slice index when a 64-byte GF(2^16) chunk is
 // split into two 32-byte unit slices (slice 2c + u is unit u of chunk c).
@@ -67,41 +71,57 @@ __device__ __forceinline__ uint32_t xcd_quarter_slice(uint32_t g, uint32_t count
     return (((g >> 5) << 5) | ((g & 7) << 2)) | ((g >> 3) & 3);
 }
 
-__device__ __forceinline__ uint4 *slot(const Tile &t, int r, int s) {
-    return reinterpret_cast<uint4 *>(t.base + ((s * t.r1 + pos(r)) << 4));
-}
+// Byte offset of row r within a slot column.
+__device__ __forceinline__ int pos_off(int r) { return pos(r) << 4; }
 
+// Slot column s (its row-position-0 slot); slot counts and row counts stay
+// far below 2^24, so 24-bit multiplies (full rate) suffice for the offsets.
+__device__ __forceinline__ uint8_t *column(const Tile &t, int s) { return t.base + (__mul24(s, t.r1) << 4); }
+__device__ __forceinline__ uint4 *cell(uint8_t *col, int off) { return reinterpret_cast<uint4 *>(col + off); }
+
+__device__ __forceinline__ uint4 *slot(const Tile &t, int r, int s) { return cell(column(t, s), pos_off(r)); }
+
+// Element access by unit column (col) and row byte offset (pos_off of the
+// row), so a butterfly's rows derive their offsets from the first one's.
 template <class F>
 struct Lds;
 template <>
 struct Lds<F8> {
-    __device__ static uint4 load(const Tile &t, int r, int u) { return *slot(t, r, u); }
-    __device__ static void store(const Tile &t, int r, int u, const uint4 &e) { *slot(t, r, u) = e; }
+    __device__ static uint8_t *col(const Tile &t, int u) { return column(t, u); }
+    __device__ static uint4 at(const Tile &, uint8_t *c, int off) { return *cell(c, off); }
+    __device__ static void put(const Tile &, uint8_t *c, int off, const uint4 &e) { *cell(c, off) = e; }
+    __device__ static uint4 load(const Tile &t, int r, int u) { return at(t, col(t, u), pos_off(r)); }
+    __device__ static void store(const Tile &t, int r, int u, const uint4 &e) { put(t, col(t, u), pos_off(r), e); }
 };
 template <>
 struct Lds<F16> {
     __device__ static int lo(const Tile &t, int u) { return t.hio == 2 ? (((u >> 1) << 2) | (u & 1)) : (u << 1); }
-    __device__ static F16::E load(const Tile &t, int r, int u) {
-        const int l = lo(t, u);
-        return {*slot(t, r, l), *slot(t, r, l + t.hio)};
+    __device__ static uint8_t *col(const Tile &t, int u) { return column(t, lo(t, u)); }
+    __device__ static int hoff(const Tile &t) { return (t.hio * t.r1) << 4; }  // lo slot -> hi slot
+    __device__ static F16::E at(const Tile &t, uint8_t *c, int off) {
+        return {*cell(c, off), *cell(c + hoff(t), off)};
     }
-    __device__ static void store(const Tile &t, int r, int u, const F16::E &e) {
-        const int l = lo(t, u);
-        *slot(t, r, l) = e.lo;
-        *slot(t, r, l + t.hio) = e.hi;
+    __device__ static void put(const Tile &t, uint8_t *c, int off, const F16::E &e) {
+        *cell(c, off) = e.lo;
+        *cell(c + hoff(t), off) = e.hi;
     }
+    __device__ static F16::E load(const Tile &t, int r, int u) { return at(t, col(t, u), pos_off(r)); }
+    __device__ static void store(const Tile &t, int r, int u, const F16::E &e) { put(t, col(t, u), pos_off(r), e); }
 };
 
 // A quarter-chunk unit is one slot: its 8 low bytes, then its 8 high bytes.
 template <>
 struct Lds<F16Q> {
-    __device__ static F16Q::E load(const Tile &t, int r, int u) {
-        const uint4 v = *slot(t, r, u);
+    __device__ static uint8_t *col(const Tile &t, int u) { return column(t, u); }
+    __device__ static F16Q::E at(const Tile &, uint8_t *c, int off) {
+        const uint4 v = *cell(c, off);
         return {make_uint2(v.x, v.y), make_uint2(v.z, v.w)};
     }
-    __device__ static void store(const Tile &t, int r, int u, const F16Q::E &e) {
-        *slot(t, r, u) = make_uint4(e.lo.x, e.lo.y, e.hi.x, e.hi.y);
+    __device__ static void put(const Tile &, uint8_t *c, int off, const F16Q::E &e) {
+        *cell(c, off) = make_uint4(e.lo.x, e.lo.y, e.hi.x, e.hi.y);
     }
+    __device__ static F16Q::E load(const Tile &t, int r, int u) { return at(t, col(t, u), pos_off(r)); }
+    __device__ static void store(const Tile &t, int r, int u, const F16Q::E &e) { put(t, col(t, u), pos_off(r), e); }
 };
 
 // fn(q, u) for q < nq, u < ucount; q varies fastest across lanes.
@@ -208,6 +228,8 @@ __device__ __forceinline__ void step4_groups(const Tile &w, int base, int dshift
                                              const uint16_t *skew, int bias, const LeoDev &f,
                                              const uint32_t *need) {
     const int dist = 1 << dshift;
+    // Rows i + j*dist == i ^ (j*dist): i has zero bits dshift and dshift + 1.
+    const int d1 = pos_off(dist), d2 = pos_off(2 * dist), d3 = pos_off(3 * dist);
     for_rows(w, groups << dshift, [&](int q, int u) {
         const int r = (q >> dshift) << (dshift + 2);
         if (need) {
@@ -216,8 +238,10 @@ __device__ __forceinline__ void step4_groups(const Tile &w, int base, int dshift
         }
         const int i = r + (q & (dist - 1));
         const int iend = base + r + dist;
-        typename F::E x0 = Lds<F>::load(w, i, u), x1 = Lds<F>::load(w, i + dist, u),
-                      x2 = Lds<F>::load(w, i + 2 * dist, u), x3 = Lds<F>::load(w, i + 3 * dist, u);
+        uint8_t *c = Lds<F>::col(w, u);
+        const int o0 = pos_off(i), o1 = o0 ^ d1, o2 = o0 ^ d2, o3 = o0 ^ d3;
+        typename F::E x0 = Lds<F>::at(w, c, o0), x1 = Lds<F>::at(w, c, o1), x2 = Lds<F>::at(w, c, o2),
+                      x3 = Lds<F>::at(w, c, o3);
         if constexpr (UNI) {
             const unsigned lm01 = uniform_skew<F>(skew, iend + bias), lm02 = uniform_skew<F>(skew, iend + dist + bias),
                            lm23 = uniform_skew<F>(skew, iend + 2 * dist + bias);
@@ -265,10 +289,10 @@ __device__ __forceinline__ void step4_groups(const Tile &w, int base, int dshift
                 fft2<F>(x2, x3, lm23, f);
             }
         }
-        Lds<F>::store(w, i, u, x0);
-        Lds<F>::store(w, i + dist, u, x1);
-        Lds<F>::store(w, i + 2 * dist, u, x2);
-        Lds<F>::store(w, i + 3 * dist, u, x3);
+        Lds<F>::put(w, c, o0, x0);
+        Lds<F>::put(w, c, o1, x1);
+        Lds<F>::put(w, c, o2, x2);
+        Lds<F>::put(w, c, o3, x3);
     });
 }
 
@@ -417,9 +441,11 @@ __device__ __forceinline__ void derivative(const Tile &w, int n, int slots) {
         const int it = threadIdx.x + j * blockDim.x;
         if (it < items) {
             const int t = it & (n - 1), s = it >> ns;
-            uint4 acc = *slot(w, t, s);
+            uint8_t *c = column(w, s);
+            const int o = pos_off(t);
+            uint4 acc = *cell(c, o);
             for (int b = 1; b < n; b <<= 1)
-                if (!(t & b)) acc = xor4(acc, *slot(w, t | b, s));
+                if (!(t & b)) acc = xor4(acc, *cell(c, o ^ pos_off(b)));  // row t | b
             res[j] = acc;
         }
     }
