@@ -92,9 +92,10 @@ template <class F>
 __device__ __forceinline__ LeoDev lds_tables(const LeoDev &f) {
     if constexpr (std::is_same<F, F8>::value) {
         constexpr int kSkew = 255, kSkewPad = 288;  // the pad reads as zero
-        __shared__ uint32_t t8[256 * F8::kTW];
+        constexpr int kRecords = 256;
+        __shared__ uint32_t t8[kRecords * F8::kTW];
         __shared__ uint16_t s8[kSkewPad];
-        for (int i = threadIdx.x; i < 256 * F8::kTW; i += blockDim.x) t8[i] = f.tables[i];
+        for (int i = threadIdx.x; i < kRecords * F8::kTW; i += blockDim.x) t8[i] = f.tables[i];
         for (int i = threadIdx.x; i < kSkewPad; i += blockDim.x) s8[i] = i < kSkew ? f.skew[i] : 0;
         __syncthreads();
         LeoDev l = f;
@@ -140,12 +141,16 @@ __global__ __launch_bounds__(kFB) void k_leo_batch(const LeoDev f0, const LeoBat
     extern __shared__ uint4 lds_raw[];
     const LeoDev f = lds_tables<F>(f0);
     uint8_t *lds = reinterpret_cast<uint8_t *>(lds_raw);
+    int lo = 0;  // task of the previous slice: g only grows, so tasks only move forward
     for (uint32_t g = blockIdx.x; g < nslices; g += gridDim.x) {
-        int lo = 0, hi = ntasks - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (tasks[mid].slice0 <= g) lo = mid;
-            else hi = mid - 1;
+        if (lo + 1 < ntasks && tasks[lo + 1].slice0 <= g) {
+            int hi = ntasks - 1;
+            ++lo;  // tasks[lo].slice0 <= g: search [lo, hi] for the last such task
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (tasks[mid].slice0 <= g) lo = mid;
+                else hi = mid - 1;
+            }
         }
         const uint32_t end = lo + 1 < ntasks ? tasks[lo + 1].slice0 : nslices;
         const uint32_t l = g - tasks[lo].slice0, count = end - tasks[lo].slice0;

@@ -147,6 +147,23 @@ __device__ __forceinline__ void for_units(const Tile &t, int nrows, Fn fn) {
     }
 }
 
+// Lane order of the butterfly steps.  GF(2^8) tiles run the column units of
+// one butterfly item fastest: the lanes of an item share its skews and
+// multiply records (LDS broadcasts instead of a record per lane), and a
+// wave's 64 >> ushift items are consecutive, so the skews are wave-uniform
+// from distance 64 >> ushift.  Data reads stay conflict-free: consecutive
+// slot columns are r1 = rows + 1 (odd) cells apart.  GF(2^16) tiles run
+// rows fastest (the row swizzle keeps a quarter-wave's rows in distinct bank
+// groups), which matters there because a unit is two slots.
+template <class F>
+constexpr bool kUnitsFastest = std::is_same<F, F8>::value;
+
+template <class F, class Fn>
+__device__ __forceinline__ void for_items(const Tile &t, int nq, Fn fn) {
+    if constexpr (kUnitsFastest<F>) for_units(t, nq, fn);
+    else for_rows(t, nq, fn);
+}
+
 // p[i] for a wave-uniform i, by a scalar load of the aligned dword holding
 // it (the skew tables are whole allocations, so every entry is 2-byte
 // aligned inside a 4-byte aligned buffer).
@@ -230,7 +247,7 @@ __device__ __forceinline__ void step4_groups(const Tile &w, int base, int dshift
     const int dist = 1 << dshift;
     // Rows i + j*dist == i ^ (j*dist): i has zero bits dshift and dshift + 1.
     const int d1 = pos_off(dist), d2 = pos_off(2 * dist), d3 = pos_off(3 * dist);
-    for_rows(w, groups << dshift, [&](int q, int u) {
+    for_items<F>(w, groups << dshift, [&](int q, int u) {
         const int r = (q >> dshift) << (dshift + 2);
         if (need) {
             const int g = (base + r) >> (dshift + 2);
@@ -306,9 +323,16 @@ __device__ __forceinline__ void step4(const Tile &w, int rows, int base, int dsh
     const int left = mtrunc - base;
     if (left <= 0) return;  // uniform across the block: no barrier is skipped by a subset
     const int groups = min(rows >> (dshift + 2), (left + (4 << dshift) - 1) >> (dshift + 2));
-    if (dshift >= 6) step4_groups<F, INV, true>(w, base, dshift, groups, skew, bias, f, need);
-    else if (SHR && dshift == 4) step4_groups<F, INV, false, 16>(w, base, dshift, groups, skew, bias, f, need);
-    else if (SHR && dshift == 2) step4_groups<F, INV, false, 4>(w, base, dshift, groups, skew, bias, f, need);
+    // A wave spans 64 items (rows fastest) or 64 >> ushift (units fastest).
+    const int wave_items_log2 = kUnitsFastest<F> ? max(6 - w.ushift, 0) : 6;
+    if (dshift >= wave_items_log2) step4_groups<F, INV, true>(w, base, dshift, groups, skew, bias, f, need);
+    // (Units fastest: the lanes of a quad or row may include units past a
+    // narrow last slice, which group_records cannot have; their items share
+    // records by LDS broadcast instead.)
+    else if (SHR && !kUnitsFastest<F> && dshift == 4)
+        step4_groups<F, INV, false, 16>(w, base, dshift, groups, skew, bias, f, need);
+    else if (SHR && !kUnitsFastest<F> && dshift == 2)
+        step4_groups<F, INV, false, 4>(w, base, dshift, groups, skew, bias, f, need);
     else step4_groups<F, INV, false>(w, base, dshift, groups, skew, bias, f, need);
     __syncthreads();
 }
@@ -326,7 +350,7 @@ __device__ __forceinline__ void ifft(const Tile &w, int n, int mtrunc, const uin
     const int dist = 1 << dshift;
     if (dist < n) {
         const unsigned lm = skew[dist + bias];
-        for_rows(w, dist, [&](int i, int u) {
+        for_items<F>(w, dist, [&](int i, int u) {
             typename F::E x = Lds<F>::load(w, i, u), y = Lds<F>::load(w, i + dist, u);
             ifft2<F>(x, y, lm, f);
             Lds<F>::store(w, i, u, x);
@@ -352,7 +376,7 @@ __device__ __forceinline__ void fft(const Tile &w, int n, int mtrunc, const LeoD
         dist >>= 2;
     }
     if (dist4 == 2) {
-        for_rows(w, (mtrunc + 1) / 2, [&](int q, int u) {
+        for_items<F>(w, (mtrunc + 1) / 2, [&](int q, int u) {
             if (need && !((need[q >> 5] >> (q & 31)) & 1u)) return;
             const int r = 2 * q;
             typename F::E x = Lds<F>::load(w, r, u), y = Lds<F>::load(w, r + 1, u);
@@ -430,23 +454,37 @@ __device__ __forceinline__ void compare_rows(const Tile &w, uint8_t *const *dst,
 
 // Formal derivative in place (leopard.go:619-623): out[t] = in[t] ^ XOR over
 // zero bits b of t (t | 2^b < n) of in[t | 2^b].  Every thread computes its
-// slots into registers, then all store after a barrier.
+// slots into registers, then all store after a barrier.  The bit loop is
+// outermost and branch-free: for each bit, every item reads row t ^ 2^b
+// (in range either way) and keeps it under a mask when bit b of t is zero,
+// so one bit's reads of all items issue together instead of one divergent,
+// latency-bound read at a time.
 template <int MAXD = kDerivMax>
 __device__ __forceinline__ void derivative(const Tile &w, int n, int slots) {
     const int ns = log2i(n);
     const int items = n * slots;
     uint4 res[MAXD];
+    uint8_t *col[MAXD];  // the item's slot column
+    int po[MAXD];        // pos_off of its row
 #pragma unroll
     for (int j = 0; j < MAXD; ++j) {
         const int it = threadIdx.x + j * blockDim.x;
-        if (it < items) {
-            const int t = it & (n - 1), s = it >> ns;
-            uint8_t *c = column(w, s);
-            const int o = pos_off(t);
-            uint4 acc = *cell(c, o);
-            for (int b = 1; b < n; b <<= 1)
-                if (!(t & b)) acc = xor4(acc, *cell(c, o ^ pos_off(b)));  // row t | b
-            res[j] = acc;
+        const int ic = it < items ? it : 0;  // spare items read item 0 (in range)
+        col[j] = column(w, ic >> ns);
+        po[j] = pos_off(ic & (n - 1));
+        res[j] = *cell(col[j], po[j]);
+    }
+    for (int s = 0; s < ns; ++s) {
+        const int ob = pos_off(1 << s);
+#pragma unroll
+        for (int j = 0; j < MAXD; ++j) {
+            const int t = (threadIdx.x + j * blockDim.x) & (n - 1);
+            const uint32_t keep = ((t >> s) & 1) ? 0u : ~0u;
+            const uint4 v = *cell(col[j], po[j] ^ ob);  // row t ^ 2^s of the same slot
+            res[j].x ^= v.x & keep;
+            res[j].y ^= v.y & keep;
+            res[j].z ^= v.z & keep;
+            res[j].w ^= v.w & keep;
         }
     }
     __syncthreads();
