@@ -136,8 +136,8 @@ __global__ __launch_bounds__(kFBMax) void k_decode_fused(const LeoDev f0, const 
 // their task by binary search over the first-slice offsets, and run that
 // task's slice.  The task is uniform per block.
 template <class F, int MAXD>
-__global__ __launch_bounds__(kFB) void k_leo_batch(const LeoDev f0, const LeoBatchTask *tasks, int ntasks,
-                                                   uint32_t nslices) {
+__device__ __forceinline__ void leo_batch_slices(const LeoDev &f0, const LeoBatchTask *tasks, int ntasks,
+                                                 uint32_t nslices) {
     extern __shared__ uint4 lds_raw[];
     const LeoDev f = lds_tables<F>(f0);
     uint8_t *lds = reinterpret_cast<uint8_t *>(lds_raw);
@@ -162,6 +162,22 @@ __global__ __launch_bounds__(kFB) void k_leo_batch(const LeoDev f0, const LeoBat
             decode_slice<F, MAXD>(f, a, lds, slice_start(l, count, a.slice));
         }
     }
+}
+
+template <class F, int MAXD>
+__global__ __launch_bounds__(kFB) void k_leo_batch(const LeoDev f0, const LeoBatchTask *tasks, int ntasks,
+                                                   uint32_t nslices) {
+    leo_batch_slices<F, MAXD>(f0, tasks, ntasks, nslices);
+}
+
+// GF(2^8) with the smaller register budget: four 256-thread workgroups per
+// CU are what LDS allows for a reconstruct tile, and four waves per SIMD
+// keep the kernel within 128 VGPRs (121, no spills; unbounded the
+// compiler takes 130, three waves per SIMD).
+template <class F, int MAXD>
+__global__ __launch_bounds__(kFB) __attribute__((amdgpu_waves_per_eu(4))) void k_leo_batch_w4(
+    const LeoDev f0, const LeoBatchTask *tasks, int ntasks, uint32_t nslices) {
+    leo_batch_slices<F, MAXD>(f0, tasks, ntasks, nslices);
 }
 
 unsigned grid_for(size_t size, int slice) {
@@ -225,8 +241,13 @@ hipError_t leo_batch_launch(const LeoDev &f, const LeoBatchTask *tasks, int ntas
         if (small) RSX_LAUNCH(F16, kDerivMax / 2);
         else RSX_LAUNCH(F16, kDerivMax);
     } else {
-        if (small) RSX_LAUNCH(F8, kDerivMax / 2);
-        else RSX_LAUNCH(F8, kDerivMax);
+        if (small) {
+            if ((e = allow_lds(k_leo_batch_w4<F8, kDerivMax / 2>, lds)) != hipSuccess) return e;
+            hipLaunchKernelGGL((k_leo_batch_w4<F8, kDerivMax / 2>), dim3(grid), dim3(kFB), lds, s, f, tasks, ntasks,
+                               nslices);
+        } else {
+            RSX_LAUNCH(F8, kDerivMax);
+        }
     }
 #undef RSX_LAUNCH
     return hipGetLastError();

@@ -389,53 +389,102 @@ __device__ __forceinline__ void fft(const Tile &w, int n, int mtrunc, const LeoD
 }
 
 // rows [0, nrows) of w <- src[t] (+col) times exp(logs[t]) when logs, zero
-// where src[t] is null.  Each thread issues kBatch row loads before it
-// stores any, so HBM latency overlaps across them.
-template <class F>
-__device__ __forceinline__ void load_rows(const Tile &w, const uint8_t *const *src, const uint16_t *logs,
-                                          int nrows, size_t col, const LeoDev &f) {
+// where src[t] is null.  Each thread first loads the row pointers (and, for
+// a reconstruct, the logs) of its kBatch items, from indices clamped into
+// range so those loads carry no branch, then issues the kBatch data loads,
+// and only then stores any: a data load never waits for an earlier item's
+// data (item j's pointer check waits for pointer j alone).  LOGS is a
+// template flag so the log loads are unconditional too.
+template <class F, bool LOGS>
+__device__ __forceinline__ void load_rows_impl(const Tile &w, const uint8_t *const *src, const uint16_t *logs,
+                                               int nrows, size_t col, const LeoDev &f) {
     constexpr int kBatch = 32 / F::kUnitBytes * 4;  // 8 GF(2^8) / 4 GF(2^16) units in flight
     const int items = nrows << w.ushift;
     const int umask = (1 << w.ushift) - 1;
     for (int it0 = threadIdx.x; it0 < items; it0 += kBatch * blockDim.x) {
+        const uint8_t *sp[kBatch];
+        unsigned lg[kBatch];
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j) {
+            const int t = min((it0 + j * static_cast<int>(blockDim.x)) >> w.ushift, nrows - 1);
+            sp[j] = src[t];
+            if constexpr (LOGS) lg[j] = logs[t];
+        }
         typename F::E v[kBatch];
 #pragma unroll
         for (int j = 0; j < kBatch; ++j) {
             const int it = it0 + j * blockDim.x;
             v[j] = F::zero();
-            if (it < items && (it & umask) < w.ucount) {
-                const uint8_t *s = src[it >> w.ushift];
-                if (s) v[j] = F::gload(s + col, (it & umask) + w.ubase);
-            }
+            if (it < items && (it & umask) < w.ucount && sp[j]) v[j] = F::gload(sp[j] + col, (it & umask) + w.ubase);
         }
 #pragma unroll
         for (int j = 0; j < kBatch; ++j) {
             const int it = it0 + j * blockDim.x;
             if (it < items && (it & umask) < w.ucount) {
-                const int t = it >> w.ushift;
-                if (logs) {
+                if constexpr (LOGS) {
                     typename F::E y = F::zero();
-                    F::mul_add(y, v[j], table<F>(f, logs[t]));
+                    F::mul_add(y, v[j], table<F>(f, lg[j]));
                     v[j] = y;
                 }
-                Lds<F>::store(w, t, it & umask, v[j]);
+                Lds<F>::store(w, it >> w.ushift, it & umask, v[j]);
             }
         }
     }
     __syncthreads();
 }
 
-// dst[q] (+col) <- row rows[q] (or row q) times exp(logs[q]) when logs.
+template <class F>
+__device__ __forceinline__ void load_rows(const Tile &w, const uint8_t *const *src, const uint16_t *logs,
+                                          int nrows, size_t col, const LeoDev &f) {
+    if (logs) load_rows_impl<F, true>(w, src, logs, nrows, col, f);
+    else load_rows_impl<F, false>(w, src, nullptr, nrows, col, f);
+}
+
+// dst[q] (+col) <- row rows[q] times exp(logs[q]) (MAPPED: a reconstruct's
+// erased rows) or row q (an encode's parity).  As in load_rows, each thread
+// loads the metadata of its kBatch items first (indices clamped into range),
+// then reads, multiplies and stores, so no item waits for another's loads.
+template <class F, bool MAPPED>
+__device__ __forceinline__ void store_rows_impl(const Tile &w, uint8_t *const *dst, const int *rows,
+                                                const uint16_t *logs, int count, size_t col, const LeoDev &f) {
+    constexpr int kBatch = 4;
+    const int items = count << w.ushift;
+    const int umask = (1 << w.ushift) - 1;
+    for (int it0 = threadIdx.x; it0 < items; it0 += kBatch * blockDim.x) {
+        uint8_t *dp[kBatch];
+        int rw[kBatch];
+        unsigned lg[kBatch];
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j) {
+            const int q = min((it0 + j * static_cast<int>(blockDim.x)) >> w.ushift, count - 1);
+            dp[j] = dst[q];
+            if constexpr (MAPPED) {
+                rw[j] = rows[q];
+                lg[j] = logs[q];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j) {
+            const int it = it0 + j * blockDim.x;
+            if (it < items && (it & umask) < w.ucount) {
+                const int u = it & umask;
+                const typename F::E x = Lds<F>::load(w, MAPPED ? rw[j] : it >> w.ushift, u);
+                typename F::E v = x;
+                if constexpr (MAPPED) {
+                    v = F::zero();
+                    F::mul_add(v, x, table<F>(f, lg[j]));
+                }
+                F::gstore(dp[j] + col, u + w.ubase, v);
+            }
+        }
+    }
+}
+
 template <class F>
 __device__ __forceinline__ void store_rows(const Tile &w, uint8_t *const *dst, const int *rows,
                                            const uint16_t *logs, int count, size_t col, const LeoDev &f) {
-    for_units(w, count, [&](int q, int u) {
-        const typename F::E x = Lds<F>::load(w, rows ? rows[q] : q, u);
-        typename F::E v = F::zero();
-        if (logs) F::mul_add(v, x, table<F>(f, logs[q]));
-        else v = x;
-        F::gstore(dst[q] + col, u + w.ubase, v);
-    });
+    if (rows) store_rows_impl<F, true>(w, dst, rows, logs, count, col, f);
+    else store_rows_impl<F, false>(w, dst, nullptr, nullptr, count, col, f);
 }
 
 // Verify: compares rows [0, count) of w with dst[q] (+col); a thread that
