@@ -233,8 +233,14 @@ __device__ __forceinline__ void pass2(const LeoDev &f, const LeoMultipassArgs &a
             uint8_t *c = column(in, sl);
             const int o = pos_off(l);
             uint4 acc = *cell(c, o);
-            for (int b = 1; b < J; b <<= 1)
-                if (!(j & b)) acc = xor4(acc, *cell(c, o ^ pos_off(b << ashift)));  // row l | (b << ashift)
+            for (int b = 0; (1 << b) < J; ++b) {  // branch-free: row l ^ (2^b << ashift), kept where bit b of j is 0
+                const uint32_t keep = ((j >> b) & 1) ? 0u : ~0u;
+                const uint4 v = *cell(c, o ^ pos_off((1 << b) << ashift));
+                acc.x ^= v.x & keep;
+                acc.y ^= v.y & keep;
+                acc.z ^= v.z & keep;
+                acc.w ^= v.w & keep;
+            }
             *cell(column(y, sl), o) = acc;
         }
         __syncthreads();
@@ -257,23 +263,37 @@ __device__ __forceinline__ void pass3(const LeoDev &f, const LeoMultipassArgs &a
     const Tile w{reinterpret_cast<uint8_t *>(lds_raw), a.P + 1, 1, 2};
     load_tile<F>(w, a.P, [&](int l) -> const uint8_t * { return wrow(a.work2, a.n, s, base + l); });  // v
     __syncthreads();
-    // z = u + L(v): L over the row bits < P.  Registers first, then LDS.
+    // z = u + L(v): L over the row bits < P.  Registers first, then LDS.  All
+    // of a thread's u loads issue before any is used, then the bit loop runs
+    // outermost and branch-free (row t ^ 2^b is read for every item and kept
+    // where bit b of t is zero), as in the tile derivative.
     {
         const int slots = kSlice / 16;
         const int ps = log2i(a.P);
         const int items = a.P * slots;
         uint4 res[MAXD];
+        uint8_t *cc[MAXD];
+        int po[MAXD];
 #pragma unroll
         for (int jj = 0; jj < MAXD; ++jj) {
             const int it = threadIdx.x + jj * blockDim.x;
-            if (it < items) {
-                const int t = it & (a.P - 1), sl = it >> ps;
-                uint4 acc = gload4(wrow(a.work, a.n, s, base + t) + sl * 16);
-                uint8_t *c = column(w, sl);
-                const int o = pos_off(t);
-                for (int b = 1; b < a.P; b <<= 1)
-                    if (!(t & b)) acc = xor4(acc, *cell(c, o ^ pos_off(b)));  // row t | b
-                res[jj] = acc;
+            const int ic = it < items ? it : 0;  // spare items read item 0 (in range)
+            const int t = ic & (a.P - 1), sl = ic >> ps;
+            res[jj] = gload4(wrow(a.work, a.n, s, base + t) + sl * 16);
+            cc[jj] = column(w, sl);
+            po[jj] = pos_off(t);
+        }
+        for (int b = 0; (1 << b) < a.P; ++b) {
+            const int ob = pos_off(1 << b);
+#pragma unroll
+            for (int jj = 0; jj < MAXD; ++jj) {
+                const int t = (threadIdx.x + jj * blockDim.x) & (a.P - 1);
+                const uint32_t keep = ((t >> b) & 1) ? 0u : ~0u;
+                const uint4 v = *cell(cc[jj], po[jj] ^ ob);  // row t ^ 2^b
+                res[jj].x ^= v.x & keep;
+                res[jj].y ^= v.y & keep;
+                res[jj].z ^= v.z & keep;
+                res[jj].w ^= v.w & keep;
             }
         }
         __syncthreads();
