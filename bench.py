@@ -13,7 +13,8 @@ Also measured (unless --quick): RS(10,4) over 1 MiB shards (1024 stripes,
 batched encode + reconstruct), RS(10,3) over one stripe of 10 x 1 MiB (plain
 Encode, Verify, Reconstruct of data shards 3 and 7), the reference's own
 benchmark shapes (README encode / reconstruct tables, Leopard 1 KB shards)
-through the plain per-call path beside the published CPU figures, the mixed
+through the plain per-call path beside the published CPU figures, Leopard
+GF(2^8) (192,56) 64 KiB batched (96 stripes per plan), the mixed
 workload (8192 stripes, (k, m) in
 {(4,2),(6,3),(10,4),(17,3),(32,8)}, shards log-uniform 4 KiB-1 MiB rounded to
 64 B, 30% reconstructs with 1..m erasures) batched vs plain, and Leopard
@@ -72,8 +73,8 @@ def _parse():
                     help="served stream: Poisson rate as a fraction of the measured saturation rate")
     ap.add_argument("--hbm-budget-gib", type=float, default=16.0,
                     help="served stream: shard bytes admitted in flight")
-    ap.add_argument("--only", choices=["rs10_4", "rs10_3", "mixed", "leopard16", "served", "stream", "reference",
-                                       "ceiling", "butterfly"],
+    ap.add_argument("--only", choices=["rs10_4", "rs10_3", "mixed", "leopard16", "leopard8", "served", "stream",
+                                       "reference", "ceiling", "butterfly"],
                     help="run one secondary section alone and print its JSON (for profiling)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="with --encode-only: also copy 1 GiB HBM->HBM 5 times (counter calibration)")
@@ -813,6 +814,62 @@ def leopard(args, torch, rs):
     return out
 
 
+def leopard8(args, torch, rs):
+    """Leopard GF(2^8) (192,56) 64 KiB shards, the served stream's Leopard
+    shape, batched: 96 stripes per plan (1.56 GB), encode, Verify, and
+    reconstruct with 1..56 seeded random erasures per stripe; each plan timed
+    per launch with HIP events on its stream.  The reconstructed bytes are
+    compared with the originals."""
+    k, m, S, nst = 192, 56, 64 << 10, 96
+    n = k + m
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(192)
+    st = Stripes([(k, m)] * nst, [S] * nst, torch, gen)
+    enc = rs.New(k, m, rs.WithLeopardGF(True))
+    codecs = {(k, m): enc}
+    rng = np.random.default_rng(56)
+    out = dict(k=k, m=m, shard_bytes=S, stripes=nst)
+    b = nst * n * S
+
+    def plan_of(ops, lens):
+        bt = rs.Batch(0)
+        bt.set_tuning(16 << 10, 8)
+        plan(rs, bt, batch_items(rs, st, codecs, ops, lens), nst)
+        return bt
+
+    benc = plan_of([rs.OP_ENCODE] * nst, st.full)
+    benc.launch()
+    benc.synchronize()
+    want = st.buf.clone()
+    lens = []
+    for _ in range(nst):
+        lost = set(rng.choice(n, int(rng.integers(1, m + 1)), replace=False).tolist())
+        lens.append((ctypes.c_size_t * n)(*[0 if j in lost else S for j in range(n)]))
+    brec = plan_of([rs.OP_RECONSTRUCT] * nst, lens)
+    bver = plan_of([rs.OP_VERIFY] * nst, st.full)
+    for name, bt in (("encode", benc), ("verify", bver), ("reconstruct", brec)):
+        for _ in range(3):
+            bt.launch()
+        bt.synchronize()
+        t = event_time(torch, bt.stream, bt.launch, 20)
+        out[name] = {"ms": round(t * 1e3, 3), "gbps": round(b / t / 1e9, 1)}
+    out["verify_ok"] = all(bver.verify_result(i) for i in range(nst))
+    # erase the reconstruct plan's lost shards, rebuild them, compare
+    for s in range(nst):
+        for j in range(n):
+            if lens[s][j] == 0:
+                st.shard(s, j).zero_()
+    torch.cuda.synchronize()
+    brec.launch()
+    brec.synchronize()
+    out["reconstruct_equals_original"] = bool(torch.equal(st.buf, want))
+    _log(f"bench: leopard8 (192,56) x {nst} batched: encode {out['encode']['gbps']} GB/s, "
+         f"reconstruct {out['reconstruct']['gbps']} GB/s, verify {out['verify']['gbps']} GB/s")
+    del st, want, benc, brec, bver
+    torch.cuda.empty_cache()
+    return out
+
+
 def served(args, torch, rs):
     """Served request stream: encode / verify / reconstruct requests (60/10/30)
     of the mixed-workload shapes and sizes plus 5% Leopard GF(2^8) (192,56)
@@ -1109,7 +1166,8 @@ def main() -> int:
 
     torch.cuda.set_device(0)
     if args.only:
-        fn = {"rs10_4": rs10_4, "rs10_3": config0, "mixed": mixed, "leopard16": leopard, "served": served,
+        fn = {"rs10_4": rs10_4, "rs10_3": config0, "mixed": mixed, "leopard16": leopard, "leopard8": leopard8,
+              "served": served,
               "stream": stream, "reference": reference_shapes, "ceiling": ceiling, "butterfly": butterfly}[args.only]
         sec = fn(args, torch, rs)
         sec.pop("sample", None)
@@ -1120,7 +1178,7 @@ def main() -> int:
         head.pop("sample", None)
         print(json.dumps({"kernel": "batch_pass<4>", **head}), flush=True)
         return 0
-    mix = leo = srv = r104 = c0 = strm = refs = ceil = bfly = None
+    mix = leo = leo8 = srv = r104 = c0 = strm = refs = ceil = bfly = None
     if not args.quick:
         ceil = ceiling(args, torch, rs)
         bfly = butterfly(args, torch, rs)
@@ -1129,6 +1187,7 @@ def main() -> int:
         refs = reference_shapes(args, torch, rs)
         mix = mixed(args, torch, rs)
         leo = leopard(args, torch, rs)
+        leo8 = leopard8(args, torch, rs)
         srv = served(args, torch, rs)
         strm = stream(args, torch, rs)
     cpu, checks = cpu_baseline(args, head.pop("sample", None), c0, leo)
@@ -1174,6 +1233,7 @@ def main() -> int:
         "reference_shapes": refs,
         "mixed_workload": mix,
         "leopard16": leo,
+        "leopard8_batched": leo8,
         "gf16_butterfly_rates": bfly,
         "served": srv,
         "stream": strm,
