@@ -34,6 +34,16 @@ __device__ __forceinline__ Tile slice_tile(uint8_t *lds, int rows, int slice, si
                 unit ? 1 : 2};
 }
 
+// The encode's end: FFT of the accumulated IFFTs, then the parity rows
+// stored (or compared, for Verify).
+template <class F>
+__device__ __forceinline__ void encode_finish(const LeoDev &f, const LeoEncodeArgs &a, const Tile &w, size_t col) {
+    fft<F, true>(w, a.m, a.p, f);
+    if (a.flag) compare_rows<F>(w, a.parity, a.p, col, a.flag, a.flag_value);
+    else store_rows<F>(w, a.parity, nullptr, nullptr, a.p, col, f);
+    __syncthreads();  // the next slice reuses the rows
+}
+
 // One column slice (bytes [sc, sc + slice) of every shard) of an encode.
 // With several m-row groups, the XOR of their IFFTs accumulates in
 // registers (MAXD slots per thread, row-fastest like the derivative) while
@@ -64,10 +74,46 @@ __device__ __forceinline__ void encode_slice(const LeoDev &f, const LeoEncodeArg
         }
         __syncthreads();  // before the next group's rows (or the FFT) reuse the tile
     }
-    fft<F, true>(w, a.m, a.p, f);
-    if (a.flag) compare_rows<F>(w, a.parity, a.p, col, a.flag, a.flag_value);
-    else store_rows<F>(w, a.parity, nullptr, nullptr, a.p, col, f);
-    __syncthreads();  // the next slice reuses the rows
+    encode_finish<F>(f, a, w, col);
+}
+
+// The batched GF(2^8) encode: the groups' XOR accumulates in a second LDS
+// region of the tile's shape (the batch kernel's LDS is sized for a
+// reconstruct tile anyway), which frees the registers for group g + 1's rows
+// to load while group g's IFFT runs: no load phase of its own per group.
+// The planner sizes LDS for both regions (leo_batch_encode_lds).
+template <class F>
+__device__ __forceinline__ void encode_slice_pipelined(const LeoDev &f, const LeoEncodeArgs &a, uint8_t *lds,
+                                                       size_t sc) {
+    constexpr int kPre = 4;  // prefetched slots per thread: m * slice / 16 <= 4 * 256 in every batched plan
+    size_t col;
+    const Tile w = slice_tile<F>(lds, a.m, a.slice, a.size, sc, col);
+    if (a.groups == 1 || !RowPrefetch<F, kPre>::fits(w, a.m)) {
+        encode_slice<F, 8>(f, a, lds, sc);
+        return;
+    }
+    Tile acc = w;
+    acc.base = lds + static_cast<size_t>(a.slice) * (a.m + 1);
+    const int ms = log2i(a.m);
+    const int items = a.m * (w.ucount * F::kUnitBytes / 16);
+    RowPrefetch<F, kPre> next;
+    next.issue(w, a.data, a.m, col);
+    for (int g = 0; g < a.groups; ++g) {
+        const int mtrunc = g == 0 ? (a.k < a.m ? a.k : a.m) : (a.k - g * a.m < a.m ? a.k - g * a.m : a.m);
+        next.put(w, a.m);
+        __syncthreads();
+        if (g + 1 < a.groups) next.issue(w, a.data + static_cast<size_t>(g + 1) * a.m, a.m, col);
+        ifft<F, true>(w, a.m, mtrunc, f.skew + a.m - 1 + g * a.m, 0, f);
+        const bool first = g == 0, last = g == a.groups - 1;
+        for (int it = threadIdx.x; it < items; it += blockDim.x) {
+            uint4 *p = slot(w, it & (a.m - 1), it >> ms), *q = slot(acc, it & (a.m - 1), it >> ms);
+            if (first) *q = *p;
+            else if (last) *p = xor4(*q, *p);
+            else *q = xor4(*q, *p);
+        }
+        __syncthreads();  // before the next group's rows (or the FFT) reuse the tile
+    }
+    encode_finish<F>(f, a, w, col);
 }
 
 // One column slice of a reconstruct; MAXD bounds the slots per thread the
@@ -135,7 +181,7 @@ __global__ __launch_bounds__(kFBMax) void k_decode_fused(const LeoDev f0, const 
 // Batched: blocks stride over the global slice index of all tasks, find
 // their task by binary search over the first-slice offsets, and run that
 // task's slice.  The task is uniform per block.
-template <class F, int MAXD>
+template <class F, int MAXD, bool PIPE = false>
 __device__ __forceinline__ void leo_batch_slices(const LeoDev &f0, const LeoBatchTask *tasks, int ntasks,
                                                  uint32_t nslices) {
     extern __shared__ uint4 lds_raw[];
@@ -156,7 +202,8 @@ __device__ __forceinline__ void leo_batch_slices(const LeoDev &f0, const LeoBatc
         const uint32_t l = g - tasks[lo].slice0, count = end - tasks[lo].slice0;
         if (tasks[lo].op == 0) {
             const LeoEncodeArgs a = tasks[lo].enc;
-            encode_slice<F, MAXD>(f, a, lds, slice_start(l, count, a.slice));
+            if constexpr (PIPE) encode_slice_pipelined<F>(f, a, lds, slice_start(l, count, a.slice));
+            else encode_slice<F, MAXD>(f, a, lds, slice_start(l, count, a.slice));
         } else {
             const LeoDecodeArgs a = tasks[lo].dec;
             decode_slice<F, MAXD>(f, a, lds, slice_start(l, count, a.slice));
@@ -177,7 +224,7 @@ __global__ __launch_bounds__(kFB) void k_leo_batch(const LeoDev f0, const LeoBat
 template <class F, int MAXD>
 __global__ __launch_bounds__(kFB) __attribute__((amdgpu_waves_per_eu(4))) void k_leo_batch_w4(
     const LeoDev f0, const LeoBatchTask *tasks, int ntasks, uint32_t nslices) {
-    leo_batch_slices<F, MAXD>(f0, tasks, ntasks, nslices);
+    leo_batch_slices<F, MAXD, true>(f0, tasks, ntasks, nslices);
 }
 
 unsigned grid_for(size_t size, int slice) {

@@ -440,6 +440,42 @@ __device__ __forceinline__ void load_rows(const Tile &w, const uint8_t *const *s
     else load_rows_impl<F, false>(w, src, nullptr, nrows, col, f);
 }
 
+// A row load split in two halves so that other work overlaps it: issue()
+// starts the data loads of the thread's items (at most N per thread: the
+// caller checks fits()), put() stores them into the tile once they arrive.
+// No logs; rows with a null pointer read as zero.
+template <class F, int N>
+struct RowPrefetch {
+    typename F::E v[N];
+
+    __device__ static bool fits(const Tile &w, int nrows) {
+        return (nrows << w.ushift) <= N * static_cast<int>(blockDim.x);
+    }
+    __device__ void issue(const Tile &w, const uint8_t *const *src, int nrows, size_t col) {
+        const int items = nrows << w.ushift;
+        const int umask = (1 << w.ushift) - 1;
+        const uint8_t *sp[N];
+#pragma unroll
+        for (int j = 0; j < N; ++j)
+            sp[j] = src[min((static_cast<int>(threadIdx.x) + j * static_cast<int>(blockDim.x)) >> w.ushift, nrows - 1)];
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            const int it = threadIdx.x + j * blockDim.x;
+            v[j] = F::zero();
+            if (it < items && (it & umask) < w.ucount && sp[j]) v[j] = F::gload(sp[j] + col, (it & umask) + w.ubase);
+        }
+    }
+    __device__ void put(const Tile &w, int nrows) const {
+        const int items = nrows << w.ushift;
+        const int umask = (1 << w.ushift) - 1;
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            const int it = threadIdx.x + j * blockDim.x;
+            if (it < items && (it & umask) < w.ucount) Lds<F>::store(w, it >> w.ushift, it & umask, v[j]);
+        }
+    }
+};
+
 // dst[q] (+col) <- row rows[q] times exp(logs[q]) (MAPPED: a reconstruct's
 // erased rows) or row q (an encode's parity).  As in load_rows, each thread
 // loads the metadata of its kBatch items first (indices clamped into range),

@@ -806,7 +806,9 @@ int leopard_plan_item(rsx_codec *c, int op, uint8_t *const *shards, const size_t
         }
         out.task.op = 0;
         out.task.enc = LeoEncodeArgs{nullptr, nullptr, size, k, p, m, groups, slice};
-        out.lds = static_cast<size_t>(slice) * (m + 1);
+        // GF(2^8) with several groups: a second tile-shaped LDS region holds
+        // the groups' XOR (the batch kernel's pipelined encode).
+        out.lds = static_cast<size_t>(slice) * (m + 1) * (s.bits == 8 && groups > 1 ? 2 : 1);
         out.regs = groups > 1 ? static_cast<size_t>(m) * slice : 0;
         out.slices = static_cast<uint32_t>((size + slice - 1) / slice);
         (void)f;
