@@ -232,6 +232,57 @@ def test_batched_leopard_items_match_oracle_and_plain():
     assert b.num_tasks() > 0
 
 
+def test_batched_many_gf8_leopard_items_match_oracle():
+    # Many GF(2^8) Leopard items per plan, so workgroups walk slices of many
+    # tasks in turn (forward task search), encodes of several groups take the
+    # pipelined path (groups' XOR in a second LDS region, the next group's rows
+    # loading during the IFFT), and reconstructs, Verify and encodes of
+    # different shapes share one launch.
+    rng = np.random.default_rng(2025)
+    shapes = [(192, 56, 64 * 40), (10, 3, 64 * 37), (100, 28, 64 * 9), (40, 24, 1024)]
+    codecs = {(k, m): rs.New(k, m, rs.WithLeopardGF(True)) for k, m, _ in shapes}
+    items, checks = [], []
+    for n in range(24):
+        k, m, size = shapes[n % len(shapes)]
+        enc = codecs[(k, m)]
+        full = _leo_codeword(rng, 8, k, m, size)
+        dev = [torch.from_numpy(x).to("cuda") for x in full]
+        kind = n % 3
+        if kind == 0:  # encode into stale parity
+            shards = dev[:k] + [torch.from_numpy(rng.integers(0, 256, size, dtype=np.uint8)).to("cuda")
+                                for _ in range(m)]
+            items.append((enc, rs.OP_ENCODE, shards))
+            checks.append(("enc", k, m, full, shards))
+        elif kind == 1:  # Verify, every other one corrupted
+            good = n % 2 == 0
+            shards = [x.clone() for x in dev]
+            if not good:
+                shards[int(rng.integers(0, k + m))][int(rng.integers(0, size))] ^= 0x5A
+            items.append((enc, rs.OP_VERIFY, shards))
+            checks.append(("ver", good))
+        else:  # reconstruct 1..m erasures
+            lost = set(rng.choice(k + m, int(rng.integers(1, m + 1)), replace=False).tolist())
+            work = [None if i in lost else dev[i].clone() for i in range(k + m)]
+            items.append((enc, rs.OP_RECONSTRUCT, work))
+            checks.append(("rec", k, m, full, work, lost))
+    b = rs.Batch(0)
+    for use_graph in (True, False):
+        b.set_graph(use_graph)
+        status = b.run(items)
+        assert status == [0] * len(items), status
+        for idx, c in enumerate(checks):
+            if c[0] == "enc":
+                _, k, m, full, shards = c
+                for i in range(k, k + m):
+                    assert np.array_equal(shards[i].cpu().numpy(), full[i]), (idx, i)
+            elif c[0] == "ver":
+                assert b.verify_result(idx) == c[1], idx
+            else:
+                _, k, m, full, work, lost = c
+                for i in lost:
+                    assert np.array_equal(work[i].cpu().numpy(), full[i]), (idx, i)
+
+
 @pytest.mark.parametrize("k,m,size", [(5000, 3000, 64 * 2), (3000, 5000, 64 * 3)])
 def test_batched_tall_gf16_encode_and_verify_match_oracle(k, m, size):
     # m = ceilPow2(parity) of 4096 with two groups, and 8192 with one: the
