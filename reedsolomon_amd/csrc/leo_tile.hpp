@@ -27,11 +27,12 @@ constexpr int kDerivMax = kLeoDerivMax;    // uint4 per thread in the derivative
 // slice holds one unit of a chunk (unit `ubase` of it, hio = 1), which halves
 // the LDS of a tile.
 //
-// Butterfly lanes take consecutive rows of one slot; pos() permutes the low
-// four row bits by bits 4 and 5 so that the 16 rows a quarter-wave touches in
-// any radix-4 or radix-2 step (consecutive rows, rows 4q+j, rows
-// 16a+b+4j, rows 2q) fall in 16 distinct bank groups.  Copy lanes take
-// consecutive slots of one row; the +1 row of padding keeps those distinct.
+// GF(2^16) butterfly lanes take consecutive rows of one slot; pos() permutes
+// the low four row bits by bits 4 and 5 so that the 16 rows a quarter-wave
+// touches in any radix-4 or radix-2 step (consecutive rows, rows 4q+j, rows
+// 16a+b+4j, rows 2q) fall in 16 distinct bank groups.  Copy lanes, and the
+// GF(2^8) butterfly lanes (kUnitsFastest), take consecutive slots of one
+// row; the +1 row of padding keeps those distinct.
 struct Tile {
     uint8_t *base;
     int r1;         // rows + 1
