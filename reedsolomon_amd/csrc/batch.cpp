@@ -136,6 +136,8 @@ struct rsx_batch {
     size_t leo_lds[3] = {0, 0, 0};
     size_t leo_deriv[3] = {0, 0, 0};  // largest register bytes (LeoItemPlan::regs) of the tasks
     uint32_t leo_slices[3] = {0, 0, 0};
+    // Workgroups per CU of the GF(2^8) Leopard launch (set per plan).
+    int leo8_per_cu = 8;
 
     // GF(2^16) reconstructs with 8192 or more work rows: three launches (one
     // per pass) over all such items, each with its two work arrays in d_mp_work.
@@ -276,7 +278,8 @@ int enqueue(rsx_batch *b, hipStream_t s) {
     }
     for (int fi = 0; fi < 3; ++fi) {
         if (b->leo_tasks[fi].empty()) continue;
-        const unsigned g = std::min<unsigned>(static_cast<unsigned>(std::max(1, b->cus * 8)), b->leo_slices[fi]);
+        const int per_cu = fi == 0 && fork ? b->leo8_per_cu : 8;
+        const unsigned g = std::min<unsigned>(static_cast<unsigned>(std::max(1, b->cus * per_cu)), b->leo_slices[fi]);
         RSX_HIP(leo_batch_launch(*b->leo[fi][0].dev, b->dev_leo_tasks[fi],
                                  static_cast<int>(b->leo_tasks[fi].size()), b->leo_slices[fi], b->leo_lds[fi], g,
                                  b->leo_deriv[fi], fi == 2, ls));
@@ -777,6 +780,20 @@ int rsx_batch_plan(rsx_batch *b, const rsx_batch_item *items, int n, int *status
         b->dev_leo_tasks[fi] = reinterpret_cast<const LeoBatchTask *>(dev + o_leo[fi]);
         if (!b->leo_tasks[fi].empty())
             std::memcpy(host + o_leo[fi], b->leo_tasks[fi].data(), b->leo_tasks[fi].size() * sizeof(LeoBatchTask));
+    }
+    // GF(2^8) Leopard work (VALU-bound) beside classic passes (HBM-bound) that
+    // move most of the plan's bytes: its launch is held to two workgroups per CU
+    // so the classic workgroups share every CU with it (one plan of a served
+    // request mix with 19% Leopard bytes: 4.98 -> 4.65 ms).  Otherwise it takes
+    // the full grid it needs alone (a Leopard-only plan: 2.0 ms, 2.94 held).
+    {
+        uint64_t classic = 0, leo8 = 0;
+        for (int mc = 1; mc <= 8; ++mc)
+            for (const auto &t : b->tasks[mc]) classic += t.len * (static_cast<uint64_t>(t.k) + t.nout);
+        for (const auto &t : b->leo_tasks[0])
+            leo8 += t.op == 0 ? t.enc.size * static_cast<uint64_t>(t.enc.k + t.enc.p)
+                              : t.dec.size * static_cast<uint64_t>(t.dec.n);
+        b->leo8_per_cu = leo8 && classic >= 3 * leo8 ? 2 : 8;
     }
     if (!b->leo_mp.empty() || !b->leo_emp.empty())
         if ((err = b->d_mp_work.reserve(b->mp_work_bytes))) return err;
